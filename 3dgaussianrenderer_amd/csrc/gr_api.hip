@@ -1,0 +1,444 @@
+// gr_api.hip -- host side of the C-ABI (include/gr_capi.h): contexts, timers,
+// the stage launchers of core/gaussian_render.cuh:113-126 and the object-level
+// renderer of core/renderer.{h,cpp}.  No entry point exits the process; every
+// failure returns a gr_status and leaves a message for gr_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gr_capi.h"
+#include "gr_internal.hpp"
+#include "gr_kernels.hpp"
+
+using namespace gr;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+gr_status fail(gr_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return st;
+}
+
+#define GR_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(e_ == hipErrorOutOfMemory ? GR_ERR_OUT_OF_MEMORY : GR_ERR_HIP,      \
+                        "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+#define GR_CHECK(st)                      \
+    do {                                  \
+        gr_status s_ = (st);              \
+        if (s_ != GR_OK) return s_;       \
+    } while (0)
+
+template <typename T>
+gr_status grow(T*& p, size_t& cap, size_t need, double headroom = 1.0) {
+    if (need <= cap && p != nullptr) return GR_OK;
+    size_t n = (size_t)((double)need * headroom);
+    if (n < need) n = need;
+    if (n == 0) n = 1;
+    if (p) GR_HIP(hipFree(p));
+    p = nullptr;
+    cap = 0;
+    GR_HIP(hipMalloc((void**)&p, n * sizeof(T)));
+    cap = n;
+    return GR_OK;
+}
+
+CameraArgs camera_args(const gr_camera_data& c) {
+    CameraArgs a;
+    a.view = load_m4(c.view);
+    a.fov_cotangent = v2{c.fov_cotangent[0], c.fov_cotangent[1]};
+    a.depth_scale_bias = v2{c.depth_scale_bias[0], c.depth_scale_bias[1]};
+    a.position = v3{c.position[0], c.position[1], c.position[2]};
+    a.aspect = c.aspect;
+    return a;
+}
+
+}  // namespace
+
+namespace gr_detail {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace gr_detail
+
+static gr_status timer_start(gr_context* ctx, gr_timer* t) {
+    if (!t) return GR_OK;
+    GR_HIP(hipEventRecord(t->start, ctx->stream));
+    return GR_OK;
+}
+static gr_status timer_stop(gr_context* ctx, gr_timer* t) {
+    if (!t) return GR_OK;
+    GR_HIP(hipEventRecord(t->stop, ctx->stream));
+    t->recorded = true;
+    return GR_OK;
+}
+
+static FrameGeom ctx_geom(const gr_context* ctx) {
+    return make_geom(ctx->args.width, ctx->args.height);
+}
+
+extern "C" {
+
+const char* gr_last_error(void) { return g_last_error.c_str(); }
+const char* gr_version(void) { return "gr-mi355x 0.1 (gfx950)"; }
+
+gr_status gr_context_create(gr_context** out, void* stream) {
+    if (!out) return fail(GR_ERR_INVALID_ARGUMENT, "gr_context_create: out is NULL");
+    auto* c = new gr_context();
+    if (stream) {
+        c->stream = static_cast<hipStream_t>(stream);
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            return fail(GR_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+        }
+        c->own_stream = true;
+    }
+    hipError_t e = hipHostMalloc((void**)&c->pinned, 64, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(GR_ERR_HIP, "hipHostMalloc failed: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return GR_OK;
+}
+
+gr_status gr_context_destroy(gr_context* ctx) {
+    if (!ctx) return GR_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->batch_counts) (void)hipFree(ctx->batch_counts);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GR_OK;
+}
+
+gr_status gr_context_synchronize(gr_context* ctx) {
+    if (!ctx) return fail(GR_ERR_INVALID_ARGUMENT, "context is NULL");
+    GR_HIP(hipStreamSynchronize(ctx->stream));
+    return GR_OK;
+}
+
+gr_status gr_timer_create(gr_timer** out) {
+    if (!out) return fail(GR_ERR_INVALID_ARGUMENT, "gr_timer_create: out is NULL");
+    auto* t = new gr_timer();
+    if (hipEventCreate(&t->start) != hipSuccess || hipEventCreate(&t->stop) != hipSuccess) {
+        delete t;
+        return fail(GR_ERR_HIP, "hipEventCreate failed");
+    }
+    *out = t;
+    return GR_OK;
+}
+
+gr_status gr_timer_destroy(gr_timer* t) {
+    if (!t) return GR_OK;
+    (void)hipEventDestroy(t->start);
+    (void)hipEventDestroy(t->stop);
+    delete t;
+    return GR_OK;
+}
+
+gr_status gr_timer_elapsed_ms(gr_timer* t, float* out_ms) {
+    if (!t || !out_ms) return fail(GR_ERR_INVALID_ARGUMENT, "gr_timer_elapsed_ms: NULL argument");
+    if (!t->recorded) return fail(GR_ERR_INVALID_ARGUMENT, "timer was never stopped");
+    GR_HIP(hipEventSynchronize(t->stop));
+    GR_HIP(hipEventElapsedTime(out_ms, t->start, t->stop));
+    return GR_OK;
+}
+
+gr_status gr_set_global_args(gr_context* ctx, const gr_global_args* args) {
+    if (!ctx || !args) return fail(GR_ERR_INVALID_ARGUMENT, "gr_set_global_args: NULL argument");
+    if (args->width <= 0 || args->height <= 0)
+        return fail(GR_ERR_INVALID_ARGUMENT, "frame size %dx%d is invalid", args->width, args->height);
+    if (args->spherical_harmonics_degree < 0 || args->spherical_harmonics_degree > 3)
+        return fail(GR_ERR_INVALID_ARGUMENT, "SH degree %d is outside 0..3",
+                    args->spherical_harmonics_degree);
+    const int want = 3 * (args->spherical_harmonics_degree + 1) * (args->spherical_harmonics_degree + 1);
+    if (args->spherical_harmonics_count != want)
+        return fail(GR_ERR_INVALID_ARGUMENT, "spherical_harmonics_count %d != 3(d+1)^2 = %d",
+                    args->spherical_harmonics_count, want);
+    ctx->args = *args;
+    ctx->has_args = true;
+    return GR_OK;
+}
+
+gr_status gr_set_tile_list_args(gr_context* ctx, const gr_tile_list_args* args) {
+    if (!ctx || !args) return fail(GR_ERR_INVALID_ARGUMENT, "gr_set_tile_list_args: NULL argument");
+    ctx->tl = *args;
+    ctx->has_tl = true;
+    return GR_OK;
+}
+
+gr_status gr_evaluate_spherical_harmonics(gr_context* ctx, gr_timer* timer, int32_t count) {
+    if (!ctx || !ctx->has_args) return fail(GR_ERR_INVALID_ARGUMENT, "global args not set");
+    const gr_global_args& g = ctx->args;
+    if (count < 0 || count > g.splat_count) return fail(GR_ERR_INVALID_ARGUMENT, "bad count %d", count);
+    if (count > 0 && (!g.position || !g.color || !g.spherical_harmonics))
+        return fail(GR_ERR_INVALID_ARGUMENT, "SH stage needs position, color and SH buffers");
+    ShArgs a;
+    a.position = reinterpret_cast<const float4*>(g.position);
+    a.sh = g.spherical_harmonics;
+    a.color = reinterpret_cast<float4*>(g.color);
+    a.camera_position = v3{g.camera_data.position[0], g.camera_data.position[1], g.camera_data.position[2]};
+    a.count = count;
+    a.sh_count = g.spherical_harmonics_count;
+    GR_CHECK(timer_start(ctx, timer));
+    GR_HIP(launch_sh(a, g.spherical_harmonics_degree, g.sh_precision, ctx->stream));
+    GR_CHECK(timer_stop(ctx, timer));
+    return GR_OK;
+}
+
+gr_status gr_evaluate_splat_clip_data(gr_context* ctx, gr_timer* timer, int32_t count) {
+    if (!ctx || !ctx->has_args) return fail(GR_ERR_INVALID_ARGUMENT, "global args not set");
+    const gr_global_args& g = ctx->args;
+    if (count < 0 || count > g.splat_count) return fail(GR_ERR_INVALID_ARGUMENT, "bad count %d", count);
+    if (count > 0 && (!g.position || !g.scale_and_rotation || !g.position_clip_xy ||
+                      !g.position_clip_z || !g.screen_ellipse || !g.conic))
+        return fail(GR_ERR_INVALID_ARGUMENT, "projection stage is missing a buffer");
+    ProjectArgs a;
+    a.position = reinterpret_cast<const float4*>(g.position);
+    a.scale_rotation = reinterpret_cast<const float4*>(g.scale_and_rotation);
+    a.clip_xy = reinterpret_cast<float2*>(g.position_clip_xy);
+    a.clip_z = g.position_clip_z;
+    a.ellipse = reinterpret_cast<float4*>(g.screen_ellipse);
+    a.conic = reinterpret_cast<float4*>(g.conic);
+    a.cam = camera_args(g.camera_data);
+    a.geom = ctx_geom(ctx);
+    a.count = count;
+    GR_CHECK(timer_start(ctx, timer));
+    GR_HIP(launch_project(a, ctx->stream));
+    GR_CHECK(timer_stop(ctx, timer));
+    return GR_OK;
+}
+
+gr_status gr_build_tile_list(gr_context* ctx, gr_timer* timer, int32_t num_blocks,
+                             int32_t tile_list_capacity, int32_t* out_size, int64_t* out_required) {
+    (void)num_blocks;
+    if (!ctx || !ctx->has_args || !ctx->has_tl)
+        return fail(GR_ERR_INVALID_ARGUMENT, "global/tile-list args not set");
+    if (tile_list_capacity < 0) return fail(GR_ERR_INVALID_ARGUMENT, "negative capacity");
+    const gr_global_args& g = ctx->args;
+    const int n = g.splat_count;
+    const int batches = (n + kBuildBatch - 1) / kBuildBatch;
+    GR_CHECK(grow(ctx->batch_counts, ctx->batch_cap, (size_t)batches + 1));
+    BuildArgs a;
+    a.clip_xy = reinterpret_cast<const float2*>(g.position_clip_xy);
+    a.clip_z = g.position_clip_z;
+    a.ellipse = reinterpret_cast<const float4*>(g.screen_ellipse);
+    a.batch_counts = ctx->batch_counts;
+    a.keys = ctx->tl.keys;
+    a.values = ctx->tl.values;
+    a.geom = ctx_geom(ctx);
+    a.count = n;
+    a.capacity = tile_list_capacity;
+    if (a.geom.tiles_x * a.geom.tiles_y >= (1 << 20))
+        return fail(GR_ERR_INVALID_ARGUMENT, "frame has too many tiles");
+    GR_CHECK(timer_start(ctx, timer));
+    uint32_t total = 0;
+    if (n > 0) {
+        GR_HIP(launch_build_count(a, ctx->stream));
+        GR_HIP(launch_build_scan(ctx->batch_counts, batches, ctx->batch_counts + batches, ctx->stream));
+        GR_HIP(hipMemcpyAsync(ctx->pinned, ctx->batch_counts + batches, sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+        GR_HIP(hipStreamSynchronize(ctx->stream));
+        total = ctx->pinned[0];
+        if (total > 0 && tile_list_capacity > 0) {
+            if (!a.keys || !a.values) return fail(GR_ERR_INVALID_ARGUMENT, "tile-list buffers are NULL");
+            GR_HIP(launch_build_emit(a, ctx->stream));
+        }
+    }
+    GR_CHECK(timer_stop(ctx, timer));
+    const int64_t k = (int64_t)total;
+    if (out_required) *out_required = k;
+    if (out_size) *out_size = (int32_t)std::min<int64_t>(k, tile_list_capacity);
+    return GR_OK;
+}
+
+gr_status gr_device_free(void* p) {
+    if (p) GR_HIP(hipFree(p));
+    return GR_OK;
+}
+
+gr_status gr_radix_sort_pairs(gr_context* ctx, int32_t count, int32_t begin_bit, int32_t end_bit,
+                              void** temp, size_t* temp_bytes, gr_double_buffer_u64* keys,
+                              gr_double_buffer_i32* values) {
+    if (!ctx || !temp || !temp_bytes || !keys || !values)
+        return fail(GR_ERR_INVALID_ARGUMENT, "gr_radix_sort_pairs: NULL argument");
+    if (count < 0 || count >= (1 << 30)) return fail(GR_ERR_INVALID_ARGUMENT, "count %d out of range", count);
+    if (begin_bit < 0 || end_bit > 64 || end_bit - begin_bit > 64)
+        return fail(GR_ERR_INVALID_ARGUMENT, "bad bit range [%d, %d)", begin_bit, end_bit);
+    const size_t need = radix_sort_temp_bytes(count, begin_bit, end_bit);
+    if (need > *temp_bytes || *temp == nullptr) {
+        if (*temp) GR_HIP(hipFree(*temp));
+        *temp = nullptr;
+        *temp_bytes = 0;
+        GR_HIP(hipMalloc(temp, need));
+        *temp_bytes = need;
+    }
+    uint64_t* kb[2] = {keys->buffers[0], keys->buffers[1]};
+    int32_t* vb[2] = {values->buffers[0], values->buffers[1]};
+    int32_t sel = keys->selector;
+    if (values->selector != sel) return fail(GR_ERR_INVALID_ARGUMENT, "key/value selectors differ");
+    GR_HIP(radix_sort_pairs(kb, vb, &sel, count, begin_bit, end_bit, *temp, *temp_bytes, ctx->stream));
+    keys->selector = sel;
+    values->selector = sel;
+    return GR_OK;
+}
+
+gr_status gr_sort_tile_list(gr_context* ctx, gr_timer* timer, int32_t size, void** temp,
+                            size_t* temp_bytes, gr_double_buffer_u64* keys, gr_double_buffer_i32* values) {
+    if (!ctx || !ctx->has_args) return fail(GR_ERR_INVALID_ARGUMENT, "global args not set");
+    const FrameGeom g = ctx_geom(ctx);
+    const int end_bit = 32 + tile_bits_for(g.tiles_x * g.tiles_y);
+    GR_CHECK(timer_start(ctx, timer));
+    GR_CHECK(gr_radix_sort_pairs(ctx, size, 0, end_bit, temp, temp_bytes, keys, values));
+    GR_CHECK(timer_stop(ctx, timer));
+    return GR_OK;
+}
+
+gr_status gr_evaluate_tile_range(gr_context* ctx, gr_timer* timer, int32_t size) {
+    if (!ctx || !ctx->has_args || !ctx->has_tl)
+        return fail(GR_ERR_INVALID_ARGUMENT, "global/tile-list args not set");
+    const FrameGeom g = ctx_geom(ctx);
+    const int tiles = g.tiles_x * g.tiles_y;
+    if (!ctx->args.tile_range) return fail(GR_ERR_INVALID_ARGUMENT, "tile_range buffer is NULL");
+    GR_CHECK(timer_start(ctx, timer));
+    GR_HIP(launch_fill_i32(ctx->args.tile_range, -1, 2 * (int64_t)tiles, ctx->stream));
+    if (size > 0) {
+        RangeArgs a;
+        a.keys = ctx->tl.keys;
+        a.tile_range = ctx->args.tile_range;
+        a.size = size;
+        a.tiles = tiles;
+        GR_HIP(launch_ranges(a, ctx->stream));
+    }
+    GR_CHECK(timer_stop(ctx, timer));
+    return GR_OK;
+}
+
+gr_status gr_rasterize_tile(gr_context* ctx, gr_timer* timer) {
+    if (!ctx || !ctx->has_args || !ctx->has_tl)
+        return fail(GR_ERR_INVALID_ARGUMENT, "global/tile-list args not set");
+    const gr_global_args& g = ctx->args;
+    if (!g.back_buffer || !g.tile_range) return fail(GR_ERR_INVALID_ARGUMENT, "raster buffers are NULL");
+    RasterArgs a;
+    a.geom = ctx_geom(ctx);
+    a.tiles = a.geom.tiles_x * a.geom.tiles_y;
+    a.tiles_per_xcd = (a.tiles + 7) / 8;
+    a.tile_range = g.tile_range;
+    a.values = ctx->tl.values;
+    a.clip_xy = reinterpret_cast<const float2*>(g.position_clip_xy);
+    a.conic = reinterpret_cast<const float4*>(g.conic);
+    a.color = reinterpret_cast<const float4*>(g.color);
+    a.out = reinterpret_cast<uchar4*>(g.back_buffer);
+    GR_CHECK(timer_start(ctx, timer));
+    GR_HIP(launch_raster(a, ctx->stream));
+    GR_CHECK(timer_stop(ctx, timer));
+    return GR_OK;
+}
+
+// ---- host helpers --------------------------------------------------------
+
+gr_status gr_realign_spherical_harmonics(const float* src, float* dst, int32_t group_size,
+                                         int32_t sh_count, int32_t splat_count) {
+    if (!src || !dst || group_size <= 0 || sh_count <= 0 || splat_count < 0 || sh_count % 3 != 0)
+        return fail(GR_ERR_INVALID_ARGUMENT, "gr_realign_spherical_harmonics: bad argument");
+    const int groups = (splat_count + group_size - 1) / group_size;
+    for (int grp = 0; grp < groups; ++grp) {
+        const size_t start = (size_t)grp * group_size * sh_count;
+        const int here = std::min(group_size, splat_count - grp * group_size);
+        for (int i = 0; i < sh_count; ++i) {
+            float* d = dst + start + (size_t)group_size * i;
+            for (int k = 0; k < here; ++k) d[k] = src[start + (size_t)sh_count * k + i];
+            for (int k = here; k < group_size; ++k) d[k] = 0.0f;
+        }
+    }
+    return GR_OK;
+}
+
+uint32_t gr_encode_vec4(float x, float y, float z, float w) {
+    const float v[4] = {x, y, z, w};
+    uint32_t r = 0;
+    for (int i = 0; i < 4; ++i) {
+        const float c = gclamp(v[i], 0.0f, 1.0f);
+        r |= (f2u_sat(c * 255.0f) & 0xffu) << (24 - 8 * i);
+    }
+    return r;
+}
+
+gr_status gr_tile_grid(int32_t width, int32_t height, int32_t* tiles_x, int32_t* tiles_y,
+                       int32_t* sort_end_bit) {
+    if (width <= 0 || height <= 0) return fail(GR_ERR_INVALID_ARGUMENT, "bad frame size");
+    const FrameGeom g = make_geom(width, height);
+    if (tiles_x) *tiles_x = g.tiles_x;
+    if (tiles_y) *tiles_y = g.tiles_y;
+    if (sort_end_bit) *sort_end_bit = 32 + tile_bits_for(g.tiles_x * g.tiles_y);
+    return GR_OK;
+}
+
+// Camera: core/camera.cpp:7-55 (glm::lookAt = lookAtRH, ext/matrix_transform.inl:99-119;
+// glm::perspective = perspectiveRH_NO, ext/matrix_clip_space.inl:249-262).
+gr_status gr_camera_make(const float eye[3], const float center[3], float fov_y, int32_t width,
+                         int32_t height, float z_near, float z_far, gr_camera_data* out) {
+    if (!eye || !center || !out || width <= 0 || height <= 0 || !(z_far > z_near))
+        return fail(GR_ERR_INVALID_ARGUMENT, "gr_camera_make: bad argument");
+    const v3 e{eye[0], eye[1], eye[2]}, c{center[0], center[1], center[2]}, up{0.0f, 1.0f, 0.0f};
+    const v3 f = normalize3(v3{c.x - e.x, c.y - e.y, c.z - e.z});
+    const v3 s = normalize3(cross3(f, up));
+    const v3 u = cross3(s, f);
+    float V[16] = {0};
+    V[0] = s.x; V[4] = s.y; V[8] = s.z;
+    V[1] = u.x; V[5] = u.y; V[9] = u.z;
+    V[2] = -f.x; V[6] = -f.y; V[10] = -f.z;
+    V[12] = -dot3(s, e);
+    V[13] = -dot3(u, e);
+    V[14] = dot3(f, e);
+    V[15] = 1.0f;
+    const float aspect = (float)width / (float)height;
+    const float tan_half = std::tan(fov_y / 2.0f);
+    float P[16] = {0};
+    P[0] = 1.0f / (aspect * tan_half);
+    P[5] = 1.0f / tan_half;
+    P[10] = -(z_far + z_near) / (z_far - z_near);
+    P[11] = -1.0f;
+    P[14] = -(2.0f * z_far * z_near) / (z_far - z_near);
+    // view_projection = P * V (glm mat4 product: column c = P * V[c])
+    float VP[16];
+    const m4 pm = load_m4(P);
+    for (int col = 0; col < 4; ++col) {
+        const v4 r = mul44v(pm, v4{V[4 * col], V[4 * col + 1], V[4 * col + 2], V[4 * col + 3]});
+        VP[4 * col] = r.x; VP[4 * col + 1] = r.y; VP[4 * col + 2] = r.z; VP[4 * col + 3] = r.w;
+    }
+    std::memcpy(out->view, V, sizeof(V));
+    std::memcpy(out->projection, P, sizeof(P));
+    std::memcpy(out->view_projection, VP, sizeof(VP));
+    const float cot_y = 1.0f / std::tan(fov_y * 0.5f);
+    out->fov_cotangent[0] = cot_y / aspect;
+    out->fov_cotangent[1] = cot_y;
+    out->depth_scale_bias[0] = -2.0f / (z_far - z_near);
+    out->depth_scale_bias[1] = -(z_far + z_near) / (z_far - z_near);
+    out->position[0] = eye[0];
+    out->position[1] = eye[1];
+    out->position[2] = eye[2];
+    out->aspect = aspect;
+    return GR_OK;
+}
+
+}  // extern "C"

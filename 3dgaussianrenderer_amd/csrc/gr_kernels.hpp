@@ -1,0 +1,100 @@
+// gr_kernels.hpp -- kernel argument blocks and launcher declarations shared by
+// the stage kernels (gr_stages.hip), the radix sort (gr_sort.hip) and the host
+// side (gr_api.hip).  Arguments travel by value in the kernarg segment; there
+// is no __constant__ state, so any number of contexts may run concurrently.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gr_math.hpp"
+
+namespace gr {
+
+constexpr int kShGroup = 256;   // Gaussians per SH group (app/main.cpp:278)
+
+struct CameraArgs {
+    m4 view;
+    v2 fov_cotangent;
+    v2 depth_scale_bias;
+    v3 position;
+    float aspect;
+};
+
+struct ShArgs {
+    const float4* position;
+    const void* sh;          // aligned [groups][sh_count][256]
+    float4* color;
+    v3 camera_position;
+    int32_t count;
+    int32_t sh_count;        // coefficients * 3
+};
+
+struct ProjectArgs {
+    const float4* position;
+    const float4* scale_rotation;
+    float2* clip_xy;
+    float* clip_z;
+    float4* ellipse;
+    float4* conic;
+    CameraArgs cam;
+    FrameGeom geom;
+    int32_t count;
+};
+
+struct BuildArgs {
+    const float2* clip_xy;
+    const float* clip_z;
+    const float4* ellipse;
+    uint32_t* batch_counts;      // [batches]  (count pass) / exclusive offsets (emit pass)
+    uint64_t* keys;
+    int32_t* values;
+    FrameGeom geom;
+    int32_t count;
+    int32_t capacity;
+};
+
+struct RangeArgs {
+    const uint64_t* keys;
+    int32_t* tile_range;
+    int32_t size;
+    int32_t tiles;
+};
+
+struct RasterArgs {
+    const int32_t* tile_range;
+    const int32_t* values;
+    const float2* clip_xy;
+    const float4* conic;
+    const float4* color;
+    uchar4* out;
+    FrameGeom geom;
+    int32_t tiles;
+    int32_t tiles_per_xcd;
+};
+
+// Gaussians per build batch: one wave64 walks the candidate tiles of 64
+// consecutive Gaussians.
+constexpr int kBuildBatch = 64;
+constexpr int kBuildThreads = 256;
+
+// launchers (all asynchronous on `stream`)
+hipError_t launch_sh(const ShArgs& a, int degree, int precision, hipStream_t stream);
+hipError_t launch_project(const ProjectArgs& a, hipStream_t stream);
+hipError_t launch_build_count(const BuildArgs& a, hipStream_t stream);
+hipError_t launch_build_scan(uint32_t* batch_counts, int32_t batches, uint32_t* total,
+                             hipStream_t stream);
+hipError_t launch_build_emit(const BuildArgs& a, hipStream_t stream);
+hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t stream);
+hipError_t launch_ranges(const RangeArgs& a, hipStream_t stream);
+hipError_t launch_raster(const RasterArgs& a, hipStream_t stream);
+
+// radix sort (gr_sort.hip)
+size_t radix_sort_temp_bytes(int32_t count, int32_t begin_bit, int32_t end_bit);
+// Sorts (keys[sel], values[sel]) on bits [begin_bit, end_bit); on return *sel
+// names the buffer pair that holds the sorted data.
+hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                            int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes,
+                            hipStream_t stream);
+
+}  // namespace gr

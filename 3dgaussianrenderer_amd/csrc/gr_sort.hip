@@ -1,0 +1,281 @@
+// gr_sort.hip -- stable LSD radix sort of (uint64 key, int32 value) pairs for
+// gfx950, replacing cub::DeviceRadixSort::SortPairs (core/gaussian_render.cu:591-632).
+//
+// Onesweep structure (Adinets & Merrill 2022, "Onesweep: A Faster Least
+// Significant Digit Radix Sort for GPUs"), written for wave64:
+//   init      zero the digit histograms, the partition tickets and the first
+//             look-back region
+//   histogram one read of all keys; 256-bin histograms of every 8-bit digit
+//             (LDS-privatised, then global atomics)
+//   scan      exclusive scan of each digit histogram -> global digit offsets
+//   pass x P  each workgroup takes a partition ticket (in-order progress for
+//             the look-back), loads kTile keys, ranks them per wave with
+//             64-bit ballots (peer mask of equal digits, popcount of lower
+//             lanes), publishes its per-digit counts, computes its global
+//             per-digit prefix by decoupled look-back (Merrill & Garland 2016)
+//             over the predecessors' status words, reorders the tile in LDS by
+//             digit and writes it out digit-run by digit-run (coalesced).
+// Stability: within a partition keys are ranked in their input order (wave
+// by wave, item row by item row, lane by lane); partitions are ordered by
+// ticket, so equal digits keep input order.  Status words are 32-bit:
+// bits 30..31 flag (0 empty, 1 aggregate, 2 inclusive prefix), bits 0..29
+// count, so one sort handles fewer than 2^30 pairs.
+#include <hip/hip_runtime.h>
+
+#include "gr_kernels.hpp"
+
+namespace gr {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kRadix = 256;
+constexpr int kMaxPasses = 8;
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagIncl = 2u << 30;
+constexpr uint32_t kCountMask = (1u << 30) - 1u;
+
+struct SortLayout {
+    size_t hist_off, ticket_off, status_off[2], total;
+    int partitions;
+};
+
+static SortLayout sort_layout(int32_t count) {
+    SortLayout l;
+    l.partitions = (count + kSortTile - 1) / kSortTile;
+    size_t off = 0;
+    l.hist_off = off;
+    off += sizeof(uint32_t) * kMaxPasses * kRadix;
+    l.ticket_off = off;
+    off += 256;
+    off = (off + 255) & ~(size_t)255;
+    const size_t region = (size_t)l.partitions * kRadix * sizeof(uint32_t);
+    l.status_off[0] = off;
+    off += (region + 255) & ~(size_t)255;
+    l.status_off[1] = off;
+    off += (region + 255) & ~(size_t)255;
+    l.total = off;
+    return l;
+}
+
+size_t radix_sort_temp_bytes(int32_t count, int32_t, int32_t) {
+    return sort_layout(count < 1 ? 1 : count).total;
+}
+
+__global__ void sort_init_kernel(uint32_t* hist, uint32_t* tickets, uint32_t* status0, int64_t status_n) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid < kMaxPasses * kRadix) hist[tid] = 0u;
+    if (tid < kMaxPasses) tickets[tid] = 0u;
+    for (int64_t i = tid; i < status_n; i += (int64_t)gridDim.x * blockDim.x) status0[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void sort_hist_kernel(const uint64_t* __restrict__ keys, int32_t n,
+                                                        int begin_bit, int end_bit, int passes,
+                                                        uint32_t* hist) {
+    __shared__ uint32_t s_h[kMaxPasses][kRadix];
+    for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += blockDim.x) (&s_h[0][0])[i] = 0u;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        for (int p = 0; p < passes; ++p) {
+            const int shift = begin_bit + 8 * p;
+            const int bits = min(8, end_bit - shift);
+            const uint32_t d = (uint32_t)(k >> shift) & ((1u << bits) - 1u);
+            atomicAdd(&s_h[p][d], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < passes * kRadix; i += blockDim.x) {
+        const uint32_t v = (&s_h[0][0])[i];
+        if (v) atomicAdd(&hist[i], v);
+    }
+}
+
+// exclusive scan of each pass's 256 bins; one workgroup of 256 lanes per pass
+__global__ __launch_bounds__(256) void sort_hist_scan_kernel(uint32_t* hist) {
+    __shared__ uint32_t s_w[4];
+    uint32_t* h = hist + blockIdx.x * kRadix;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t v = h[t];
+    uint32_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) s_w[w] = incl;
+    __syncthreads();
+    uint32_t off = 0;
+    for (int k = 0; k < w; ++k) off += s_w[k];
+    h[t] = off + incl - v;
+}
+
+__device__ __forceinline__ uint32_t ld_status(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
+    const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+    int32_t* __restrict__ vout, int32_t n, int shift, int bits, const uint32_t* __restrict__ digit_off,
+    uint32_t* status, uint32_t* status_next, uint32_t* ticket) {
+    __shared__ uint32_t s_part;
+    __shared__ uint32_t s_wcnt[kSortWaves][kRadix];
+    __shared__ uint32_t s_dstart[kRadix];
+    __shared__ uint32_t s_gbase[kRadix];
+    __shared__ uint32_t s_wsum[kSortWaves];
+    __shared__ uint64_t s_keys[kSortTile];
+    __shared__ int32_t s_vals[kSortTile];
+
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) s_part = atomicAdd(ticket, 1u);
+    for (int i = t; i < kSortWaves * kRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0u;
+    __syncthreads();
+    const int part = (int)s_part;
+    if (status_next) status_next[(size_t)part * kRadix + t] = 0u;
+
+    const int64_t base = (int64_t)part * kSortTile;
+    const uint32_t dmask = (1u << bits) - 1u;
+    uint64_t key[kSortItems];
+    int32_t val[kSortItems];
+    uint32_t rank[kSortItems];
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int64_t idx = base + (int64_t)w * (64 * kSortItems) + i * 64 + lane;
+        if (idx < n) {
+            key[i] = kin[idx];
+            val[i] = vin[idx];
+        } else {
+            key[i] = 0;
+            val[i] = 0;
+        }
+    }
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int64_t idx = base + (int64_t)w * (64 * kSortItems) + i * 64 + lane;
+        const bool valid = idx < n;
+        const uint32_t d = (uint32_t)(key[i] >> shift) & dmask;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < bits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        const uint32_t below = (uint32_t)__popcll(peers & lt);
+        const uint32_t prev = s_wcnt[w][d];
+        rank[i] = prev + below;
+        if (valid && below == 0u) s_wcnt[w][d] = prev + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+
+    // per digit: wave prefix and partition total (thread t owns digit t)
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < kSortWaves; ++k) {
+        const uint32_t c = s_wcnt[k][t];
+        s_wcnt[k][t] = run;
+        run += c;
+    }
+    const uint32_t total = run;
+
+    // decoupled look-back for the global prefix of digit t
+    uint32_t* my = status + (size_t)part * kRadix + t;
+    uint32_t excl = 0;
+    if (part == 0) {
+        st_status(my, kFlagIncl | total);
+    } else {
+        st_status(my, kFlagAgg | total);
+        int j = part - 1;
+        while (true) {
+            const uint32_t s = ld_status(status + (size_t)j * kRadix + t);
+            const uint32_t flag = s & ~kCountMask;
+            if (flag == 0u) continue;
+            excl += s & kCountMask;
+            if (flag == kFlagIncl) break;
+            --j;
+        }
+        st_status(my, kFlagIncl | (excl + total));
+    }
+
+    // exclusive scan of the partition totals over digits
+    uint32_t incl = total;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) s_wsum[w] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (int k = 0; k < w; ++k) woff += s_wsum[k];
+    const uint32_t dstart = woff + incl - total;
+    s_dstart[t] = dstart;
+    s_gbase[t] = digit_off[t] + excl - dstart;
+    __syncthreads();
+
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int64_t idx = base + (int64_t)w * (64 * kSortItems) + i * 64 + lane;
+        if (idx < n) {
+            const uint32_t d = (uint32_t)(key[i] >> shift) & dmask;
+            const uint32_t lp = s_dstart[d] + s_wcnt[w][d] + rank[i];
+            s_keys[lp] = key[i];
+            s_vals[lp] = val[i];
+        }
+    }
+    __syncthreads();
+    const int nvalid = (int)min((int64_t)kSortTile, (int64_t)n - base);
+    for (int j = t; j < nvalid; j += kSortThreads) {
+        const uint64_t k = s_keys[j];
+        const uint32_t d = (uint32_t)(k >> shift) & dmask;
+        const uint32_t gpos = s_gbase[d] + (uint32_t)j;
+        kout[gpos] = k;
+        vout[gpos] = s_vals[j];
+    }
+}
+
+hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                            int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes,
+                            hipStream_t s) {
+    if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
+    if ((uint32_t)count > kCountMask || begin_bit < 0 || end_bit > 64) return hipErrorInvalidValue;
+    const int passes = (end_bit - begin_bit + 7) / 8;
+    if (passes > kMaxPasses) return hipErrorInvalidValue;
+    const SortLayout l = sort_layout(count);
+    if (temp_bytes < l.total || temp == nullptr) return hipErrorInvalidValue;
+    char* tb = static_cast<char*>(temp);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(tb + l.hist_off);
+    uint32_t* tickets = reinterpret_cast<uint32_t*>(tb + l.ticket_off);
+    uint32_t* status[2] = {reinterpret_cast<uint32_t*>(tb + l.status_off[0]),
+                           reinterpret_cast<uint32_t*>(tb + l.status_off[1])};
+    const int64_t status_n = (int64_t)l.partitions * kRadix;
+
+    int64_t init_blocks = (status_n + 255) / 256;
+    if (init_blocks < (kMaxPasses * kRadix + 255) / 256) init_blocks = (kMaxPasses * kRadix + 255) / 256;
+    if (init_blocks > 4096) init_blocks = 4096;
+    hipLaunchKernelGGL(sort_init_kernel, dim3((unsigned)init_blocks), dim3(256), 0, s, hist, tickets,
+                       status[0], status_n);
+    int hist_blocks = (count + 256 * 16 - 1) / (256 * 16);
+    if (hist_blocks > 2048) hist_blocks = 2048;
+    hipLaunchKernelGGL(sort_hist_kernel, dim3(hist_blocks), dim3(256), 0, s, keys[*sel], count,
+                       begin_bit, end_bit, passes, hist);
+    hipLaunchKernelGGL(sort_hist_scan_kernel, dim3(passes), dim3(256), 0, s, hist);
+    for (int p = 0; p < passes; ++p) {
+        const int shift = begin_bit + 8 * p;
+        const int bits = min(8, end_bit - shift);
+        const int cur = *sel;
+        hipLaunchKernelGGL(onesweep_kernel, dim3(l.partitions), dim3(kSortThreads), 0, s, keys[cur],
+                           values[cur], keys[cur ^ 1], values[cur ^ 1], count, shift, bits,
+                           hist + p * kRadix, status[p & 1], (p + 1 < passes) ? status[(p + 1) & 1] : nullptr,
+                           tickets + p);
+        *sel = cur ^ 1;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace gr

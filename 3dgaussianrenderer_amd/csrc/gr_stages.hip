@@ -1,0 +1,596 @@
+// gr_stages.hip -- per-frame stage kernels for gfx950 (MI355X, wave64).
+//
+//   SH colour      one lane per Gaussian, coefficients read from the aligned
+//                  [group][coef][256] layout so that each wave load is 256
+//                  contiguous bytes (HBM-bound).
+//   projection     one lane per Gaussian, float4 SoA loads (HBM-bound).
+//   build          deterministic count -> scan -> emit; one wave walks the
+//                  candidate tiles of 64 Gaussians, 64 candidates at a time,
+//                  and emits by ballot/popcount so no atomics decide order.
+//   ranges         per-entry boundary test on sorted keys.
+//   raster         one 256-lane workgroup per 16x16 tile (four 8x8 wave
+//                  quadrants), entries staged through LDS 256 at a time,
+//                  front-to-back blending, tile-wide stop vote every 32
+//                  entries (the reference's rule).
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "gr_kernels.hpp"
+
+namespace gr {
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (1ull << lane) - 1ull; }
+
+// ---------------------------------------------------------------------------
+// SH colour: core/gaussian_render.cu:50-116 (bands 0-2), band 3 from
+// core/cuda_renderer.cu:119-157.  Sums are taken band by band, each band's
+// terms left to right, then added to the running colour, as the reference
+// source is written.
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ float sh_ld(const T* p);
+template <> __device__ __forceinline__ float sh_ld<float>(const float* p) {
+    return __builtin_nontemporal_load(p);
+}
+template <> __device__ __forceinline__ float sh_ld<__half>(const __half* p) {
+    return __half2float(*p);
+}
+template <> __device__ __forceinline__ float sh_ld<uint16_t>(const uint16_t* p) {
+    // bf16 stored as its upper 16 bits
+    return __uint_as_float(((uint32_t)*p) << 16);
+}
+
+template <int DEG, typename T>
+__global__ __launch_bounds__(256) void sh_kernel(ShArgs a) {
+    const int i = blockIdx.x * kShGroup + threadIdx.x;
+    if (i >= a.count) return;
+    const float4 p = a.position[i];
+    const v3 dir = normalize3(v3{a.camera_position.x - p.x, a.camera_position.y - p.y,
+                                 a.camera_position.z - p.z});
+    const float x = dir.x, y = dir.y, z = dir.z;
+    const T* sh = static_cast<const T*>(a.sh) + (size_t)blockIdx.x * kShGroup * a.sh_count + threadIdx.x;
+    auto c = [&](int k) { return sh_ld<T>(sh + (size_t)k * kShGroup); };
+
+    float res[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) res[ch] = 0.0f + 0.282094792f * c(ch);
+
+    if (DEG > 0) {
+        const float k1 = 0.488602512f * y, k2 = 0.488602512f * z, k3 = 0.488602512f * x;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const float t = (k1 * c(3 + ch) + k2 * c(6 + ch)) + k3 * c(9 + ch);
+            res[ch] = res[ch] + t;
+        }
+        if (DEG > 1) {
+            const float xx = x * x, yy = y * y;
+            const float k4 = (1.09254843f * x) * y;
+            const float k5 = (1.09254843f * y) * z;
+            const float k6 = ((-0.946174696f * xx) - 0.946174696f * yy) + 0.630783131f;
+            const float k7 = (1.09254843f * x) * z;
+            const float k8 = (0.546274215f * (x - y)) * (x + y);
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                const float t = (((k4 * c(12 + ch) + k5 * c(15 + ch)) + k6 * c(18 + ch)) +
+                                 k7 * c(21 + ch)) + k8 * c(24 + ch);
+                res[ch] = res[ch] + t;
+            }
+            if (DEG > 2) {
+                const float k9 = (0.295021795f * y) * (6.0f * xx - 2.0f * yy);
+                const float k10 = ((2.89061144f * x) * y) * z;
+                const float k11 = (3.6563664f * y) * ((-0.625f * xx - 0.625f * yy) + 0.5f);
+                const float k12 = (0.373176333f * z) * ((-5.0f * xx - 5.0f * yy) + 2.0f);
+                const float k13 = (0.457045799f * x) * ((-5.0f * xx - 5.0f * yy) + 4.0f);
+                const float k14 = ((1.44530572f * z) * (x - y)) * (x + y);
+                const float k15 = (0.59004359f * x) * (xx - 3.0f * yy);
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) {
+                    const float t = (((((k9 * c(27 + ch) + k10 * c(30 + ch)) + k11 * c(33 + ch)) +
+                                       k12 * c(36 + ch)) + k13 * c(39 + ch)) + k14 * c(42 + ch)) +
+                                    k15 * c(45 + ch);
+                    res[ch] = res[ch] + t;
+                }
+            }
+        }
+    }
+    float4 out;
+    out.x = gclamp(res[0] + 0.5f, 0.0f, 1.0f);
+    out.y = gclamp(res[1] + 0.5f, 0.0f, 1.0f);
+    out.z = gclamp(res[2] + 0.5f, 0.0f, 1.0f);
+    out.w = p.w;
+    a.color[i] = out;
+}
+
+template <typename T>
+static hipError_t launch_sh_t(const ShArgs& a, int degree, hipStream_t s) {
+    const dim3 grid((a.count + kShGroup - 1) / kShGroup), block(kShGroup);
+    switch (degree) {
+        case 0: hipLaunchKernelGGL((sh_kernel<0, T>), grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((sh_kernel<1, T>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((sh_kernel<2, T>), grid, block, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((sh_kernel<3, T>), grid, block, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sh(const ShArgs& a, int degree, int precision, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    switch (precision) {
+        case 0: return launch_sh_t<float>(a, degree, s);
+        case 1: return launch_sh_t<__half>(a, degree, s);
+        case 2: return launch_sh_t<uint16_t>(a, degree, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// EWA projection: core/gaussian_render.cu:135-215 (quaternion decode :45-48).
+// The eigenvector (cos, sin) is written directly instead of going through
+// atan2 -> cos/sin (core/gaussian_render.cu:200,212); the two differ by about
+// one ulp and the direct form is exactly reproducible on the host.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.count) return;
+    const float4 pd = a.position[i];
+    const float4 sr = a.scale_rotation[i];
+
+    m3 S;
+    S.c[0] = v3{sr.x, 0.0f, 0.0f};
+    S.c[1] = v3{0.0f, sr.y, 0.0f};
+    S.c[2] = v3{0.0f, 0.0f, sr.z};
+
+    const uint32_t q8 = __float_as_uint(sr.w);
+    const float rx = ((float)((q8 >> 24u) & 0xffu) / 255.0f) * 2.0f - 1.0f;
+    const float ry = ((float)((q8 >> 16u) & 0xffu) / 255.0f) * 2.0f - 1.0f;
+    const float rz = ((float)((q8 >> 8u) & 0xffu) / 255.0f) * 2.0f - 1.0f;
+    const float rw = ((float)(q8 & 0xffu) / 255.0f) * 2.0f - 1.0f;
+    const m3 RS = mul33(quat_to_m3(rw, rx, ry, rz), S);
+    const m3 sigma = mul33(RS, transpose3(RS));
+
+    const v4 vp4 = mul44v(a.cam.view, v4{pd.x, pd.y, pd.z, 1.0f});
+    const v3 v = v3{vp4.x, vp4.y, vp4.z};
+    const v2 fc = a.cam.fov_cotangent;
+    const v2 dsb = a.cam.depth_scale_bias;
+
+    const float zRcp = 1.0f / v.z;
+    const float zRcpSqr = zRcp * zRcp;
+    const float scaleX = -fc.x * zRcp;
+    const float scaleY = -fc.y * zRcp;
+    const float ftx = fc.x * v.x, fty = fc.y * v.y;
+    const float shearX = ftx * zRcpSqr;
+    const float shearY = fty * zRcpSqr;
+    const float transX = -ftx * zRcp;
+    const float transY = -fty * zRcp;
+
+    m4 A;
+    A.c[0] = v4{scaleX, 0.0f, 0.0f, 0.0f};
+    A.c[1] = v4{0.0f, scaleY, 0.0f, 0.0f};
+    A.c[2] = v4{shearX, shearY, dsb.x, 0.0f};
+    A.c[3] = v4{transX, transY, dsb.y, 1.0f};
+
+    const m3 VP = mul33(upper3(A), upper3(a.cam.view));
+    m3 cc = mul33(mul33(VP, transpose3(sigma)), transpose3(VP));
+    const v4 clip4 = mul44v(A, v4{v.x, v.y, v.z, 1.0f});
+    v3 clip = v3{clip4.x, clip4.y, clip4.z};
+
+    cc.c[0].x = cc.c[0].x + a.geom.trace_bump_x;
+    cc.c[1].y = cc.c[1].y + a.geom.trace_bump_y;
+
+    const float ca = cc.c[0].x, cb = cc.c[1].x, ccc = cc.c[1].y;
+    const float det = ca * ccc - cb * cb;
+    const float mid = 0.5f * (ca + ccc);
+    const float eps = 1e-12f;
+    const float radius = sqrtf(gmax(eps, mid * mid - det));
+    const float lambda0 = mid + radius;
+    const float lambda1 = gmax(0.0f, mid - radius);
+    const v2 e0 = normalize2(v2{cb, lambda0 - ca});
+    float extx = sqrtf(lambda0) * 3.0f;
+    float exty = sqrtf(lambda1) * 3.0f;
+    const float invDet = 1.0f / gmax(eps, det);
+    const float conx = ccc * invDet, cony = (-cb) * invDet, conz = ca * invDet;
+
+    const float ex = gstep(-1.0f, clip.x) * gstep(clip.x, 1.0f);
+    const float ey = gstep(-1.0f, clip.y) * gstep(clip.y, 1.0f);
+    const float ez = gstep(-1.0f, clip.z) * gstep(clip.z, 1.0f);
+    const float vis = ((ex * ey) * ez) * gstep(0.0f, lambda1);
+    clip.x = -128.0f * (1.0f - vis) + clip.x * vis;
+    clip.y = -128.0f * (1.0f - vis) + clip.y * vis;
+    clip.z = -128.0f * (1.0f - vis) + clip.z * vis;
+    extx = extx * vis;
+    exty = exty * vis;
+
+    a.clip_xy[i] = make_float2(clip.x, clip.y);
+    a.clip_z[i] = clip.z;
+    a.ellipse[i] = make_float4(e0.x, e0.y, extx, exty);
+    a.conic[i] = make_float4(conx, cony, conz, 0.0f);
+}
+
+hipError_t launch_project(const ProjectArgs& a, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(project_kernel, dim3((a.count + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Tile-list build.
+// Footprint: getAABBRect core/gaussian_render.cu:274-298 and the tile rect of
+// :379-389.  Exact test: ellipseRectOverlap :250-272 with
+// convertToEllipseCoordinates :234-239 and intersectsUnitCircle :241-248.
+// Key: getKey :306-310.  The reference lets atomics decide entry order
+// (:485, :498); here entries follow Gaussian index, then row-major candidate
+// order, so the stable sort that follows is deterministic.
+// ---------------------------------------------------------------------------
+struct Footprint {
+    int x0, y0, nx, ny;
+};
+
+__device__ __forceinline__ Footprint footprint(float cx, float cy, float csx, float csy, float ex,
+                                               float ey, const FrameGeom& g) {
+    const float rx = csx * ex, ry = csy * ex;        // principal axis
+    const float ux = csy * ey, uy = (-csx) * ey;     // minor axis
+    float mnx = 1e12f, mny = 1e12f, mxx = -1e12f, mxy = -1e12f;
+#pragma unroll
+    for (int j = 0; j != 4; ++j) {
+        const int bottom = j & 1, top = j >> 1;
+        const float r = (float)(bottom ^ top) * 2.0f - 1.0f;
+        const float u = (float)top * 2.0f - 1.0f;
+        const float vx = rx * r + ux * u;
+        const float vy = ry * r + uy * u;
+        mnx = gmin(mnx, vx);
+        mny = gmin(mny, vy);
+        mxx = gmax(mxx, vx);
+        mxy = gmax(mxy, vy);
+    }
+    mnx = mnx + cx;
+    mny = mny + cy;
+    mxx = mxx + cx;
+    mxy = mxy + cy;
+    const float tfx = ((mnx + 1.0f) * 0.5f) * g.tiles_per_axis_x;
+    const float tfy = ((mny + 1.0f) * 0.5f) * g.tiles_per_axis_y;
+    const float tfz = ((mxx + 1.0f) * 0.5f) * g.tiles_per_axis_x;
+    const float tfw = ((mxy + 1.0f) * 0.5f) * g.tiles_per_axis_y;
+    Footprint f;
+    f.x0 = iclamp(f2i_sat(floorf(tfx)), 0, g.tiles_x);
+    f.y0 = iclamp(f2i_sat(floorf(tfy)), 0, g.tiles_y);
+    const int x1 = iclamp(f2i_sat(ceilf(tfz)), 0, g.tiles_x);
+    const int y1 = iclamp(f2i_sat(ceilf(tfw)), 0, g.tiles_y);
+    f.nx = x1 - f.x0;
+    f.ny = y1 - f.y0;
+    return f;
+}
+
+__device__ __forceinline__ v2 to_ellipse(float px, float py, float cx, float cy, float csx, float csy,
+                                         float ex, float ey) {
+    const float dx = px - cx, dy = py - cy;
+    return v2{(dx * csx + dy * csy) / ex, (dy * csx - dx * csy) / ey};
+}
+
+__device__ __forceinline__ bool hits_unit_circle(v2 v0, v2 v1) {
+    const float ddx = v1.x - v0.x, ddy = v1.y - v0.y;
+    const float lsq = ddx * ddx + ddy * ddy;
+    const float t = saturate(((-v0.x) * ddx + (-v0.y) * ddy) / lsq);
+    const float px = v0.x + t * (v1.x - v0.x);
+    const float py = v0.y + t * (v1.y - v0.y);
+    return (px * px + py * py) < 1.0f;
+}
+
+__device__ __forceinline__ bool tile_overlap(int gx, int gy, float cx, float cy, float csx, float csy,
+                                             float ex, float ey, const FrameGeom& g) {
+    const float mnx = (float)gx * g.tile_clip_x - 1.0f;
+    const float mny = (float)gy * g.tile_clip_y - 1.0f;
+    const float mxx = mnx + g.tile_clip_x;
+    const float mxy = mny + g.tile_clip_y;
+    if (cx > mnx && cx < mxx && cy > mny && cy < mxy) return true;
+    const float ctx = mnx + (mxx - mnx) * 0.5f;
+    const float cty = mny + (mxy - mny) * 0.5f;
+    if (len2sq(to_ellipse(ctx, cty, cx, cy, csx, csy, ex, ey)) < 1.0f) return true;
+    const v2 p0 = to_ellipse(mnx, mny, cx, cy, csx, csy, ex, ey);
+    const v2 p1 = to_ellipse(mxx, mny, cx, cy, csx, csy, ex, ey);
+    const v2 p2 = to_ellipse(mxx, mxy, cx, cy, csx, csy, ex, ey);
+    const v2 p3 = to_ellipse(mnx, mxy, cx, cy, csx, csy, ex, ey);
+    return hits_unit_circle(p0, p1) || hits_unit_circle(p1, p2) || hits_unit_circle(p2, p3) ||
+           hits_unit_circle(p3, p0);
+}
+
+__device__ __forceinline__ uint32_t depth_word(float z) {
+    return f2u_sat(gclamp((z + 1.0f) * 0.5f, 0.0f, 1.0f) * 4294967296.0f);
+}
+
+// q = t / n, r = t % n for 0 <= t < 2^20, 1 <= n < 2^16 (float estimate + one fix)
+__device__ __forceinline__ void divmod_small(int t, int n, int& q, int& r) {
+    q = (int)((float)t * __builtin_amdgcn_rcpf((float)n));
+    r = t - q * n;
+    if (r < 0) { --q; r += n; }
+    else if (r >= n) { ++q; r -= n; }
+}
+
+template <bool EMIT>
+__global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
+    constexpr int W = kBuildThreads / 64;
+    __shared__ int s_excl[W][64];
+    __shared__ int s_x0[W][64];
+    __shared__ int s_y0[W][64];
+    __shared__ int s_nx[W][64];
+    __shared__ uint32_t s_depth[W][64];
+    __shared__ float s_cx[W][64], s_cy[W][64], s_csx[W][64], s_csy[W][64], s_ex[W][64], s_ey[W][64];
+
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int batch = blockIdx.x * W + w;
+    const int first = batch * kBuildBatch;
+    if (first >= a.count) return;   // wave-uniform; no block barrier below
+    const int g = first + lane;
+
+    int cnt = 0;
+    if (g < a.count) {
+        const float2 c = a.clip_xy[g];
+        const float4 e = a.ellipse[g];
+        const Footprint f = footprint(c.x, c.y, e.x, e.y, e.z, e.w, a.geom);
+        // the reference keeps max(0, w*h); a rect with w <= 0 or h <= 0 only
+        // arises from NaN corners and never passes the exact test
+        cnt = (f.nx > 0 && f.ny > 0) ? f.nx * f.ny : 0;
+        s_x0[w][lane] = f.x0;
+        s_y0[w][lane] = f.y0;
+        s_nx[w][lane] = f.nx;
+        s_cx[w][lane] = c.x;
+        s_cy[w][lane] = c.y;
+        s_csx[w][lane] = e.x;
+        s_csy[w][lane] = e.y;
+        s_ex[w][lane] = e.z;
+        s_ey[w][lane] = e.w;
+        if (EMIT) s_depth[w][lane] = depth_word(a.clip_z[g]);
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int n = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += n;
+    }
+    const int total = __shfl(incl, 63, 64);
+    s_excl[w][lane] = incl - cnt;
+    wave_sync();
+
+    uint32_t running = 0;
+    const uint32_t out_base = EMIT ? a.batch_counts[batch] : 0u;
+    for (int base = 0; base < total; base += 64) {
+        const int c = base + lane;
+        bool pass = false;
+        int o = 0, gx = 0, gy = 0;
+        if (c < total) {
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1)
+                if (s_excl[w][o + step] <= c) o += step;
+            int q, r;
+            divmod_small(c - s_excl[w][o], s_nx[w][o], q, r);
+            gx = s_x0[w][o] + r;
+            gy = s_y0[w][o] + q;
+            pass = tile_overlap(gx, gy, s_cx[w][o], s_cy[w][o], s_csx[w][o], s_csy[w][o],
+                                s_ex[w][o], s_ey[w][o], a.geom);
+        }
+        const uint64_t m = __ballot(pass);
+        if (EMIT && pass) {
+            const uint32_t pos = out_base + running + (uint32_t)__popcll(m & lanemask_lt(lane));
+            if (pos < (uint32_t)a.capacity) {
+                a.keys[pos] = ((uint64_t)(uint32_t)(gy * a.geom.tiles_x + gx) << 32) | s_depth[w][o];
+                a.values[pos] = first + o;
+            }
+        }
+        running += (uint32_t)__popcll(m);
+    }
+    if (!EMIT && lane == 0) a.batch_counts[batch] = running;
+}
+
+static int build_grid(int count) {
+    const int batches = (count + kBuildBatch - 1) / kBuildBatch;
+    constexpr int W = kBuildThreads / 64;
+    return (batches + W - 1) / W;
+}
+
+hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(build_kernel<false>, dim3(build_grid(a.count)), dim3(kBuildThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(build_kernel<true>, dim3(build_grid(a.count)), dim3(kBuildThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+// Exclusive scan of the per-batch counts, in place, one workgroup; writes the
+// grand total (saturated at 2^32-1) to *total.
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 8;
+
+__global__ __launch_bounds__(kScanThreads) void scan_kernel(uint32_t* data, int32_t n, uint32_t* total) {
+    __shared__ uint64_t s_wave[kScanThreads / 64];
+    __shared__ uint64_t s_carry;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < n; base += (int64_t)kScanThreads * kScanItems) {
+        uint64_t v[kScanItems];
+        uint64_t sum = 0;
+        const int64_t my = base + (int64_t)threadIdx.x * kScanItems;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            v[k] = (my + k < n) ? data[my + k] : 0u;
+            sum += v[k];
+        }
+        uint64_t incl = sum;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t t = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += t;
+        }
+        if (lane == 63) s_wave[w] = incl;
+        __syncthreads();
+        uint64_t wave_off = 0;
+        for (int k = 0; k < w; ++k) wave_off += s_wave[k];
+        const uint64_t carry = s_carry;
+        uint64_t run = carry + wave_off + incl - sum;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (my + k < n) data[my + k] = (uint32_t)(run > 0xffffffffull ? 0xffffffffull : run);
+            run += v[k];
+        }
+        __syncthreads();
+        if (threadIdx.x == kScanThreads - 1) s_carry = run;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = (uint32_t)(s_carry > 0xffffffffull ? 0xffffffffull : s_carry);
+}
+
+hipError_t launch_build_scan(uint32_t* batch_counts, int32_t batches, uint32_t* total, hipStream_t s) {
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kScanThreads), 0, s, batch_counts, batches, total);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Tile ranges: core/gaussian_render.cu:634-660; the -1 pre-fill is
+// app/main.cpp:365.
+// ---------------------------------------------------------------------------
+__global__ void fill_i32_kernel(int32_t* p, int32_t value, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, value, n);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void ranges_kernel(RangeArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        const uint32_t first = (uint32_t)(a.keys[0] >> 32);
+        a.tile_range[(size_t)first * 2] = 0;
+        const uint32_t last = (uint32_t)(a.keys[a.size - 1] >> 32);
+        a.tile_range[(size_t)last * 2 + 1] = a.size;
+    } else if (i < a.size) {
+        const uint32_t prev = (uint32_t)(a.keys[i - 1] >> 32);
+        const uint32_t cur = (uint32_t)(a.keys[i] >> 32);
+        if (cur != prev) {
+            a.tile_range[(size_t)prev * 2 + 1] = i;
+            a.tile_range[(size_t)cur * 2] = i;
+        }
+    }
+}
+
+hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
+    if (a.size <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ranges_kernel, dim3((a.size + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Raster: core/gaussian_render.cu:679-759.  Pixel (px, py) sits at clip
+// (px * 2/W - 1, py * 2/H - 1) (no half-pixel offset, :701-704).  Entries are
+// blended front to back; after every 32 entries of the tile the whole tile
+// stops if no pixel has T > 0.02 (:739-748).  Pixels of edge tiles outside the
+// frame take part in the vote but are not written.  The blend arithmetic is
+// the one of gr::blend_exponent / gr::blend_alpha:
+//   w = T * alpha;  C = fma(rgb, w, C);  T = T - w    (= C += rgb*T*alpha, T *= 1-alpha)
+// A wave skips an entry outright when every lane's exponent is below -25,
+// which is exactly the per-lane rule (alpha = 0 leaves C and T unchanged).
+// ---------------------------------------------------------------------------
+constexpr int kRasterThreads = 256;
+constexpr int kRasterBatch = 32;
+
+__global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
+    __shared__ float4 s_a[kRasterThreads];   // cx, cy, qa, qb
+    __shared__ float4 s_b[kRasterThreads];   // qc, opacity, r, g
+    __shared__ float s_c[kRasterThreads];    // b
+    __shared__ int s_alive[2][4];
+
+    const int bid = blockIdx.x;
+    const int tile = (bid & 7) * a.tiles_per_xcd + (bid >> 3);
+    if (tile >= a.tiles) return;   // uniform over the workgroup
+    const FrameGeom& g = a.geom;
+    const int tx = tile % g.tiles_x, ty = tile / g.tiles_x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int px = tx * kTileSize + (w & 1) * 8 + (lane & 7);
+    const int py = ty * kTileSize + (w >> 1) * 8 + (lane >> 3);
+    const float pcx = (float)px * g.texel_clip_x - 1.0f;
+    const float pcy = (float)py * g.texel_clip_y - 1.0f;
+
+    const int start = a.tile_range[2 * tile];
+    const int end = a.tile_range[2 * tile + 1];
+    const int n = end - start;
+
+    float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, T = 1.0f;
+    int parity = 0;
+    if (n > 0) {
+        bool stop = false;
+        for (int r0 = 0; r0 < n && !stop; r0 += kRasterThreads) {
+            const int nr = min(kRasterThreads, n - r0);
+            __syncthreads();
+            if ((int)threadIdx.x < nr) {
+                const int gi = a.values[start + r0 + threadIdx.x];
+                const float2 c = a.clip_xy[gi];
+                const float4 q = a.conic[gi];
+                const float4 col = a.color[gi];
+                s_a[threadIdx.x] = make_float4(c.x, c.y, q.x * kHalfLog2e, q.y * kLog2e);
+                s_b[threadIdx.x] = make_float4(q.z * kHalfLog2e, col.w, col.x, col.y);
+                s_c[threadIdx.x] = col.z;
+            }
+            __syncthreads();
+            for (int b0 = 0; b0 < nr; b0 += kRasterBatch) {
+                const int b1 = min(b0 + kRasterBatch, nr);
+                for (int j = b0; j < b1; ++j) {
+                    const float4 ea = s_a[j];
+                    const float dx = pcx - ea.x, dy = pcy - ea.y;
+                    const float4 eb = s_b[j];
+                    const float e = blend_exponent(ea.z, ea.w, eb.x, dx, dy);
+                    if (__ballot(e >= kMinExponent) != 0ull) {
+                        const float al = blend_alpha(e, eb.y);
+                        const float wgt = T * al;
+                        C0 = fmaf(eb.z, wgt, C0);
+                        C1 = fmaf(eb.w, wgt, C1);
+                        C2 = fmaf(s_c[j], wgt, C2);
+                        T = T - wgt;
+                    }
+                }
+                const bool alive = __ballot(T > kTransmittanceStop) != 0ull;
+                if (lane == 0) s_alive[parity][w] = alive ? 1 : 0;
+                __syncthreads();
+                const int any = s_alive[parity][0] | s_alive[parity][1] | s_alive[parity][2] |
+                                s_alive[parity][3];
+                parity ^= 1;
+                if (!any) { stop = true; break; }
+            }
+        }
+    }
+    if (px < g.width && py < g.height) {
+        uchar4 o;
+        if (n > 0) {
+            o.x = (unsigned char)f2u_sat(gclamp(C0, 0.0f, 1.0f) * 255.0f);
+            o.y = (unsigned char)f2u_sat(gclamp(C1, 0.0f, 1.0f) * 255.0f);
+            o.z = (unsigned char)f2u_sat(gclamp(C2, 0.0f, 1.0f) * 255.0f);
+            o.w = 255;
+        } else {
+            o = make_uchar4(0, 0, 0, 0);
+        }
+        a.out[(size_t)py * g.width + px] = o;
+    }
+}
+
+hipError_t launch_raster(const RasterArgs& a, hipStream_t s) {
+    if (a.tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(raster_kernel, dim3(8 * a.tiles_per_xcd), dim3(kRasterThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace gr
