@@ -171,6 +171,8 @@ gr_status gr_set_global_args(gr_context* ctx, const gr_global_args* args) {
         return fail(GR_ERR_INVALID_ARGUMENT, "SH degree %d is outside 0..3",
                     args->spherical_harmonics_degree);
     const int want = 3 * (args->spherical_harmonics_degree + 1) * (args->spherical_harmonics_degree + 1);
+    if (args->sh_precision < GR_PRECISION_FP32 || args->sh_precision > GR_PRECISION_BF16)
+        return fail(GR_ERR_INVALID_ARGUMENT, "unknown precision %d", args->sh_precision);
     if (args->spherical_harmonics_count != want)
         return fail(GR_ERR_INVALID_ARGUMENT, "spherical_harmonics_count %d != 3(d+1)^2 = %d",
                     args->spherical_harmonics_count, want);
@@ -192,9 +194,12 @@ gr_status gr_evaluate_spherical_harmonics(gr_context* ctx, gr_timer* timer, int3
     if (count < 0 || count > g.splat_count) return fail(GR_ERR_INVALID_ARGUMENT, "bad count %d", count);
     if (count > 0 && (!g.position || !g.color || !g.spherical_harmonics))
         return fail(GR_ERR_INVALID_ARGUMENT, "SH stage needs position, color and SH buffers");
+    if (g.sh_precision != GR_PRECISION_FP32 && count > 0 && !g.packed_scale_opacity)
+        return fail(GR_ERR_INVALID_ARGUMENT, "precision form needs packed_scale_opacity");
     ShArgs a;
     a.position = reinterpret_cast<const float4*>(g.position);
     a.sh = g.spherical_harmonics;
+    a.packed = reinterpret_cast<const uint2*>(g.packed_scale_opacity);
     a.color = reinterpret_cast<float4*>(g.color);
     a.camera_position = v3{g.camera_data.position[0], g.camera_data.position[1], g.camera_data.position[2]};
     a.count = count;
@@ -209,12 +214,15 @@ gr_status gr_evaluate_splat_clip_data(gr_context* ctx, gr_timer* timer, int32_t 
     if (!ctx || !ctx->has_args) return fail(GR_ERR_INVALID_ARGUMENT, "global args not set");
     const gr_global_args& g = ctx->args;
     if (count < 0 || count > g.splat_count) return fail(GR_ERR_INVALID_ARGUMENT, "bad count %d", count);
-    if (count > 0 && (!g.position || !g.scale_and_rotation || !g.position_clip_xy ||
-                      !g.position_clip_z || !g.screen_ellipse || !g.conic))
+    const bool packed = g.sh_precision != GR_PRECISION_FP32;
+    if (count > 0 && (!g.position || (!packed && !g.scale_and_rotation) || (packed && !g.packed_scale_opacity) ||
+                      !g.position_clip_xy || !g.position_clip_z || !g.screen_ellipse || !g.conic))
         return fail(GR_ERR_INVALID_ARGUMENT, "projection stage is missing a buffer");
     ProjectArgs a;
     a.position = reinterpret_cast<const float4*>(g.position);
     a.scale_rotation = reinterpret_cast<const float4*>(g.scale_and_rotation);
+    a.packed = reinterpret_cast<const uint2*>(g.packed_scale_opacity);
+    a.precision = g.sh_precision;
     a.clip_xy = reinterpret_cast<float2*>(g.position_clip_xy);
     a.clip_z = g.position_clip_z;
     a.ellipse = reinterpret_cast<float4*>(g.screen_ellipse);
@@ -368,6 +376,67 @@ gr_status gr_realign_spherical_harmonics(const float* src, float* dst, int32_t g
             float* d = dst + start + (size_t)group_size * i;
             for (int k = 0; k < here; ++k) d[k] = src[start + (size_t)sh_count * k + i];
             for (int k = here; k < group_size; ++k) d[k] = 0.0f;
+        }
+    }
+    return GR_OK;
+}
+
+uint16_t gr_float_to_half(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    const uint32_t a = u & 0x7fffffffu;
+    if (a >= 0x7f800000u) return (uint16_t)(sign | 0x7c00u | (a > 0x7f800000u ? 0x0200u : 0u));
+    if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);  // rounds to infinity
+    if (a < 0x38800000u) {                                       // FP16 subnormal or zero
+        float ax;
+        std::memcpy(&ax, &a, 4);
+        const float m = std::nearbyint(ax * 16777216.0f);         // units of 2^-24, ties to even
+        return (uint16_t)(sign | (uint32_t)m);
+    }
+    uint32_t h = (((a >> 23) - 112u) << 10) | ((a >> 13) & 0x3ffu);
+    const uint32_t rem = a & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;
+    return (uint16_t)(sign | h);
+}
+
+uint16_t gr_float_to_bf16(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+gr_status gr_pack_precision_model(const gr_model_host* m, int32_t precision, float* position_out,
+                                  uint16_t* packed_out, uint16_t* sh_out) {
+    if (!m || !position_out || !packed_out || m->num_gaussians < 0 || m->sh_degree < 0 || m->sh_degree > 3 ||
+        (precision != GR_PRECISION_FP16 && precision != GR_PRECISION_BF16) ||
+        (m->num_gaussians > 0 && (!m->position || !m->scale_and_rotation)) ||
+        (m->sh_degree > 0 && m->num_gaussians > 0 && (!m->sh || !sh_out)))
+        return fail(GR_ERR_INVALID_ARGUMENT, "gr_pack_precision_model: bad argument");
+    auto cv = precision == GR_PRECISION_FP16 ? gr_float_to_half : gr_float_to_bf16;
+    const int n = m->num_gaussians;
+    for (int i = 0; i < n; ++i) {
+        position_out[4 * i + 0] = m->position[4 * i + 0];
+        position_out[4 * i + 1] = m->position[4 * i + 1];
+        position_out[4 * i + 2] = m->position[4 * i + 2];
+        position_out[4 * i + 3] = m->scale_and_rotation[4 * i + 3];  // packed quaternion bits
+        packed_out[4 * i + 0] = cv(m->scale_and_rotation[4 * i + 0]);
+        packed_out[4 * i + 1] = cv(m->scale_and_rotation[4 * i + 1]);
+        packed_out[4 * i + 2] = cv(m->scale_and_rotation[4 * i + 2]);
+        packed_out[4 * i + 3] = cv(m->position[4 * i + 3]);           // opacity
+    }
+    if (m->sh_degree > 0) {
+        const int shc = 3 * (m->sh_degree + 1) * (m->sh_degree + 1);
+        const int groups = (n + kShGroup - 1) / kShGroup;
+        for (int grp = 0; grp < groups; ++grp) {
+            const int here = std::min(kShGroup, n - grp * kShGroup);
+            for (int j = 0; j < shc; ++j) {
+                uint16_t* d = sh_out + ((size_t)grp * shc + j) * kShGroup;
+                for (int k = 0; k < kShGroup; ++k)
+                    d[k] = k < here ? cv(m->sh[((size_t)grp * kShGroup + k) * shc + j]) : (uint16_t)0;
+            }
         }
     }
     return GR_OK;

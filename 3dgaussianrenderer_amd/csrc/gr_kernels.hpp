@@ -24,6 +24,7 @@ struct CameraArgs {
 struct ShArgs {
     const float4* position;
     const void* sh;          // aligned [groups][sh_count][256]
+    const uint2* packed;     // precision form: (scale x, y, z, opacity) 16-bit; else NULL
     float4* color;
     v3 camera_position;
     int32_t count;
@@ -32,7 +33,9 @@ struct ShArgs {
 
 struct ProjectArgs {
     const float4* position;
-    const float4* scale_rotation;
+    const float4* scale_rotation;   // FP32 form
+    const uint2* packed;            // precision form (scale + opacity, 16-bit)
+    int32_t precision;              // 0 FP32, 1 FP16, 2 BF16
     float2* clip_xy;
     float* clip_z;
     float4* ellipse;

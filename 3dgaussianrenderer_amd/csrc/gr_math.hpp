@@ -197,7 +197,8 @@ GR_HD int tile_bits_for(int tiles) {
 //   Chebyshev nodes, csrc/fit_exp2.py; max relative error 1.7e-7 with FP32
 //   Horner/FMA evaluation), result = ldexp(p(f), n).  p(f) <= 1 for f <= 0,
 //   so the density never exceeds 1.
-// Only e in [-25, 0] reaches it (see blend_alpha below).
+// Callers pass e in [-26, 0] (see blend_alpha below), so n fits an int
+// without saturation.
 // ---------------------------------------------------------------------------
 constexpr float kExp2C0 = 1.0f;
 constexpr float kExp2C1 = 6.931470037e-01f;
@@ -214,7 +215,7 @@ GR_HD float exp2_poly(float e) {
     p = fmaf(p, f, kExp2C2);
     p = fmaf(p, f, kExp2C1);
     p = fmaf(p, f, kExp2C0);
-    return ldexpf(p, f2i_sat(n));
+    return ldexpf(p, (int)n);
 }
 
 // Blend weights.  conic (a, b, c) of the reference becomes
@@ -232,7 +233,8 @@ GR_HD float blend_exponent(float qa, float qb, float qc, float dx, float dy) {
     return fmaf(dx, fmaf(qa, dx, qb * dy), (qc * dy) * dy);
 }
 GR_HD float blend_alpha(float e, float opacity) {
-    const float ec = (e < 0.0f) ? e : 0.0f;
+    float ec = (e < 0.0f) ? e : 0.0f;          // NaN -> 0 (discarded below)
+    ec = (ec > -26.0f) ? ec : -26.0f;          // keeps rint(ec) in int range
     const float a = opacity * exp2_poly(ec);
     return (e >= kMinExponent) ? a : 0.0f;
 }

@@ -76,8 +76,10 @@ struct gr_renderer {
     int device = 0;
     gr_context* ctx = nullptr;
     hipEvent_t ev[7] = {};
-    int32_t n = 0, degree = 0, sh_count = 3;
+    int32_t n = 0, degree = 0, sh_count = 3, precision = GR_PRECISION_FP32;
     DevBuf<float4> position, scale_rot, color, conic, ellipse;
+    DevBuf<uint2> packed;        // precision form: scale + opacity, 16-bit
+    DevBuf<uint16_t> sh16;       // precision form: aligned SH, 16-bit
     DevBuf<float2> clip_xy;
     DevBuf<float> clip_z, sh;
     DevBuf<int32_t> tile_range;
@@ -114,6 +116,7 @@ gr_status gr_renderer_destroy(gr_renderer* r) {
     if (r->ctx) (void)gr_context_synchronize(r->ctx);
     r->position.release(); r->scale_rot.release(); r->color.release(); r->conic.release();
     r->ellipse.release(); r->clip_xy.release(); r->clip_z.release(); r->sh.release();
+    r->packed.release(); r->sh16.release();
     r->tile_range.release(); r->keys[0].release(); r->keys[1].release();
     r->values[0].release(); r->values[1].release(); r->out.release();
     if (r->sort_temp) (void)hipFree(r->sort_temp);
@@ -133,11 +136,14 @@ gr_status gr_renderer_upload_model(gr_renderer* r, const gr_model_host* m) {
         return rfail(GR_ERR_INVALID_ARGUMENT, "degree-0 model needs the color array");
     if (m->sh_degree > 0 && m->num_gaussians > 0 && !m->sh)
         return rfail(GR_ERR_INVALID_ARGUMENT, "model SH array is NULL");
+    if (m->precision < GR_PRECISION_FP32 || m->precision > GR_PRECISION_BF16)
+        return rfail(GR_ERR_INVALID_ARGUMENT, "unknown precision");
     R_HIP(hipSetDevice(r->device));
     const int n = m->num_gaussians;
     r->n = n;
     r->degree = m->sh_degree;
     r->sh_count = 3 * (m->sh_degree + 1) * (m->sh_degree + 1);
+    r->precision = m->precision;
     const size_t nn = (size_t)std::max(n, 1);
     R_CHECK(r->position.ensure(nn));
     R_CHECK(r->scale_rot.ensure(nn));
@@ -146,6 +152,26 @@ gr_status gr_renderer_upload_model(gr_renderer* r, const gr_model_host* m) {
     R_CHECK(r->ellipse.ensure(nn));
     R_CHECK(r->clip_xy.ensure(nn));
     R_CHECK(r->clip_z.ensure(nn));
+    if (m->precision != GR_PRECISION_FP32) {
+        // precision form: 16-bit SH and scale/opacity, quaternion bits in position.w
+        const size_t groups = ((size_t)n + kShGroup - 1) / kShGroup;
+        std::vector<float> pos((size_t)4 * nn);
+        std::vector<uint16_t> pk((size_t)4 * nn);
+        std::vector<uint16_t> sh16(m->sh_degree > 0 ? groups * kShGroup * (size_t)r->sh_count : 1);
+        gr_status st = gr_pack_precision_model(m, m->precision, pos.data(), pk.data(), sh16.data());
+        if (st != GR_OK) return st;
+        R_CHECK(r->packed.ensure(nn));
+        R_CHECK(r->sh16.ensure(sh16.size()));
+        if (n > 0) {
+            R_HIP(hipMemcpy(r->position.p, pos.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+            R_HIP(hipMemcpy(r->packed.p, pk.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+            if (m->sh_degree > 0)
+                R_HIP(hipMemcpy(r->sh16.p, sh16.data(), sh16.size() * 2, hipMemcpyHostToDevice));
+            else
+                R_HIP(hipMemcpy(r->color.p, m->color, (size_t)n * 16, hipMemcpyHostToDevice));
+        }
+        return GR_OK;
+    }
     if (n > 0) {
         R_HIP(hipMemcpy(r->position.p, m->position, (size_t)n * 16, hipMemcpyHostToDevice));
         R_HIP(hipMemcpy(r->scale_rot.p, m->scale_and_rotation, (size_t)n * 16, hipMemcpyHostToDevice));
@@ -186,7 +212,8 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     ga.color = reinterpret_cast<float*>(r->color.p);
     ga.spherical_harmonics_degree = r->degree;
     ga.spherical_harmonics_count = r->sh_count;
-    ga.spherical_harmonics = r->sh.p;
+    ga.spherical_harmonics = r->precision != GR_PRECISION_FP32 ? (const void*)r->sh16.p : (const void*)r->sh.p;
+    ga.packed_scale_opacity = r->precision != GR_PRECISION_FP32 ? (const void*)r->packed.p : nullptr;
     ga.position_clip_xy = reinterpret_cast<float*>(r->clip_xy.p);
     ga.position_clip_z = r->clip_z.p;
     ga.conic = reinterpret_cast<float*>(r->conic.p);
@@ -196,7 +223,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     ga.tile_range = r->tile_range.p;
     ga.width = width;
     ga.height = height;
-    ga.sh_precision = GR_PRECISION_FP32;
+    ga.sh_precision = r->precision;
     R_CHECK(gr_set_global_args(ctx, &ga));
 
     gr_tile_list_args tl{};

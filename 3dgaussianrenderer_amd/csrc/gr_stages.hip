@@ -45,6 +45,15 @@ template <> __device__ __forceinline__ float sh_ld<uint16_t>(const uint16_t* p) 
     return __uint_as_float(((uint32_t)*p) << 16);
 }
 
+// 16-bit halves of the precision form -> float (exact)
+template <typename T> __device__ __forceinline__ float h16(uint32_t bits);
+template <> __device__ __forceinline__ float h16<__half>(uint32_t bits) {
+    return __half2float(__ushort_as_half((unsigned short)bits));
+}
+template <> __device__ __forceinline__ float h16<uint16_t>(uint32_t bits) {
+    return __uint_as_float(bits << 16);
+}
+
 template <int DEG, typename T>
 __global__ __launch_bounds__(256) void sh_kernel(ShArgs a) {
     const int i = blockIdx.x * kShGroup + threadIdx.x;
@@ -102,7 +111,11 @@ __global__ __launch_bounds__(256) void sh_kernel(ShArgs a) {
     out.x = gclamp(res[0] + 0.5f, 0.0f, 1.0f);
     out.y = gclamp(res[1] + 0.5f, 0.0f, 1.0f);
     out.z = gclamp(res[2] + 0.5f, 0.0f, 1.0f);
-    out.w = p.w;
+    if constexpr (sizeof(T) == 2) {
+        out.w = h16<T>(a.packed[i].y >> 16);   // opacity of the precision form
+    } else {
+        out.w = p.w;
+    }
     a.color[i] = out;
 }
 
@@ -135,11 +148,19 @@ hipError_t launch_sh(const ShArgs& a, int degree, int precision, hipStream_t s) 
 // atan2 -> cos/sin (core/gaussian_render.cu:200,212); the two differ by about
 // one ulp and the direct form is exactly reproducible on the host.
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.count) return;
     const float4 pd = a.position[i];
-    const float4 sr = a.scale_rotation[i];
+    float4 sr;
+    if constexpr (sizeof(T) == 2) {
+        // precision form: scale from 16-bit halves, quaternion bits in position.w
+        const uint2 pk = a.packed[i];
+        sr = make_float4(h16<T>(pk.x & 0xffffu), h16<T>(pk.x >> 16), h16<T>(pk.y & 0xffffu), pd.w);
+    } else {
+        sr = a.scale_rotation[i];
+    }
 
     m3 S;
     S.c[0] = v3{sr.x, 0.0f, 0.0f};
@@ -214,7 +235,13 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
 
 hipError_t launch_project(const ProjectArgs& a, hipStream_t s) {
     if (a.count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(project_kernel, dim3((a.count + 255) / 256), dim3(256), 0, s, a);
+    const dim3 grid((a.count + 255) / 256), block(256);
+    switch (a.precision) {
+        case 0: hipLaunchKernelGGL(project_kernel<float>, grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL(project_kernel<__half>, grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(project_kernel<uint16_t>, grid, block, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -99,6 +99,11 @@ typedef struct gr_global_args {
     int32_t width;                         /* pixels; reference: 1024 */
     int32_t height;                        /* pixels; reference: 1024 */
     int32_t sh_precision;                  /* gr_precision of spherical_harmonics */
+    /* Precision form only (sh_precision != FP32): 4 x 16-bit per Gaussian
+     * (scale x, y, z, opacity) in the same 16-bit format as the SH, and
+     * position.w then holds the packed quaternion bits instead of opacity.
+     * NULL in the FP32 form. */
+    const void* packed_scale_opacity;
 } gr_global_args;
 
 /* core/gaussian_render.cuh:74-80 (TileListArgs). */
@@ -139,6 +144,7 @@ typedef struct gr_model_host {
     const float* scale_and_rotation;   /* float4[N] */
     const float* color;                /* float4[N], used when sh_degree == 0; may be NULL otherwise */
     const float* sh;                   /* [N][3(d+1)^2], coefficient j channel c at 3j+c; NULL when d == 0 */
+    int32_t precision;                 /* gr_precision of the device copy (FP32 = the reference's form) */
 } gr_model_host;
 
 typedef struct gr_context gr_context;   /* stream + stage arguments + build scratch */
@@ -206,6 +212,16 @@ GR_API gr_status gr_rasterize_tile(gr_context* ctx, gr_timer* timer);
  * floats; padding lanes are zero. */
 GR_API gr_status gr_realign_spherical_harmonics(const float* src, float* dst, int32_t group_size,
                                                 int32_t sh_count, int32_t splat_count);
+/* Precision form (FP16/BF16 storage, FP32 arithmetic): converts an FP32
+ * model into the device layout of that form, with round-to-nearest-even.
+ *   position_out  float4[N]  (x, y, z, bit pattern of the packed quaternion)
+ *   packed_out    uint16[4N] (scale x, y, z, opacity) as FP16 or BF16
+ *   sh_out        uint16[ceil(N/256)*256*3(d+1)^2] aligned SH (may be NULL when d == 0) */
+GR_API gr_status gr_pack_precision_model(const gr_model_host* model, int32_t precision, float* position_out,
+                                         uint16_t* packed_out, uint16_t* sh_out);
+/* Scalar conversions of the precision form (round to nearest even). */
+GR_API uint16_t gr_float_to_half(float x);
+GR_API uint16_t gr_float_to_bf16(float x);
 /* encodeVec4 (core/ply_parser.cpp:231-239) */
 GR_API uint32_t gr_encode_vec4(float x, float y, float z, float w);
 /* Camera (core/camera.cpp:7-55): fill CameraData from a pose. */

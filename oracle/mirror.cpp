@@ -21,7 +21,8 @@
 //     of cos/sin(atan2(e0)) (core/gaussian_render.cu:200,212);
 //   - blend: e = dx*(qa*dx + qb*dy) + (qc*dy)*dy with qa = a*(-0.5 log2 e),
 //     qb = b*(-log2 e), qc = c*(-0.5 log2 e) evaluated with two FMAs; density
-//     2^min(e,0) from a degree-5 polynomial + ldexp; alpha = 0 when e < -25;
+//     2^clamp(e,-26,0) from a degree-5 polynomial + ldexp; alpha = 0 when
+//     e < -25 (or NaN);
 //     w = T*alpha, C = fma(rgb, w, C), T = T - w.
 // Parity pins for this file live in tests/test_oracle.py (closed forms,
 // brute force, FP64 agreement, textbook sort) -- see the header of that file.
@@ -503,7 +504,7 @@ inline float exp2_poly(float e) {
     p = std::fma(p, f, kE2[2]);
     p = std::fma(p, f, kE2[1]);
     p = std::fma(p, f, kE2[0]);
-    return std::ldexp(p, to_i32((double)nn));
+    return std::ldexp(p, (int)nn);
 }
 
 // Raster FP32 (core/gaussian_render.cu:679-759 with the blend above).
@@ -533,7 +534,8 @@ void raster32(const int32_t* range, const int32_t* vals, const float* xy, const 
                 for (int p = 0; p < 256; ++p) {
                     const float dx = pcx[p] - cx, dy = pcy[p] - cy;
                     const float e = std::fma(dx, std::fma(qa, dx, qb * dy), (qc * dy) * dy);
-                    const float ec = (e < 0.0f) ? e : 0.0f;
+                    float ec = (e < 0.0f) ? e : 0.0f;
+                    ec = (ec > -26.0f) ? ec : -26.0f;
                     float al = op * exp2_poly(ec);
                     al = (e >= -25.0f) ? al : 0.0f;
                     const float w = T[p] * al;
