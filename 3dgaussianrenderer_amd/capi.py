@@ -45,7 +45,7 @@ class GlobalArgs(C.Structure):
                 ("position_clip_xy", C.c_void_p), ("position_clip_z", C.c_void_p), ("conic", C.c_void_p),
                 ("screen_ellipse", C.c_void_p), ("splat_count", C.c_int32), ("back_buffer", C.c_void_p),
                 ("tile_range", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32),
-                ("sh_precision", C.c_int32)]
+                ("sh_precision", C.c_int32), ("packed_scale_opacity", C.c_void_p)]
 
 
 class TileListArgs(C.Structure):
@@ -71,7 +71,8 @@ class RenderStats(C.Structure):
 
 class ModelHost(C.Structure):
     _fields_ = [("num_gaussians", C.c_int32), ("sh_degree", C.c_int32), ("position", C.c_void_p),
-                ("scale_and_rotation", C.c_void_p), ("color", C.c_void_p), ("sh", C.c_void_p)]
+                ("scale_and_rotation", C.c_void_p), ("color", C.c_void_p), ("sh", C.c_void_p),
+                ("precision", C.c_int32)]
 
 
 class SceneParams(C.Structure):
@@ -111,6 +112,9 @@ SIGNATURES = [
     ("gr_rasterize_tile", C.c_int, [_P, _P]),
     ("gr_realign_spherical_harmonics", C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32]),
     ("gr_encode_vec4", C.c_uint32, [C.c_float, C.c_float, C.c_float, C.c_float]),
+    ("gr_pack_precision_model", C.c_int, [C.POINTER(ModelHost), C.c_int32, _P, _P, _P]),
+    ("gr_float_to_half", C.c_uint16, [C.c_float]),
+    ("gr_float_to_bf16", C.c_uint16, [C.c_float]),
     ("gr_camera_make", C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.c_int32, C.c_int32,
                                  C.c_float, C.c_float, C.POINTER(CameraData)]),
     ("gr_tile_grid", C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -181,6 +185,23 @@ def realign_sh(sh_aos: np.ndarray, n: int, sh_count: int, group: int = 256) -> n
     src = np.ascontiguousarray(sh_aos, dtype=np.float32)
     check(lib().gr_realign_spherical_harmonics(_fp(src), _fp(out), group, sh_count, n))
     return out
+
+
+def model_host(scene: dict, precision: int = 0) -> ModelHost:
+    return ModelHost(scene["count"], scene["degree"], _fp(scene["position"]), _fp(scene["scale_rot"]),
+                     _fp(scene["color"]), _fp(scene["sh"]) if scene["sh"] is not None else None, precision)
+
+
+def pack_precision(scene: dict, precision: int):
+    """FP32 scene -> precision-form arrays (position with quaternion bits, packed u16[N,4], aligned u16 SH)."""
+    n, d = scene["count"], scene["degree"]
+    pos = np.empty((n, 4), np.float32)
+    packed = np.empty((n, 4), np.uint16)
+    groups = (n + 255) // 256
+    sh = np.empty(groups * 256 * 3 * (d + 1) ** 2 if d else 1, np.uint16)
+    m = model_host(scene)
+    check(lib().gr_pack_precision_model(C.byref(m), precision, _fp(pos), _fp(packed), _fp(sh)))
+    return pos, packed, (sh if d else None)
 
 
 def generate_scene(kind: int, count: int, degree: int = 3, seed: int = 1) -> dict:
@@ -291,10 +312,9 @@ class Renderer:
         except Exception:
             pass
 
-    def upload(self, scene: dict):
+    def upload(self, scene: dict, precision: int = 0):
         self._keep = scene
-        m = ModelHost(scene["count"], scene["degree"], _fp(scene["position"]), _fp(scene["scale_rot"]),
-                      _fp(scene["color"]), _fp(scene["sh"]) if scene["sh"] is not None else None)
+        m = model_host(scene, precision)
         check(lib().gr_renderer_upload_model(self.h, C.byref(m)))
 
     def render(self, cam: CameraData, width: int, height: int, out=None):

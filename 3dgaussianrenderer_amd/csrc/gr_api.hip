@@ -245,7 +245,7 @@ gr_status gr_build_tile_list(gr_context* ctx, gr_timer* timer, int32_t num_block
     const gr_global_args& g = ctx->args;
     const int n = g.splat_count;
     const int batches = (n + kBuildBatch - 1) / kBuildBatch;
-    GR_CHECK(grow(ctx->batch_counts, ctx->batch_cap, (size_t)batches + 1));
+    GR_CHECK(grow(ctx->batch_counts, ctx->batch_cap, (size_t)batches + 1 + scan_scratch_words(batches)));
     BuildArgs a;
     a.clip_xy = reinterpret_cast<const float2*>(g.position_clip_xy);
     a.clip_z = g.position_clip_z;
@@ -262,7 +262,8 @@ gr_status gr_build_tile_list(gr_context* ctx, gr_timer* timer, int32_t num_block
     uint32_t total = 0;
     if (n > 0) {
         GR_HIP(launch_build_count(a, ctx->stream));
-        GR_HIP(launch_build_scan(ctx->batch_counts, batches, ctx->batch_counts + batches, ctx->stream));
+        GR_HIP(launch_build_scan(ctx->batch_counts, batches, ctx->batch_counts + batches,
+                                 ctx->batch_counts + batches + 1, ctx->stream));
         GR_HIP(hipMemcpyAsync(ctx->pinned, ctx->batch_counts + batches, sizeof(uint32_t),
                               hipMemcpyDeviceToHost, ctx->stream));
         GR_HIP(hipStreamSynchronize(ctx->stream));

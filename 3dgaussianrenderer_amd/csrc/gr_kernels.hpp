@@ -85,7 +85,9 @@ constexpr int kBuildThreads = 256;
 hipError_t launch_sh(const ShArgs& a, int degree, int precision, hipStream_t stream);
 hipError_t launch_project(const ProjectArgs& a, hipStream_t stream);
 hipError_t launch_build_count(const BuildArgs& a, hipStream_t stream);
-hipError_t launch_build_scan(uint32_t* batch_counts, int32_t batches, uint32_t* total,
+// exclusive scan of n uint32 in place; scratch holds scan_scratch_words(n) words
+int scan_scratch_words(int32_t n);
+hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_t* scratch,
                              hipStream_t stream);
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t stream);
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t stream);

@@ -26,12 +26,22 @@
 
 namespace gr {
 
-constexpr int kSortThreads = 256;
+#ifndef GR_SORT_THREADS
+#define GR_SORT_THREADS 512
+#endif
+#ifndef GR_SORT_ITEMS
+#define GR_SORT_ITEMS 8
+#endif
+#ifndef GR_SORT_LOOKBACK
+#define GR_SORT_LOOKBACK 16
+#endif
+constexpr int kSortThreads = GR_SORT_THREADS;
 constexpr int kSortWaves = kSortThreads / 64;
-constexpr int kSortItems = 8;
-constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kSortItems = GR_SORT_ITEMS;
+constexpr int kSortTile = kSortThreads * kSortItems;   // pairs per partition
 constexpr int kRadix = 256;
 constexpr int kMaxPasses = 8;
+constexpr int kLookback = GR_SORT_LOOKBACK;            // predecessors read per look-back step
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagIncl = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1u;
@@ -70,21 +80,59 @@ __global__ void sort_init_kernel(uint32_t* hist, uint32_t* tickets, uint32_t* st
     for (int64_t i = tid; i < status_n; i += (int64_t)gridDim.x * blockDim.x) status0[i] = 0u;
 }
 
-__global__ __launch_bounds__(256) void sort_hist_kernel(const uint64_t* __restrict__ keys, int32_t n,
-                                                        int begin_bit, int end_bit, int passes,
-                                                        uint32_t* hist) {
+// Digit histograms of every pass in one read.  A workgroup stages 4096 keys
+// in LDS with coalesced loads; each lane then walks 16 consecutive keys and
+// adds runs of equal digits with one LDS atomic per run.  Tile-list keys
+// arrive in emission order, where neighbours share digits, so a lane per key
+// would put many lanes of one instruction on one LDS address.  The staging
+// array is skewed by one slot per 16 keys to spread the per-lane walks over
+// the LDS banks.
+constexpr int kHistThreads = 256;
+constexpr int kHistSeg = 16;
+constexpr int kHistChunk = kHistThreads * kHistSeg;
+
+__global__ __launch_bounds__(kHistThreads) void sort_hist_kernel(const uint64_t* __restrict__ keys, int32_t n,
+                                                                 int begin_bit, int end_bit, int passes,
+                                                                 uint32_t* hist) {
     __shared__ uint32_t s_h[kMaxPasses][kRadix];
+    __shared__ uint64_t s_k[kHistChunk + kHistChunk / kHistSeg];
     for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += blockDim.x) (&s_h[0][0])[i] = 0u;
-    __syncthreads();
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t k = keys[i];
-        for (int p = 0; p < passes; ++p) {
-            const int shift = begin_bit + 8 * p;
-            const int bits = min(8, end_bit - shift);
-            const uint32_t d = (uint32_t)(k >> shift) & ((1u << bits) - 1u);
-            atomicAdd(&s_h[p][d], 1u);
+    for (int64_t base = (int64_t)blockIdx.x * kHistChunk; base < n; base += (int64_t)gridDim.x * kHistChunk) {
+        const int m = (int)min((int64_t)kHistChunk, (int64_t)n - base);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kHistSeg; ++k) {
+            const int i = k * kHistThreads + threadIdx.x;
+            if (i < m) s_k[i + i / kHistSeg] = keys[base + i];
         }
+        __syncthreads();
+        uint32_t cur[kMaxPasses], cnt[kMaxPasses];
+#pragma unroll
+        for (int p = 0; p < kMaxPasses; ++p) {
+            cur[p] = 0u;
+            cnt[p] = 0u;
+        }
+        const int my = threadIdx.x * kHistSeg;
+        for (int j = 0; j < kHistSeg && my + j < m; ++j) {
+            const uint64_t k = s_k[my + j + threadIdx.x];
+#pragma unroll
+            for (int p = 0; p < kMaxPasses; ++p) {
+                if (p < passes) {
+                    const int shift = begin_bit + 8 * p;
+                    const int bits = min(8, end_bit - shift);
+                    const uint32_t d = (uint32_t)(k >> shift) & ((1u << bits) - 1u);
+                    if (d != cur[p]) {
+                        if (cnt[p]) atomicAdd(&s_h[p][cur[p]], cnt[p]);
+                        cur[p] = d;
+                        cnt[p] = 0u;
+                    }
+                    ++cnt[p];
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < kMaxPasses; ++p)
+            if (p < passes && cnt[p]) atomicAdd(&s_h[p][cur[p]], cnt[p]);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < passes * kRadix; i += blockDim.x) {
@@ -127,7 +175,7 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     __shared__ uint32_t s_wcnt[kSortWaves][kRadix];
     __shared__ uint32_t s_dstart[kRadix];
     __shared__ uint32_t s_gbase[kRadix];
-    __shared__ uint32_t s_wsum[kSortWaves];
+    __shared__ uint32_t s_wsum[kRadix / 64];
     __shared__ uint64_t s_keys[kSortTile];
     __shared__ int32_t s_vals[kSortTile];
 
@@ -136,29 +184,25 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     for (int i = t; i < kSortWaves * kRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0u;
     __syncthreads();
     const int part = (int)s_part;
-    if (status_next) status_next[(size_t)part * kRadix + t] = 0u;
+    if (status_next && t < kRadix) status_next[(size_t)part * kRadix + t] = 0u;
 
     const int64_t base = (int64_t)part * kSortTile;
+    const int64_t wbase = base + (int64_t)w * (64 * kSortItems) + lane;
     const uint32_t dmask = (1u << bits) - 1u;
     uint64_t key[kSortItems];
     int32_t val[kSortItems];
     uint32_t rank[kSortItems];
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
-        const int64_t idx = base + (int64_t)w * (64 * kSortItems) + i * 64 + lane;
-        if (idx < n) {
-            key[i] = kin[idx];
-            val[i] = vin[idx];
-        } else {
-            key[i] = 0;
-            val[i] = 0;
-        }
+        const int64_t idx = wbase + i * 64;
+        key[i] = idx < n ? kin[idx] : 0ull;
+        val[i] = idx < n ? vin[idx] : 0;
     }
+    // per-wave stable ranking: peers = lanes with the same digit (8 ballots)
     const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
-        const int64_t idx = base + (int64_t)w * (64 * kSortItems) + i * 64 + lane;
-        const bool valid = idx < n;
+        const bool valid = wbase + i * 64 < n;
         const uint32_t d = (uint32_t)(key[i] >> shift) & dmask;
         uint64_t peers = __ballot(valid);
         for (int b = 0; b < bits; ++b) {
@@ -173,55 +217,66 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     }
     __syncthreads();
 
-    // per digit: wave prefix and partition total (thread t owns digit t)
-    uint32_t run = 0;
+    // threads 0..255 own one digit each: wave prefix, partition total, look-back
+    uint32_t total = 0, excl = 0;
+    if (t < kRadix) {
 #pragma unroll
-    for (int k = 0; k < kSortWaves; ++k) {
-        const uint32_t c = s_wcnt[k][t];
-        s_wcnt[k][t] = run;
-        run += c;
-    }
-    const uint32_t total = run;
-
-    // decoupled look-back for the global prefix of digit t
-    uint32_t* my = status + (size_t)part * kRadix + t;
-    uint32_t excl = 0;
-    if (part == 0) {
-        st_status(my, kFlagIncl | total);
-    } else {
-        st_status(my, kFlagAgg | total);
-        int j = part - 1;
-        while (true) {
-            const uint32_t s = ld_status(status + (size_t)j * kRadix + t);
-            const uint32_t flag = s & ~kCountMask;
-            if (flag == 0u) continue;
-            excl += s & kCountMask;
-            if (flag == kFlagIncl) break;
-            --j;
+        for (int k = 0; k < kSortWaves; ++k) {
+            const uint32_t c = s_wcnt[k][t];
+            s_wcnt[k][t] = total;
+            total += c;
         }
-        st_status(my, kFlagIncl | (excl + total));
-    }
-
-    // exclusive scan of the partition totals over digits
-    uint32_t incl = total;
+        uint32_t* my = status + (size_t)part * kRadix + t;
+        if (part == 0) {
+            st_status(my, kFlagIncl | total);
+        } else {
+            st_status(my, kFlagAgg | total);
+            int j = part - 1;
+            bool done = false;
+            while (!done) {
+                uint32_t sv[kLookback];
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t o = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += o;
+                for (int q = 0; q < kLookback; ++q)
+                    sv[q] = (j - q >= 0) ? ld_status(status + (size_t)(j - q) * kRadix + t) : kFlagIncl;
+                int q = 0;
+                for (; q < kLookback; ++q) {
+                    const uint32_t flag = sv[q] & ~kCountMask;
+                    if (flag == 0u) break;               // not yet published: poll again from here
+                    excl += sv[q] & kCountMask;
+                    if (flag == kFlagIncl) {
+                        done = true;
+                        break;
+                    }
+                }
+                if (!done) j -= q;
+            }
+            st_status(my, kFlagIncl | (excl + total));
+        }
     }
-    if (lane == 63) s_wsum[w] = incl;
+    // exclusive scan of the partition totals over the 256 digits (waves 0..3)
+    if (t < kRadix) {
+        uint32_t incl = total;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += o;
+        }
+        if (lane == 63) s_wsum[w] = incl;
+        s_dstart[t] = incl - total;    // within-wave part, completed below
+    }
     __syncthreads();
-    uint32_t woff = 0;
-    for (int k = 0; k < w; ++k) woff += s_wsum[k];
-    const uint32_t dstart = woff + incl - total;
-    s_dstart[t] = dstart;
-    s_gbase[t] = digit_off[t] + excl - dstart;
+    if (t < kRadix) {
+        uint32_t woff = 0;
+        for (int k = 0; k < w; ++k) woff += s_wsum[k];
+        const uint32_t dstart = s_dstart[t] + woff;
+        s_dstart[t] = dstart;
+        s_gbase[t] = digit_off[t] + excl - dstart;
+    }
     __syncthreads();
 
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
-        const int64_t idx = base + (int64_t)w * (64 * kSortItems) + i * 64 + lane;
-        if (idx < n) {
+        if (wbase + i * 64 < n) {
             const uint32_t d = (uint32_t)(key[i] >> shift) & dmask;
             const uint32_t lp = s_dstart[d] + s_wcnt[w][d] + rank[i];
             s_keys[lp] = key[i];
@@ -260,9 +315,9 @@ hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel,
     if (init_blocks > 4096) init_blocks = 4096;
     hipLaunchKernelGGL(sort_init_kernel, dim3((unsigned)init_blocks), dim3(256), 0, s, hist, tickets,
                        status[0], status_n);
-    int hist_blocks = (count + 256 * 16 - 1) / (256 * 16);
-    if (hist_blocks > 2048) hist_blocks = 2048;
-    hipLaunchKernelGGL(sort_hist_kernel, dim3(hist_blocks), dim3(256), 0, s, keys[*sel], count,
+    int hist_blocks = (count + kHistChunk - 1) / kHistChunk;
+    if (hist_blocks > 1024) hist_blocks = 1024;
+    hipLaunchKernelGGL(sort_hist_kernel, dim3(hist_blocks), dim3(kHistThreads), 0, s, keys[*sel], count,
                        begin_bit, end_bit, passes, hist);
     hipLaunchKernelGGL(sort_hist_scan_kernel, dim3(passes), dim3(256), 0, s, hist);
     for (int p = 0; p < passes; ++p) {

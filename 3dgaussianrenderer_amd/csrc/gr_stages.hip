@@ -293,12 +293,40 @@ __device__ __forceinline__ Footprint footprint(float cx, float cy, float csx, fl
     return f;
 }
 
-__device__ __forceinline__ v2 to_ellipse(float px, float py, float cx, float cy, float csx, float csy,
-                                         float ex, float ey) {
-    const float dx = px - cx, dy = py - cy;
-    return v2{(dx * csx + dy * csy) / ex, (dy * csx - dx * csy) / ey};
+// IEEE quotient a / b from rb = RN(1/b): q0 = RN(a*rb), e = a - b*q0 (exact
+// through the FMA), q = RN(q0 + e*rb) is the correctly rounded quotient when
+// nothing underflows or overflows (Markstein 1990, "Computation of elementary
+// functions on the IBM RISC System/6000 processor", Theorem 1).  The caller
+// guarantees 2^-40 <= |b| <= 2^40 (else fast == false); tiny or zero
+// numerators take the exact paths below.  Same bits as a / b, a third of the
+// instructions of the full division sequence.
+__device__ __forceinline__ float div_by(float a, float b, float rb, bool fast) {
+    if (fast && fabsf(a) >= 0x1p-40f) {
+        const float q0 = a * rb;
+        const float e = fmaf(-b, q0, a);
+        return fmaf(e, rb, q0);
+    }
+    if (fast && a == 0.0f) return a * rb;
+    return a / b;
 }
 
+struct EllipseRec {
+    float cx, cy, csx, csy;     // centre, (cos, sin)
+    float ex, ey, rex, rey;     // extents and their correctly rounded reciprocals
+};
+
+__device__ __forceinline__ bool div_fast_ok(float b) {
+    const float ab = fabsf(b);
+    return ab >= 0x1p-40f && ab <= 0x1p40f;
+}
+
+// convertToEllipseCoordinates (core/gaussian_render.cu:234-239)
+__device__ __forceinline__ v2 to_ellipse(float px, float py, const EllipseRec& e, bool fx, bool fy) {
+    const float dx = px - e.cx, dy = py - e.cy;
+    return v2{div_by(dx * e.csx + dy * e.csy, e.ex, e.rex, fx), div_by(dy * e.csx - dx * e.csy, e.ey, e.rey, fy)};
+}
+
+// intersectsUnitCircle (core/gaussian_render.cu:241-248)
 __device__ __forceinline__ bool hits_unit_circle(v2 v0, v2 v1) {
     const float ddx = v1.x - v0.x, ddy = v1.y - v0.y;
     const float lsq = ddx * ddx + ddy * ddy;
@@ -308,20 +336,22 @@ __device__ __forceinline__ bool hits_unit_circle(v2 v0, v2 v1) {
     return (px * px + py * py) < 1.0f;
 }
 
-__device__ __forceinline__ bool tile_overlap(int gx, int gy, float cx, float cy, float csx, float csy,
-                                             float ex, float ey, const FrameGeom& g) {
+// ellipseRectOverlap (core/gaussian_render.cu:250-272) for tile (gx, gy); the
+// reference ORs all six tests, here the first true one returns.
+__device__ __forceinline__ bool tile_overlap(int gx, int gy, const EllipseRec& e, const FrameGeom& g) {
     const float mnx = (float)gx * g.tile_clip_x - 1.0f;
     const float mny = (float)gy * g.tile_clip_y - 1.0f;
     const float mxx = mnx + g.tile_clip_x;
     const float mxy = mny + g.tile_clip_y;
-    if (cx > mnx && cx < mxx && cy > mny && cy < mxy) return true;
+    if (e.cx > mnx && e.cx < mxx && e.cy > mny && e.cy < mxy) return true;
+    const bool fx = div_fast_ok(e.ex), fy = div_fast_ok(e.ey);
     const float ctx = mnx + (mxx - mnx) * 0.5f;
     const float cty = mny + (mxy - mny) * 0.5f;
-    if (len2sq(to_ellipse(ctx, cty, cx, cy, csx, csy, ex, ey)) < 1.0f) return true;
-    const v2 p0 = to_ellipse(mnx, mny, cx, cy, csx, csy, ex, ey);
-    const v2 p1 = to_ellipse(mxx, mny, cx, cy, csx, csy, ex, ey);
-    const v2 p2 = to_ellipse(mxx, mxy, cx, cy, csx, csy, ex, ey);
-    const v2 p3 = to_ellipse(mnx, mxy, cx, cy, csx, csy, ex, ey);
+    if (len2sq(to_ellipse(ctx, cty, e, fx, fy)) < 1.0f) return true;
+    const v2 p0 = to_ellipse(mnx, mny, e, fx, fy);
+    const v2 p1 = to_ellipse(mxx, mny, e, fx, fy);
+    const v2 p2 = to_ellipse(mxx, mxy, e, fx, fy);
+    const v2 p3 = to_ellipse(mnx, mxy, e, fx, fy);
     return hits_unit_circle(p0, p1) || hits_unit_circle(p1, p2) || hits_unit_circle(p2, p3) ||
            hits_unit_circle(p3, p0);
 }
@@ -342,11 +372,8 @@ template <bool EMIT>
 __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
     constexpr int W = kBuildThreads / 64;
     __shared__ int s_excl[W][64];
-    __shared__ int s_x0[W][64];
-    __shared__ int s_y0[W][64];
-    __shared__ int s_nx[W][64];
-    __shared__ uint32_t s_depth[W][64];
-    __shared__ float s_cx[W][64], s_cy[W][64], s_csx[W][64], s_csy[W][64], s_ex[W][64], s_ey[W][64];
+    __shared__ int4 s_rect[W][64];          // x0, y0, nx, depth word
+    __shared__ EllipseRec s_ell[W][64];
 
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -363,16 +390,10 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
         // the reference keeps max(0, w*h); a rect with w <= 0 or h <= 0 only
         // arises from NaN corners and never passes the exact test
         cnt = (f.nx > 0 && f.ny > 0) ? f.nx * f.ny : 0;
-        s_x0[w][lane] = f.x0;
-        s_y0[w][lane] = f.y0;
-        s_nx[w][lane] = f.nx;
-        s_cx[w][lane] = c.x;
-        s_cy[w][lane] = c.y;
-        s_csx[w][lane] = e.x;
-        s_csy[w][lane] = e.y;
-        s_ex[w][lane] = e.z;
-        s_ey[w][lane] = e.w;
-        if (EMIT) s_depth[w][lane] = depth_word(a.clip_z[g]);
+        if (cnt > 0) {
+            s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, EMIT ? (int)depth_word(a.clip_z[g]) : 0);
+            s_ell[w][lane] = EllipseRec{c.x, c.y, e.x, e.y, e.z, e.w, 1.0f / e.z, 1.0f / e.w};
+        }
     }
     int incl = cnt;
 #pragma unroll
@@ -390,22 +411,23 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
         const int c = base + lane;
         bool pass = false;
         int o = 0, gx = 0, gy = 0;
+        int4 rc = make_int4(0, 0, 1, 0);
         if (c < total) {
 #pragma unroll
             for (int step = 32; step >= 1; step >>= 1)
                 if (s_excl[w][o + step] <= c) o += step;
+            rc = s_rect[w][o];
             int q, r;
-            divmod_small(c - s_excl[w][o], s_nx[w][o], q, r);
-            gx = s_x0[w][o] + r;
-            gy = s_y0[w][o] + q;
-            pass = tile_overlap(gx, gy, s_cx[w][o], s_cy[w][o], s_csx[w][o], s_csy[w][o],
-                                s_ex[w][o], s_ey[w][o], a.geom);
+            divmod_small(c - s_excl[w][o], rc.z, q, r);
+            gx = rc.x + r;
+            gy = rc.y + q;
+            pass = tile_overlap(gx, gy, s_ell[w][o], a.geom);
         }
         const uint64_t m = __ballot(pass);
         if (EMIT && pass) {
             const uint32_t pos = out_base + running + (uint32_t)__popcll(m & lanemask_lt(lane));
             if (pos < (uint32_t)a.capacity) {
-                a.keys[pos] = ((uint64_t)(uint32_t)(gy * a.geom.tiles_x + gx) << 32) | s_depth[w][o];
+                a.keys[pos] = ((uint64_t)(uint32_t)(gy * a.geom.tiles_x + gx) << 32) | (uint32_t)rc.w;
                 a.values[pos] = first + o;
             }
         }
@@ -432,52 +454,107 @@ hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Exclusive scan of the per-batch counts, in place, one workgroup; writes the
-// grand total (saturated at 2^32-1) to *total.
-constexpr int kScanThreads = 1024;
-constexpr int kScanItems = 8;
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t value, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
 
-__global__ __launch_bounds__(kScanThreads) void scan_kernel(uint32_t* data, int32_t n, uint32_t* total) {
-    __shared__ uint64_t s_wave[kScanThreads / 64];
+// Exclusive scan of the per-batch counts, in place, in three launches
+// (reduce per 2048-element block, scan of the block sums, local scan + block
+// offset); the grand total (saturated at 2^32-1) goes to *total.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t block_sum_256(uint32_t v, uint32_t* s_w) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_down(v, d, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* data, int32_t n, uint32_t* sums) {
+    __shared__ uint32_t s_w[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k * kScanThreads + threadIdx.x;
+        if (i < n) v += data[i];
+    }
+    const uint32_t tot = block_sum_256(v, s_w);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void scan_sums_kernel(uint32_t* sums, int32_t nb, uint32_t* total) {
+    __shared__ uint64_t s_w[16];
     __shared__ uint64_t s_carry;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (threadIdx.x == 0) s_carry = 0;
     __syncthreads();
-    for (int64_t base = 0; base < n; base += (int64_t)kScanThreads * kScanItems) {
-        uint64_t v[kScanItems];
-        uint64_t sum = 0;
-        const int64_t my = base + (int64_t)threadIdx.x * kScanItems;
-#pragma unroll
-        for (int k = 0; k < kScanItems; ++k) {
-            v[k] = (my + k < n) ? data[my + k] : 0u;
-            sum += v[k];
-        }
-        uint64_t incl = sum;
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + threadIdx.x;
+        const uint64_t v = i < nb ? sums[i] : 0u;
+        unsigned long long incl = v;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
-            const uint64_t t = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += t;
+            const unsigned long long o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += o;
         }
-        if (lane == 63) s_wave[w] = incl;
+        if (lane == 63) s_w[w] = incl;
         __syncthreads();
-        uint64_t wave_off = 0;
-        for (int k = 0; k < w; ++k) wave_off += s_wave[k];
-        const uint64_t carry = s_carry;
-        uint64_t run = carry + wave_off + incl - sum;
-#pragma unroll
-        for (int k = 0; k < kScanItems; ++k) {
-            if (my + k < n) data[my + k] = (uint32_t)(run > 0xffffffffull ? 0xffffffffull : run);
-            run += v[k];
-        }
+        uint64_t off = s_carry;
+        for (int k = 0; k < w; ++k) off += s_w[k];
+        const uint64_t ex = off + incl - v;
+        if (i < nb) sums[i] = (uint32_t)(ex > 0xffffffffull ? 0xffffffffull : ex);
         __syncthreads();
-        if (threadIdx.x == kScanThreads - 1) s_carry = run;
+        if (threadIdx.x == 1023) s_carry = off + incl;
         __syncthreads();
     }
     if (threadIdx.x == 0) *total = (uint32_t)(s_carry > 0xffffffffull ? 0xffffffffull : s_carry);
 }
 
-hipError_t launch_build_scan(uint32_t* batch_counts, int32_t batches, uint32_t* total, hipStream_t s) {
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kScanThreads), 0, s, batch_counts, batches, total);
+__global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data, int32_t n, const uint32_t* sums) {
+    __shared__ uint32_t s_w[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t my = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (my + k < n) ? data[my + k] : 0u;
+        sum += v[k];
+    }
+    uint32_t incl = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) s_w[w] = incl;
+    __syncthreads();
+    uint32_t run = sums[blockIdx.x] + incl - sum;
+    for (int k = 0; k < w; ++k) run += s_w[k];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (my + k < n) data[my + k] = run;
+        run += v[k];
+    }
+}
+
+int scan_scratch_words(int32_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_t* scratch, hipStream_t s) {
+    if (n <= 0) {
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, total, 0u, (int64_t)1);
+        return hipGetLastError();
+    }
+    const int nb = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, scratch, nb, total);
+    hipLaunchKernelGGL(scan_down_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch);
     return hipGetLastError();
 }
 

@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <type_traits>
 #include <vector>
@@ -698,6 +699,44 @@ ORC_API void orc_raster(const int32_t* range, const int32_t* values, const float
 }
 
 ORC_API float orc_exp2(float e) { return exp2_poly(e); }
+
+// 16-bit storage of the precision form -> binary32 (both conversions exact).
+static float half_to_float(uint16_t h) {
+    const int sign = (h >> 15) & 1, e = (h >> 10) & 31, m = h & 1023;
+    float v;
+    if (e == 0) v = std::ldexp((float)m, -24);
+    else if (e == 31) v = m ? std::numeric_limits<float>::quiet_NaN() : std::numeric_limits<float>::infinity();
+    else v = std::ldexp((float)(m | 1024), e - 25);
+    return sign ? -v : v;
+}
+static float bf16_to_float(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float v;
+    std::memcpy(&v, &u, 4);
+    return v;
+}
+
+// Decodes the precision form (position.w = quaternion bits, packed = scale
+// x, y, z + opacity in 16 bits, SH aligned in 16 bits) into the FP32 form that
+// the other oracle functions take; precision 1 = FP16, 2 = BF16.
+ORC_API void orc_decode_precision(const float* position_p, const uint16_t* packed, const uint16_t* sh16, int n,
+                                  int degree, int precision, float* position, float* scale_rot, float* sh_aligned) {
+    auto dec = precision == 1 ? half_to_float : bf16_to_float;
+    for (int i = 0; i < n; ++i) {
+        position[4 * i + 0] = position_p[4 * i + 0];
+        position[4 * i + 1] = position_p[4 * i + 1];
+        position[4 * i + 2] = position_p[4 * i + 2];
+        position[4 * i + 3] = dec(packed[4 * i + 3]);
+        scale_rot[4 * i + 0] = dec(packed[4 * i + 0]);
+        scale_rot[4 * i + 1] = dec(packed[4 * i + 1]);
+        scale_rot[4 * i + 2] = dec(packed[4 * i + 2]);
+        scale_rot[4 * i + 3] = position_p[4 * i + 3];
+    }
+    if (degree > 0) {
+        const size_t total = (size_t)((n + 255) / 256) * 256 * 3 * (degree + 1) * (degree + 1);
+        for (size_t k = 0; k < total; ++k) sh_aligned[k] = dec(sh16[k]);
+    }
+}
 
 // Per-stage wall-clock (ms) of the last orc_render call.
 static double g_stage_ms[6];

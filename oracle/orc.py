@@ -35,6 +35,7 @@ def lib():
             "orc_ranges": (None, [_P, C.c_int64, C.c_int, _P]),
             "orc_raster": (None, [_P, _P, _P, _P, _P, C.c_int, C.c_int, _P]),
             "orc_exp2": (C.c_float, [C.c_float]),
+            "orc_decode_precision": (None, [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
             "orc_last_stage_ms": (None, [_P]),
             "orc_render": (C.c_int64, [_P, _P, _P, C.c_int, C.c_int, _P, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P,
                                        _P, _P, C.c_int64]),
@@ -132,6 +133,17 @@ def raster(rng, vals, xy, con, col, W, H):
     img = np.empty((H, W, 4), np.uint8)
     lib().orc_raster(_p(rng), _p(np.ascontiguousarray(vals, np.int32)), _p(xy), _p(con), _p(col), W, H, _p(img))
     return img
+
+
+def decode_precision(position_p, packed, sh16, n, degree, precision):
+    """Precision-form storage -> FP32 form (position, scale_rot, aligned SH)."""
+    pos = np.empty((n, 4), np.float32)
+    sr = np.empty((n, 4), np.float32)
+    groups = (n + 255) // 256
+    sh = np.empty(groups * 256 * 3 * (degree + 1) ** 2 if degree else 1, np.float32)
+    lib().orc_decode_precision(_p(position_p), _p(packed), _p(sh16), n, degree, precision, _p(pos), _p(sr),
+                               _p(sh))
+    return pos, sr, (sh if degree else None)
 
 
 def exp2(e: float) -> float:

@@ -159,3 +159,46 @@ int main() {
                     f"-L{gr.LIB_PATH.parent}", "-lgr_hip", f"-Wl,-rpath,{gr.LIB_PATH.parent}"], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 0 and r.stdout.strip() == "ok", (r.returncode, r.stdout, r.stderr)
+
+
+def test_half_conversion_matches_numpy():
+    rng = np.random.default_rng(5)
+    vals = np.concatenate([rng.normal(0, 1, 20000), rng.uniform(-70000, 70000, 2000), 10.0 ** rng.uniform(-9, -4, 4000),
+                           [0.0, -0.0, 65504.0, 65519.0, 65520.0, 1e9, -1e9, np.inf, -np.inf, 6.1035156e-05,
+                            5.9604645e-08, 2.9802322e-08, 2.9802326e-08]]).astype(np.float32)
+    lib = gr.lib()
+    got = np.array([lib.gr_float_to_half(float(v)) for v in vals], np.uint16)
+    np.testing.assert_array_equal(got, vals.astype(np.float16).view(np.uint16))
+    assert lib.gr_float_to_half(float("nan")) & 0x7c00 == 0x7c00 and lib.gr_float_to_half(float("nan")) & 0x3ff
+
+
+def test_bf16_conversion_is_round_to_nearest_even():
+    rng = np.random.default_rng(6)
+    vals = np.concatenate([rng.normal(0, 1, 5000), 10.0 ** rng.uniform(-30, 30, 5000)]).astype(np.float32)
+    lib = gr.lib()
+    for v in vals:
+        h = lib.gr_float_to_bf16(float(v))
+        dec = np.array([h << 16], np.uint32).view(np.float32)[0]
+        lo = np.array([(h - 1) << 16], np.uint32).view(np.float32)[0]
+        hi = np.array([(h + 1) << 16], np.uint32).view(np.float32)[0]
+        err = abs(float(dec) - float(v))
+        assert err <= abs(float(lo) - float(v)) and err <= abs(float(hi) - float(v))
+        if err == abs(float(hi) - float(v)) or err == abs(float(lo) - float(v)):
+            assert h % 2 == 0 or err == 0
+
+
+def test_pack_precision_layout(orc):
+    s = gr.generate_scene(2, 1000, 3, 8)
+    for prec in (1, 2):
+        pos, packed, sh16 = gr.pack_precision(s, prec)
+        np.testing.assert_array_equal(pos[:, :3], s["position"][:, :3])
+        np.testing.assert_array_equal(pos[:, 3].view(np.uint32), s["scale_rot"][:, 3].view(np.uint32))
+        p2, sr2, sh2 = orc.decode_precision(pos, packed, sh16, 1000, 3, prec)
+        tol = 2.0 ** -11 if prec == 1 else 2.0 ** -8
+        np.testing.assert_allclose(sr2[:, :3], s["scale_rot"][:, :3], rtol=tol)
+        np.testing.assert_allclose(p2[:, 3], s["position"][:, 3], rtol=tol, atol=1e-7)
+        al = orc.realign_sh(s["sh"], 1000, 48)
+        np.testing.assert_allclose(sh2, al, rtol=tol, atol=1e-6 if prec == 1 else 0)
+    if True:
+        pos, packed, _ = gr.pack_precision(s, 1)
+        np.testing.assert_array_equal(packed[:, :3], s["scale_rot"][:, :3].astype(np.float16).view(np.uint16))

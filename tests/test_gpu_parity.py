@@ -182,3 +182,29 @@ def test_large_frame_properties(torch_cuda, orc):
     assert np.array_equal(kk, ref["keys"])
     assert np.array_equal(img, ref["image"])
     r.close()
+
+
+@pytest.mark.parametrize("precision", [1, 2], ids=["fp16", "bf16"])
+def test_precision_form_parity(torch_cuda, orc, precision):
+    """FP16/BF16 storage, FP32 arithmetic: bit-exact against the oracle on the decoded inputs,
+    and within tolerance of the FP64 render of the original FP32 scene."""
+    say(f"precision form {precision}")
+    scene = gr.generate_scene(2, 60000, 3, 17)
+    W, H = 1920, 1080
+    eye = gr.pose_eye(17, 1, scene["centroid"], 8.0)
+    cam = cam_for(eye, scene["centroid"], FOV50, W, H)
+    r = gr.Renderer()
+    r.upload(scene, precision)
+    img, stats = r.render(cam, W, H)
+    pos_p, packed, sh16 = gr.pack_precision(scene, precision)
+    pos, sr, shal = orc.decode_precision(pos_p, packed, sh16, scene["count"], 3, precision)
+    ref = orc.render(pos, sr, shal, None, scene["count"], 3, cam.packed(), W, H)
+    assert stats["tile_list_size"] == ref["k"]
+    assert np.array_equal(img, ref["image"])
+    shal32 = orc.realign_sh(scene["sh"], scene["count"], 48)
+    r64 = orc.render64(scene["position"], scene["scale_rot"], shal32, scene["color"], scene["count"], 3,
+                       cam.packed(), W, H)
+    p = orc.psnr(img, r64["image"])
+    say(f"precision {precision}: PSNR vs FP64 {p:.2f} dB")
+    assert p >= (50.0 if precision == 1 else 40.0)
+    r.close()
