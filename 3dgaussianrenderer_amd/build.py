@@ -28,7 +28,7 @@ LIB = PKG / "libgr_hip.so"
 HEADLESS = PKG / "gr_headless"
 ORACLE_DIR = ROOT / "oracle"
 
-HIP_SOURCES = ["gr_stages.hip", "gr_sort.hip", "gr_api.hip", "gr_renderer.hip"]
+HIP_SOURCES = ["gr_stages.hip", "gr_sort.hip", "gr_api.hip", "gr_renderer.hip", "gr_server.hip"]
 CXX_SOURCES = ["gr_scene.cpp", "gauss_render.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -93,6 +93,7 @@ SIGNATURES = [
     ("gr_last_error", C.c_char_p, []),
     ("gr_version", C.c_char_p, []),
     ("gr_context_create", C.c_int, [_PP, _P]),
+    ("gr_context_create_ex", C.c_int, [_PP, _P, C.c_int32]),
     ("gr_context_destroy", C.c_int, [_P]),
     ("gr_context_synchronize", C.c_int, [_P]),
     ("gr_timer_create", C.c_int, [_PP]),
@@ -230,8 +231,13 @@ class Context:
     """gr_context: one HIP stream + stage arguments (replaces __constant__ globals)."""
 
     def __init__(self, stream: int | None = None):
+        """stream None: the context creates its own stream; an int (0 = the legacy default stream)
+        is used as given, so work is ordered with other work on that stream."""
         self.h = C.c_void_p()
-        check(lib().gr_context_create(C.byref(self.h), C.c_void_p(stream) if stream else None))
+        if stream is None:
+            check(lib().gr_context_create(C.byref(self.h), None))
+        else:
+            check(lib().gr_context_create_ex(C.byref(self.h), C.c_void_p(stream) if stream else None, 1))
 
     def close(self):
         if self.h:

@@ -92,15 +92,28 @@ static FrameGeom ctx_geom(const gr_context* ctx) {
     return make_geom(ctx->args.width, ctx->args.height);
 }
 
+// The one view of the stage-level API (tile ids start at 0).
+static ViewDesc ctx_view(const gr_context* ctx) {
+    ViewDesc v{};
+    v.cam = camera_args(ctx->args.camera_data);
+    v.geom = ctx_geom(ctx);
+    v.tile_base = 0;
+    v.tiles = v.geom.tiles_x * v.geom.tiles_y;
+    v.out = reinterpret_cast<uchar4*>(ctx->args.back_buffer);
+    return v;
+}
+
 extern "C" {
 
 const char* gr_last_error(void) { return g_last_error.c_str(); }
 const char* gr_version(void) { return "gr-mi355x 0.1 (gfx950)"; }
 
-gr_status gr_context_create(gr_context** out, void* stream) {
+gr_status gr_context_create(gr_context** out, void* stream) { return gr_context_create_ex(out, stream, 0); }
+
+gr_status gr_context_create_ex(gr_context** out, void* stream, int32_t flags) {
     if (!out) return fail(GR_ERR_INVALID_ARGUMENT, "gr_context_create: out is NULL");
     auto* c = new gr_context();
-    if (stream) {
+    if (stream || (flags & GR_CONTEXT_BORROW_STREAM)) {
         c->stream = static_cast<hipStream_t>(stream);
     } else {
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -201,9 +214,11 @@ gr_status gr_evaluate_spherical_harmonics(gr_context* ctx, gr_timer* timer, int3
     a.sh = g.spherical_harmonics;
     a.packed = reinterpret_cast<const uint2*>(g.packed_scale_opacity);
     a.color = reinterpret_cast<float4*>(g.color);
-    a.camera_position = v3{g.camera_data.position[0], g.camera_data.position[1], g.camera_data.position[2]};
     a.count = count;
     a.sh_count = g.spherical_harmonics_count;
+    a.nviews = 1;
+    a.v0 = ctx_view(ctx);
+    a.views = nullptr;
     GR_CHECK(timer_start(ctx, timer));
     GR_HIP(launch_sh(a, g.spherical_harmonics_degree, g.sh_precision, ctx->stream));
     GR_CHECK(timer_stop(ctx, timer));
@@ -227,9 +242,10 @@ gr_status gr_evaluate_splat_clip_data(gr_context* ctx, gr_timer* timer, int32_t 
     a.clip_z = g.position_clip_z;
     a.ellipse = reinterpret_cast<float4*>(g.screen_ellipse);
     a.conic = reinterpret_cast<float4*>(g.conic);
-    a.cam = camera_args(g.camera_data);
-    a.geom = ctx_geom(ctx);
     a.count = count;
+    a.nviews = 1;
+    a.v0 = ctx_view(ctx);
+    a.views = nullptr;
     GR_CHECK(timer_start(ctx, timer));
     GR_HIP(launch_project(a, ctx->stream));
     GR_CHECK(timer_stop(ctx, timer));
@@ -253,10 +269,13 @@ gr_status gr_build_tile_list(gr_context* ctx, gr_timer* timer, int32_t num_block
     a.batch_counts = ctx->batch_counts;
     a.keys = ctx->tl.keys;
     a.values = ctx->tl.values;
-    a.geom = ctx_geom(ctx);
     a.count = n;
     a.capacity = tile_list_capacity;
-    if (a.geom.tiles_x * a.geom.tiles_y >= (1 << 20))
+    a.batches = batches;
+    a.nviews = 1;
+    a.v0 = ctx_view(ctx);
+    a.views = nullptr;
+    if (a.v0.tiles >= (1 << 20))
         return fail(GR_ERR_INVALID_ARGUMENT, "frame has too many tiles");
     GR_CHECK(timer_start(ctx, timer));
     uint32_t total = 0;
@@ -305,7 +324,7 @@ gr_status gr_radix_sort_pairs(gr_context* ctx, int32_t count, int32_t begin_bit,
     int32_t* vb[2] = {values->buffers[0], values->buffers[1]};
     int32_t sel = keys->selector;
     if (values->selector != sel) return fail(GR_ERR_INVALID_ARGUMENT, "key/value selectors differ");
-    GR_HIP(radix_sort_pairs(kb, vb, &sel, count, begin_bit, end_bit, *temp, *temp_bytes, ctx->stream));
+    GR_HIP(radix_sort_pairs(kb, vb, &sel, count, nullptr, begin_bit, end_bit, *temp, *temp_bytes, ctx->stream));
     keys->selector = sel;
     values->selector = sel;
     return GR_OK;
@@ -334,6 +353,7 @@ gr_status gr_evaluate_tile_range(gr_context* ctx, gr_timer* timer, int32_t size)
         RangeArgs a;
         a.keys = ctx->tl.keys;
         a.tile_range = ctx->args.tile_range;
+        a.size_dev = nullptr;
         a.size = size;
         a.tiles = tiles;
         GR_HIP(launch_ranges(a, ctx->stream));
@@ -348,15 +368,17 @@ gr_status gr_rasterize_tile(gr_context* ctx, gr_timer* timer) {
     const gr_global_args& g = ctx->args;
     if (!g.back_buffer || !g.tile_range) return fail(GR_ERR_INVALID_ARGUMENT, "raster buffers are NULL");
     RasterArgs a;
-    a.geom = ctx_geom(ctx);
-    a.tiles = a.geom.tiles_x * a.geom.tiles_y;
+    a.v0 = ctx_view(ctx);
+    a.views = nullptr;
+    a.nviews = 1;
+    a.count = g.splat_count;
+    a.tiles = a.v0.tiles;
     a.tiles_per_xcd = (a.tiles + 7) / 8;
     a.tile_range = g.tile_range;
     a.values = ctx->tl.values;
     a.clip_xy = reinterpret_cast<const float2*>(g.position_clip_xy);
     a.conic = reinterpret_cast<const float4*>(g.conic);
     a.color = reinterpret_cast<const float4*>(g.color);
-    a.out = reinterpret_cast<uchar4*>(g.back_buffer);
     GR_CHECK(timer_start(ctx, timer));
     GR_HIP(launch_raster(a, ctx->stream));
     GR_CHECK(timer_stop(ctx, timer));

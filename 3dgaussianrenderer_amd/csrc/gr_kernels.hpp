@@ -1,7 +1,15 @@
 // gr_kernels.hpp -- kernel argument blocks and launcher declarations shared by
 // the stage kernels (gr_stages.hip), the radix sort (gr_sort.hip) and the host
-// side (gr_api.hip).  Arguments travel by value in the kernarg segment; there
-// is no __constant__ state, so any number of contexts may run concurrently.
+// side (gr_api.hip, gr_renderer.hip, gr_server.hip).  Arguments travel by
+// value in the kernarg segment; there is no __constant__ state, so any number
+// of contexts may run concurrently.
+//
+// Every stage kernel renders a batch of `nviews` views of one scene in one
+// launch.  A single view (the reference's case) travels by value in `v0`; a
+// batch reads its per-view descriptors from the device array `views`.  Per-view
+// buffers are laid out view-major: element i of view v sits at v * count + i.
+// Tile ids of a batch are global: view v owns tiles
+// [views[v].tile_base, views[v].tile_base + views[v].tiles).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,14 +29,28 @@ struct CameraArgs {
     float aspect;
 };
 
+struct ViewDesc {
+    CameraArgs cam;
+    FrameGeom geom;
+    int32_t tile_base;
+    int32_t tiles;
+    uchar4* out;            // RGBA8 image of this view (device)
+};
+
+__host__ __device__ __forceinline__ const ViewDesc& view_of(int v, const ViewDesc& v0, const ViewDesc* views) {
+    return views ? views[v] : v0;
+}
+
 struct ShArgs {
     const float4* position;
-    const void* sh;          // aligned [groups][sh_count][256]
-    const uint2* packed;     // precision form: (scale x, y, z, opacity) 16-bit; else NULL
-    float4* color;
-    v3 camera_position;
+    const void* sh;             // aligned [groups][sh_count][256]
+    const uint2* packed;        // precision form: (scale x, y, z, opacity) 16-bit; else NULL
+    float4* color;              // [nviews][count]
     int32_t count;
-    int32_t sh_count;        // coefficients * 3
+    int32_t sh_count;           // coefficients * 3
+    int32_t nviews;
+    ViewDesc v0;
+    const ViewDesc* views;
 };
 
 struct ProjectArgs {
@@ -36,44 +58,51 @@ struct ProjectArgs {
     const float4* scale_rotation;   // FP32 form
     const uint2* packed;            // precision form (scale + opacity, 16-bit)
     int32_t precision;              // 0 FP32, 1 FP16, 2 BF16
-    float2* clip_xy;
+    float2* clip_xy;                // [nviews][count]
     float* clip_z;
     float4* ellipse;
     float4* conic;
-    CameraArgs cam;
-    FrameGeom geom;
     int32_t count;
+    int32_t nviews;
+    ViewDesc v0;
+    const ViewDesc* views;
 };
 
 struct BuildArgs {
-    const float2* clip_xy;
+    const float2* clip_xy;       // [nviews][count]
     const float* clip_z;
     const float4* ellipse;
-    uint32_t* batch_counts;      // [batches]  (count pass) / exclusive offsets (emit pass)
+    uint32_t* batch_counts;      // [nviews][batches]: counts (count pass) / exclusive offsets (emit pass)
     uint64_t* keys;
-    int32_t* values;
-    FrameGeom geom;
+    int32_t* values;             // Gaussian index within the scene
     int32_t count;
     int32_t capacity;
+    int32_t batches;             // ceil(count / 64)
+    int32_t nviews;
+    ViewDesc v0;
+    const ViewDesc* views;
 };
 
 struct RangeArgs {
     const uint64_t* keys;
     int32_t* tile_range;
-    int32_t size;
+    const uint32_t* size_dev;    // device-side entry count (NULL: use `size`)
+    int32_t size;                // entries, or the upper bound when size_dev is set
     int32_t tiles;
 };
 
 struct RasterArgs {
     const int32_t* tile_range;
     const int32_t* values;
-    const float2* clip_xy;
+    const float2* clip_xy;       // [nviews][count]
     const float4* conic;
     const float4* color;
-    uchar4* out;
-    FrameGeom geom;
-    int32_t tiles;
+    int32_t count;
+    int32_t tiles;               // all views
     int32_t tiles_per_xcd;
+    int32_t nviews;
+    ViewDesc v0;
+    const ViewDesc* views;
 };
 
 // Gaussians per build batch: one wave64 walks the candidate tiles of 64
@@ -97,9 +126,12 @@ hipError_t launch_raster(const RasterArgs& a, hipStream_t stream);
 // radix sort (gr_sort.hip)
 size_t radix_sort_temp_bytes(int32_t count, int32_t begin_bit, int32_t end_bit);
 // Sorts (keys[sel], values[sel]) on bits [begin_bit, end_bit); on return *sel
-// names the buffer pair that holds the sorted data.
+// names the buffer pair that holds the sorted data.  The number of pairs is
+// min(*count_dev, count) when count_dev is not NULL (a device-side count, so
+// the sort can follow a build without a host round trip), else count; grids
+// are sized for `count`.
 hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
-                            int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes,
-                            hipStream_t stream);
+                            const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                            size_t temp_bytes, hipStream_t stream);
 
 }  // namespace gr

@@ -1,0 +1,860 @@
+// gr_server.hip -- continuous-batching render server over resident scenes
+// (include/gr_capi.h, "serving").  No counterpart in the reference, which
+// renders one scene one frame at a time (app/main.cpp:326-472).
+//
+// Threads: callers submit from any thread into a locked queue; one scheduler
+// thread owns every HIP call of the render path.  Each slot is a stream plus
+// a grow-only workspace; a batch (up to max_views requests of one scene, any
+// mix of resolutions) runs on one slot in two phases:
+//   A  camera block H2D -> SH (all views) -> projection (all views) ->
+//      build count -> scan -> 4-byte K readback
+//   B  emit -> onesweep sort (device-side K) -> ranges -> raster (all views)
+//      -> images D2H into pinned staging
+// Between A and B the host reads K and grows the slot's tile-list buffers
+// within the HBM budget (exact admission).  Both phases are captured once
+// per (scene, batch shape, workspace) as hipGraphs and replayed; the camera
+// block is the only per-batch input and travels through a memcpy node from
+// pinned memory.  Several slots run concurrently, so batches of different
+// scenes (or of one scene) overlap on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "../../include/gr_capi.h"
+#include "gr_internal.hpp"
+#include "gr_kernels.hpp"
+
+using namespace gr;
+
+namespace {
+
+gr_status sfail(gr_status st, const std::string& msg) {
+    gr_detail::set_last_error(msg);
+    return st;
+}
+
+#define S_HIP(call)                                                                          \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return sfail(e_ == hipErrorOutOfMemory ? GR_ERR_OUT_OF_MEMORY : GR_ERR_HIP,      \
+                         std::string(#call) + " failed: " + hipGetErrorString(e_));          \
+    } while (0)
+
+#define S_CHECK(st)                  \
+    do {                             \
+        gr_status s_ = (st);         \
+        if (s_ != GR_OK) return s_;  \
+    } while (0)
+
+// device allocation with byte accounting
+struct Dev {
+    void* p = nullptr;
+    size_t bytes = 0;
+    gr_status ensure(size_t need, std::atomic<int64_t>& acct, bool* grew = nullptr) {
+        if (need <= bytes && p) return GR_OK;
+        const size_t want = std::max<size_t>(need + need / 4, 256);
+        if (p) {
+            S_HIP(hipFree(p));
+            acct -= (int64_t)bytes;
+        }
+        p = nullptr;
+        bytes = 0;
+        S_HIP(hipMalloc(&p, want));
+        bytes = want;
+        acct += (int64_t)want;
+        if (grew) *grew = true;
+        return GR_OK;
+    }
+    void release(std::atomic<int64_t>& acct) {
+        if (p) {
+            (void)hipFree(p);
+            acct -= (int64_t)bytes;
+        }
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Pinned {
+    void* p = nullptr;
+    size_t bytes = 0;
+    gr_status ensure(size_t need) {
+        if (need <= bytes && p) return GR_OK;
+        const size_t want = std::max<size_t>(need + need / 4, 256);
+        if (p) S_HIP(hipHostFree(p));
+        p = nullptr;
+        bytes = 0;
+        S_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
+        bytes = want;
+        return GR_OK;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Scene {
+    int32_t id = -1;
+    int32_t n = 0, degree = 0, sh_count = 3, precision = 0;
+    Dev position, scale_rot, packed, sh, color0;
+    int64_t bytes = 0;
+    int inflight = 0;
+    bool removing = false;
+};
+
+struct Pending {
+    uint64_t id;
+    gr_request req;
+    double submit_ms;
+};
+
+enum class SlotState { Idle, PhaseA, WaitMemory, PhaseB };
+
+struct Workspace {
+    Dev color, xy, z, ell, con;      // [V][N]
+    Dev counts;                       // V*batches + 1 + scan scratch
+    Dev keys0, keys1, vals0, vals1;   // tile lists
+    Dev sort_temp;
+    Dev ranges;                       // 2 * tiles
+    Dev images;                       // sum of W*H*4
+    Dev views;                        // ViewDesc[V]
+    Pinned views_host, k_host, img_host;
+    uint64_t generation = 0;          // bumps when any device pointer changes
+};
+
+struct GraphKey {
+    int32_t scene;
+    uint64_t gen;
+    std::vector<int32_t> shape;       // V, then (W, H) per view, then capacity for phase B
+    bool operator<(const GraphKey& o) const {
+        return std::tie(scene, gen, shape) < std::tie(o.scene, o.gen, o.shape);
+    }
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    Workspace ws;
+    SlotState state = SlotState::Idle;
+    std::vector<Pending> batch;
+    int32_t scene = -1;
+    int32_t tiles_total = 0;
+    int32_t end_bit = 32;
+    int64_t img_bytes = 0;
+    uint32_t k = 0;
+    uint32_t capacity = 0;
+    double start_ms = 0.0;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+};
+
+}  // namespace
+
+struct gr_server {
+    gr_server_config cfg{};
+    int device = 0;
+    std::chrono::steady_clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv_submit, cv_done;
+    std::deque<Pending> incoming;
+    std::deque<gr_completion> done;
+    std::map<int32_t, Scene*> scenes;
+    int32_t next_scene = 0;
+    uint64_t next_request = 1;
+    std::atomic<int64_t> resident{0}, workspace{0};
+    std::atomic<int64_t> peak{0}, batches{0}, views{0}, graph_launches{0}, budget_waits{0};
+    std::vector<Slot> slots;
+    std::map<int32_t, std::deque<Pending>> queues;   // scheduler-owned
+    std::thread worker;
+    std::atomic<bool> stop{false};
+    std::string fatal;
+
+    double now_ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    int64_t used() const { return resident.load() + workspace.load(); }
+    void note_peak() {
+        const int64_t u = used();
+        int64_t p = peak.load();
+        while (u > p && !peak.compare_exchange_weak(p, u)) {
+        }
+    }
+    bool over_budget(int64_t extra) const {
+        return cfg.hbm_budget_bytes > 0 && used() + extra > cfg.hbm_budget_bytes;
+    }
+};
+
+namespace {
+
+CameraArgs cam_args(const gr_camera_data& c) {
+    CameraArgs a;
+    a.view = load_m4(c.view);
+    a.fov_cotangent = v2{c.fov_cotangent[0], c.fov_cotangent[1]};
+    a.depth_scale_bias = v2{c.depth_scale_bias[0], c.depth_scale_bias[1]};
+    a.position = v3{c.position[0], c.position[1], c.position[2]};
+    a.aspect = c.aspect;
+    return a;
+}
+
+Scene* scene_ptr(gr_server* s, int32_t id) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->scenes.find(id);
+    return it == s->scenes.end() ? nullptr : it->second;
+}
+
+void complete(gr_server* s, Slot& sl, int32_t status) {
+    const double t = s->now_ms();
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (const Pending& p : sl.batch) {
+        gr_completion c{};
+        c.request_id = p.id;
+        c.user_tag = p.req.user_tag;
+        c.status = status;
+        c.batch_views = (int32_t)sl.batch.size();
+        c.submit_ms = p.submit_ms;
+        c.start_ms = sl.start_ms;
+        c.finish_ms = t;
+        c.tile_list_size = sl.k;
+        s->done.push_back(c);
+    }
+    auto it = s->scenes.find(sl.scene);
+    if (it != s->scenes.end()) it->second->inflight -= (int)sl.batch.size();
+    s->cv_done.notify_all();
+}
+
+int64_t per_view_bytes(const Scene& sc) { return (int64_t)sc.n * (16 + 8 + 4 + 16 + 16); }
+
+// Bytes phase A would add to the slot's workspace for this batch.
+int64_t phase_a_growth(const Slot& sl, const Scene& sc, const std::vector<Pending>& b, int64_t img_bytes,
+                       int32_t tiles_total) {
+    const int64_t V = (int64_t)b.size(), N = sc.n;
+    const int64_t batches = (N + kBuildBatch - 1) / kBuildBatch;
+    auto need = [](const Dev& d, int64_t bytes) { return d.p && (int64_t)d.bytes >= bytes ? 0 : bytes + bytes / 4; };
+    return need(sl.ws.color, V * N * 16) + need(sl.ws.xy, V * N * 8) + need(sl.ws.z, V * N * 4) +
+           need(sl.ws.ell, V * N * 16) + need(sl.ws.con, V * N * 16) +
+           need(sl.ws.counts, 4 * (V * batches + 2 + scan_scratch_words((int32_t)(V * batches)))) +
+           need(sl.ws.images, img_bytes) + need(sl.ws.ranges, 8 * (int64_t)tiles_total) +
+           need(sl.ws.views, V * (int64_t)sizeof(ViewDesc));
+}
+
+gr_status ensure_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
+    Workspace& w = sl.ws;
+    const size_t V = sl.batch.size(), N = (size_t)sc.n;
+    const size_t batches = (N + kBuildBatch - 1) / kBuildBatch;
+    bool grew = false;
+    S_CHECK(w.color.ensure(V * N * 16, s->workspace, &grew));
+    S_CHECK(w.xy.ensure(V * N * 8, s->workspace, &grew));
+    S_CHECK(w.z.ensure(V * N * 4, s->workspace, &grew));
+    S_CHECK(w.ell.ensure(V * N * 16, s->workspace, &grew));
+    S_CHECK(w.con.ensure(V * N * 16, s->workspace, &grew));
+    S_CHECK(w.counts.ensure(4 * (V * batches + 2 + scan_scratch_words((int32_t)(V * batches))), s->workspace,
+                            &grew));
+    S_CHECK(w.images.ensure((size_t)sl.img_bytes, s->workspace, &grew));
+    S_CHECK(w.ranges.ensure(8 * (size_t)sl.tiles_total, s->workspace, &grew));
+    S_CHECK(w.views.ensure(V * sizeof(ViewDesc), s->workspace, &grew));
+    S_CHECK(w.views_host.ensure(V * sizeof(ViewDesc)));
+    S_CHECK(w.k_host.ensure(64));
+    S_CHECK(w.img_host.ensure((size_t)sl.img_bytes));
+    if (grew) ++w.generation;
+    s->note_peak();
+    return GR_OK;
+}
+
+// Launch sequences (shared by eager runs and graph capture).
+gr_status enqueue_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
+    Workspace& w = sl.ws;
+    const int V = (int)sl.batch.size();
+    const int N = sc.n;
+    const int batches = (N + kBuildBatch - 1) / kBuildBatch;
+    ViewDesc* vdev = w.views.as<ViewDesc>();
+    const ViewDesc* vhost = w.views_host.as<ViewDesc>();
+    S_HIP(hipMemcpyAsync(vdev, vhost, V * sizeof(ViewDesc), hipMemcpyHostToDevice, sl.stream));
+    if (sc.degree > 0) {
+        ShArgs a{};
+        a.position = sc.position.as<float4>();
+        a.sh = sc.sh.p;
+        a.packed = sc.packed.as<uint2>();
+        a.color = w.color.as<float4>();
+        a.count = N;
+        a.sh_count = sc.sh_count;
+        a.nviews = V;
+        a.v0 = vhost[0];
+        a.views = vdev;
+        S_HIP(launch_sh(a, sc.degree, sc.precision, sl.stream));
+    } else {
+        for (int v = 0; v < V; ++v)
+            S_HIP(hipMemcpyAsync(w.color.as<float4>() + (size_t)v * N, sc.color0.p, (size_t)N * 16,
+                                 hipMemcpyDeviceToDevice, sl.stream));
+    }
+    ProjectArgs p{};
+    p.position = sc.position.as<float4>();
+    p.scale_rotation = sc.scale_rot.as<float4>();
+    p.packed = sc.packed.as<uint2>();
+    p.precision = sc.precision;
+    p.clip_xy = w.xy.as<float2>();
+    p.clip_z = w.z.as<float>();
+    p.ellipse = w.ell.as<float4>();
+    p.conic = w.con.as<float4>();
+    p.count = N;
+    p.nviews = V;
+    p.v0 = vhost[0];
+    p.views = vdev;
+    S_HIP(launch_project(p, sl.stream));
+    BuildArgs b{};
+    b.clip_xy = w.xy.as<float2>();
+    b.clip_z = w.z.as<float>();
+    b.ellipse = w.ell.as<float4>();
+    b.batch_counts = w.counts.as<uint32_t>();
+    b.count = N;
+    b.capacity = 0;
+    b.batches = batches;
+    b.nviews = V;
+    b.v0 = vhost[0];
+    b.views = vdev;
+    S_HIP(launch_build_count(b, sl.stream));
+    uint32_t* counts = w.counts.as<uint32_t>();
+    const int nb = V * batches;
+    S_HIP(launch_build_scan(counts, nb, counts + nb, counts + nb + 2, sl.stream));
+    S_HIP(hipMemcpyAsync(w.k_host.p, counts + nb, 4, hipMemcpyDeviceToHost, sl.stream));
+    return GR_OK;
+}
+
+gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
+    Workspace& w = sl.ws;
+    const int V = (int)sl.batch.size();
+    const int N = sc.n;
+    const int batches = (N + kBuildBatch - 1) / kBuildBatch;
+    const ViewDesc* vhost = w.views_host.as<ViewDesc>();
+    ViewDesc* vdev = w.views.as<ViewDesc>();
+    uint32_t* counts = w.counts.as<uint32_t>();
+    const int nb = V * batches;
+    BuildArgs b{};
+    b.clip_xy = w.xy.as<float2>();
+    b.clip_z = w.z.as<float>();
+    b.ellipse = w.ell.as<float4>();
+    b.batch_counts = counts;
+    b.keys = w.keys0.as<uint64_t>();
+    b.values = w.vals0.as<int32_t>();
+    b.count = N;
+    b.capacity = (int32_t)sl.capacity;
+    b.batches = batches;
+    b.nviews = V;
+    b.v0 = vhost[0];
+    b.views = vdev;
+    S_HIP(launch_build_emit(b, sl.stream));
+    uint64_t* kb[2] = {w.keys0.as<uint64_t>(), w.keys1.as<uint64_t>()};
+    int32_t* vb[2] = {w.vals0.as<int32_t>(), w.vals1.as<int32_t>()};
+    int32_t sel = 0;
+    S_HIP(radix_sort_pairs(kb, vb, &sel, (int32_t)sl.capacity, counts + nb, 0, sl.end_bit, w.sort_temp.p,
+                           w.sort_temp.bytes, sl.stream));
+    S_HIP(launch_fill_i32(w.ranges.as<int32_t>(), -1, 2 * (int64_t)sl.tiles_total, sl.stream));
+    RangeArgs r{};
+    r.keys = kb[sel];
+    r.tile_range = w.ranges.as<int32_t>();
+    r.size_dev = counts + nb;
+    r.size = (int32_t)sl.capacity;
+    r.tiles = sl.tiles_total;
+    S_HIP(launch_ranges(r, sl.stream));
+    RasterArgs ra{};
+    ra.tile_range = w.ranges.as<int32_t>();
+    ra.values = vb[sel];
+    ra.clip_xy = w.xy.as<float2>();
+    ra.conic = w.con.as<float4>();
+    ra.color = w.color.as<float4>();
+    ra.count = N;
+    ra.tiles = sl.tiles_total;
+    ra.tiles_per_xcd = (sl.tiles_total + 7) / 8;
+    ra.nviews = V;
+    ra.v0 = vhost[0];
+    ra.views = vdev;
+    S_HIP(launch_raster(ra, sl.stream));
+    S_HIP(hipMemcpyAsync(w.img_host.p, w.images.p, (size_t)sl.img_bytes, hipMemcpyDeviceToHost, sl.stream));
+    return GR_OK;
+}
+
+GraphKey graph_key(const Slot& sl, bool phase_b) {
+    GraphKey k;
+    k.scene = sl.scene;
+    k.gen = sl.ws.generation;
+    k.shape.push_back(phase_b ? 1 : 0);
+    k.shape.push_back((int32_t)sl.batch.size());
+    for (const Pending& p : sl.batch) {
+        k.shape.push_back(p.req.width);
+        k.shape.push_back(p.req.height);
+    }
+    if (phase_b) k.shape.push_back((int32_t)sl.capacity);
+    return k;
+}
+
+void drop_graphs(Slot& sl) {
+    for (auto& kv : sl.graphs) (void)hipGraphExecDestroy(kv.second);
+    sl.graphs.clear();
+}
+
+gr_status run_phase(gr_server* s, Slot& sl, const Scene& sc, bool phase_b) {
+    auto enqueue = [&]() { return phase_b ? enqueue_phase_b(s, sl, sc) : enqueue_phase_a(s, sl, sc); };
+    if (!s->cfg.use_graphs) return enqueue();
+    const GraphKey key = graph_key(sl, phase_b);
+    auto it = sl.graphs.find(key);
+    if (it == sl.graphs.end()) {
+        S_HIP(hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal));
+        gr_status st = enqueue();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(sl.stream, &g);
+        if (st != GR_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return sfail(GR_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        hipGraphExec_t ex = nullptr;
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return sfail(GR_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        if (sl.graphs.size() > 64) drop_graphs(sl);
+        it = sl.graphs.emplace(key, ex).first;
+    }
+    S_HIP(hipGraphLaunch(it->second, sl.stream));
+    ++s->graph_launches;
+    return GR_OK;
+}
+
+// Takes up to max_views pending requests of the scene with the oldest request.
+bool form_batch(gr_server* s, Slot& sl) {
+    int32_t best = -1;
+    double oldest = 0.0;
+    for (auto& kv : s->queues) {
+        if (kv.second.empty()) continue;
+        if (best < 0 || kv.second.front().submit_ms < oldest) {
+            best = kv.first;
+            oldest = kv.second.front().submit_ms;
+        }
+    }
+    if (best < 0) return false;
+    auto& q = s->queues[best];
+    sl.batch.clear();
+    while (!q.empty() && (int)sl.batch.size() < s->cfg.max_views_per_batch) {
+        sl.batch.push_back(q.front());
+        q.pop_front();
+    }
+    sl.scene = best;
+    return true;
+}
+
+void requeue_front(gr_server* s, Slot& sl) {
+    auto& q = s->queues[sl.scene];
+    for (auto it = sl.batch.rbegin(); it != sl.batch.rend(); ++it) q.push_front(*it);
+    sl.batch.clear();
+    sl.state = SlotState::Idle;
+}
+
+bool other_slots_busy(gr_server* s, const Slot& me) {
+    for (const Slot& o : s->slots)
+        if (&o != &me && o.state != SlotState::Idle) return true;
+    return false;
+}
+
+void release_idle_workspaces(gr_server* s, const Slot& me) {
+    for (Slot& o : s->slots) {
+        if (&o == &me || o.state != SlotState::Idle) continue;
+        Workspace& w = o.ws;
+        for (Dev* d : {&w.color, &w.xy, &w.z, &w.ell, &w.con, &w.counts, &w.keys0, &w.keys1, &w.vals0,
+                       &w.vals1, &w.sort_temp, &w.ranges, &w.images, &w.views})
+            d->release(s->workspace);
+        ++w.generation;
+        drop_graphs(o);
+    }
+}
+
+// Starts phase A of a new batch on an idle slot; returns true if it launched.
+bool start_batch(gr_server* s, Slot& sl) {
+    if (!form_batch(s, sl)) return false;
+    Scene* sc = scene_ptr(s, sl.scene);
+    if (!sc) {
+        gr_detail::set_last_error("scene was removed");
+        complete(s, sl, GR_ERR_NOT_FOUND);
+        sl.batch.clear();
+        return false;
+    }
+    // per-view geometry, global tile numbering
+    int32_t tiles = 0;
+    int64_t img = 0;
+    std::vector<ViewDesc> vds(sl.batch.size());
+    for (size_t v = 0; v < sl.batch.size(); ++v) {
+        const gr_request& r = sl.batch[v].req;
+        ViewDesc& d = vds[v];
+        d.cam = cam_args(r.camera);
+        d.geom = make_geom(r.width, r.height);
+        d.tile_base = tiles;
+        d.tiles = d.geom.tiles_x * d.geom.tiles_y;
+        d.out = nullptr;
+        tiles += d.tiles;
+        img += (int64_t)r.width * r.height * 4;
+    }
+    sl.tiles_total = tiles;
+    sl.img_bytes = img;
+    sl.end_bit = 32 + tile_bits_for(tiles);
+    const int64_t growth = phase_a_growth(sl, *sc, sl.batch, img, tiles);
+    if (s->over_budget(growth)) {
+        if (other_slots_busy(s, sl)) {
+            ++s->budget_waits;
+            requeue_front(s, sl);
+            return false;
+        }
+        release_idle_workspaces(s, sl);
+        while (s->over_budget(phase_a_growth(sl, *sc, sl.batch, img, tiles)) && sl.batch.size() > 1) {
+            // halve the batch: the tail goes back to the queue
+            const size_t keep = sl.batch.size() / 2;
+            auto& q = s->queues[sl.scene];
+            for (size_t i = sl.batch.size(); i-- > keep;) q.push_front(sl.batch[i]);
+            sl.batch.resize(keep);
+            vds.resize(keep);
+            tiles = 0;
+            img = 0;
+            for (size_t v = 0; v < keep; ++v) {
+                vds[v].tile_base = tiles;
+                tiles += vds[v].tiles;
+                img += (int64_t)sl.batch[v].req.width * sl.batch[v].req.height * 4;
+            }
+            sl.tiles_total = tiles;
+            sl.img_bytes = img;
+            sl.end_bit = 32 + tile_bits_for(tiles);
+        }
+        if (s->over_budget(phase_a_growth(sl, *sc, sl.batch, img, tiles))) {
+            gr_detail::set_last_error("request does not fit the HBM budget");
+            sl.k = 0;
+            sl.start_ms = s->now_ms();
+            complete(s, sl, GR_ERR_BUDGET);
+            sl.batch.clear();
+            return false;
+        }
+    }
+    if (ensure_phase_a(s, sl, *sc) != GR_OK) {
+        complete(s, sl, GR_ERR_OUT_OF_MEMORY);
+        sl.batch.clear();
+        return false;
+    }
+    int64_t off = 0;
+    for (size_t v = 0; v < vds.size(); ++v) {
+        vds[v].out = reinterpret_cast<uchar4*>(sl.ws.images.as<uint8_t>() + off);
+        off += (int64_t)sl.batch[v].req.width * sl.batch[v].req.height * 4;
+    }
+    std::memcpy(sl.ws.views_host.p, vds.data(), vds.size() * sizeof(ViewDesc));
+    sl.start_ms = s->now_ms();
+    sl.k = 0;
+    if (run_phase(s, sl, *sc, false) != GR_OK || hipEventRecord(sl.ev_a, sl.stream) != hipSuccess) {
+        complete(s, sl, GR_ERR_HIP);
+        sl.batch.clear();
+        return false;
+    }
+    sl.state = SlotState::PhaseA;
+    ++s->batches;
+    s->views += (int64_t)sl.batch.size();
+    return true;
+}
+
+// Phase A finished: size the tile list and launch phase B.  Returns false if
+// the slot has to wait for memory.
+bool start_phase_b(gr_server* s, Slot& sl) {
+    Scene* sc = scene_ptr(s, sl.scene);   // pinned by the batch's in-flight count
+    Workspace& w = sl.ws;
+    sl.k = w.k_host.as<uint32_t>()[0];
+    if (sl.k >= (1u << 30)) {
+        gr_detail::set_last_error("tile list of a batch exceeds 2^30 entries");
+        complete(s, sl, GR_ERR_CAPACITY);
+        sl.batch.clear();
+        sl.state = SlotState::Idle;
+        return true;
+    }
+    const size_t need = std::max<size_t>(sl.k, 1);
+    const bool have = w.keys0.p && w.keys0.bytes >= need * 8 && w.vals0.bytes >= need * 4;
+    if (!have) {
+        const size_t cap = need + need / 4;
+        const int64_t sort_bytes = (int64_t)radix_sort_temp_bytes((int32_t)cap, 0, sl.end_bit);
+        const int64_t grow = (int64_t)cap * 24 + sort_bytes;
+        if (s->over_budget(grow)) {
+            if (other_slots_busy(s, sl)) {
+                ++s->budget_waits;
+                sl.state = SlotState::WaitMemory;
+                return false;
+            }
+            release_idle_workspaces(s, sl);
+            if (s->over_budget(grow)) {
+                gr_detail::set_last_error("tile list does not fit the HBM budget");
+                complete(s, sl, GR_ERR_BUDGET);
+                sl.batch.clear();
+                sl.state = SlotState::Idle;
+                return true;
+            }
+        }
+        bool grew = false;
+        if (w.keys0.ensure(cap * 8, s->workspace, &grew) != GR_OK ||
+            w.keys1.ensure(cap * 8, s->workspace, &grew) != GR_OK ||
+            w.vals0.ensure(cap * 4, s->workspace, &grew) != GR_OK ||
+            w.vals1.ensure(cap * 4, s->workspace, &grew) != GR_OK) {
+            complete(s, sl, GR_ERR_OUT_OF_MEMORY);
+            sl.batch.clear();
+            sl.state = SlotState::Idle;
+            return true;
+        }
+        if (grew) ++w.generation;
+        s->note_peak();
+    }
+    sl.capacity = (uint32_t)std::min<size_t>(w.keys0.bytes / 8, w.vals0.bytes / 4);
+    sl.capacity = std::min<uint32_t>(sl.capacity, (1u << 30) - 1);
+    const size_t sort_bytes = radix_sort_temp_bytes((int32_t)sl.capacity, 0, sl.end_bit);
+    bool grew = false;
+    if (w.sort_temp.ensure(sort_bytes, s->workspace, &grew) != GR_OK) {
+        complete(s, sl, GR_ERR_OUT_OF_MEMORY);
+        sl.batch.clear();
+        sl.state = SlotState::Idle;
+        return true;
+    }
+    if (grew) ++w.generation;
+    if (run_phase(s, sl, *sc, true) != GR_OK || hipEventRecord(sl.ev_b, sl.stream) != hipSuccess) {
+        complete(s, sl, GR_ERR_HIP);
+        sl.batch.clear();
+        sl.state = SlotState::Idle;
+        return true;
+    }
+    sl.state = SlotState::PhaseB;
+    return true;
+}
+
+void finish_batch(gr_server* s, Slot& sl) {
+    int64_t off = 0;
+    for (const Pending& p : sl.batch) {
+        const size_t bytes = (size_t)p.req.width * p.req.height * 4;
+        if (p.req.out_rgba8) std::memcpy(p.req.out_rgba8, sl.ws.img_host.as<uint8_t>() + off, bytes);
+        off += (int64_t)bytes;
+    }
+    complete(s, sl, GR_OK);
+    sl.batch.clear();
+    sl.state = SlotState::Idle;
+}
+
+void scheduler(gr_server* s) {
+    (void)hipSetDevice(s->device);
+    while (!s->stop.load()) {
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            bool busy = false;
+            for (const Slot& sl : s->slots) busy |= sl.state != SlotState::Idle;
+            bool queued = false;
+            for (auto& kv : s->queues) queued |= !kv.second.empty();
+            if (!busy && !queued && s->incoming.empty())
+                s->cv_submit.wait_for(lk, std::chrono::milliseconds(2));
+            while (!s->incoming.empty()) {
+                Pending p = s->incoming.front();
+                s->incoming.pop_front();
+                s->queues[p.req.scene].push_back(p);
+            }
+        }
+        bool progress = false;
+        for (Slot& sl : s->slots) {
+            if (sl.state == SlotState::PhaseA && hipEventQuery(sl.ev_a) == hipSuccess) {
+                progress |= start_phase_b(s, sl);
+            } else if (sl.state == SlotState::WaitMemory) {
+                progress |= start_phase_b(s, sl);
+            } else if (sl.state == SlotState::PhaseB && hipEventQuery(sl.ev_b) == hipSuccess) {
+                finish_batch(s, sl);
+                progress = true;
+            }
+        }
+        for (Slot& sl : s->slots)
+            if (sl.state == SlotState::Idle) progress |= start_batch(s, sl);
+        if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+gr_status gr_server_create(gr_server** out, const gr_server_config* cfg) {
+    if (!out || !cfg || cfg->slots < 1 || cfg->slots > 16 || cfg->max_views_per_batch < 1 ||
+        cfg->max_views_per_batch > 1024 || cfg->hbm_budget_bytes < 0)
+        return sfail(GR_ERR_INVALID_ARGUMENT, "gr_server_create: bad configuration");
+    if (cfg->device >= 0) S_HIP(hipSetDevice(cfg->device));
+    auto* s = new gr_server();
+    s->cfg = *cfg;
+    S_HIP(hipGetDevice(&s->device));
+    s->t0 = std::chrono::steady_clock::now();
+    s->slots.resize(cfg->slots);
+    for (Slot& sl : s->slots) {
+        S_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+        S_HIP(hipEventCreateWithFlags(&sl.ev_a, hipEventDisableTiming));
+        S_HIP(hipEventCreateWithFlags(&sl.ev_b, hipEventDisableTiming));
+    }
+    s->worker = std::thread(scheduler, s);
+    *out = s;
+    return GR_OK;
+}
+
+gr_status gr_server_destroy(gr_server* s) {
+    if (!s) return GR_OK;
+    s->stop = true;
+    s->cv_submit.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->device);
+    for (Slot& sl : s->slots) {
+        (void)hipStreamSynchronize(sl.stream);
+        drop_graphs(sl);
+        Workspace& w = sl.ws;
+        for (Dev* d : {&w.color, &w.xy, &w.z, &w.ell, &w.con, &w.counts, &w.keys0, &w.keys1, &w.vals0,
+                       &w.vals1, &w.sort_temp, &w.ranges, &w.images, &w.views})
+            d->release(s->workspace);
+        w.views_host.release();
+        w.k_host.release();
+        w.img_host.release();
+        (void)hipEventDestroy(sl.ev_a);
+        (void)hipEventDestroy(sl.ev_b);
+        (void)hipStreamDestroy(sl.stream);
+    }
+    for (auto& kv : s->scenes) {
+        Scene* sc = kv.second;
+        for (Dev* d : {&sc->position, &sc->scale_rot, &sc->packed, &sc->sh, &sc->color0}) d->release(s->resident);
+        delete sc;
+    }
+    delete s;
+    return GR_OK;
+}
+
+gr_status gr_server_add_scene(gr_server* s, const gr_model_host* m, int32_t* scene_id) {
+    if (!s || !m || !scene_id || m->num_gaussians <= 0 || m->sh_degree < 0 || m->sh_degree > 3 ||
+        m->precision < 0 || m->precision > 2 || !m->position || !m->scale_and_rotation ||
+        (m->sh_degree > 0 && !m->sh) || (m->sh_degree == 0 && !m->color))
+        return sfail(GR_ERR_INVALID_ARGUMENT, "gr_server_add_scene: bad model");
+    (void)hipSetDevice(s->device);
+    auto sc = std::make_unique<Scene>();
+    const int n = m->num_gaussians;
+    sc->n = n;
+    sc->degree = m->sh_degree;
+    sc->sh_count = 3 * (m->sh_degree + 1) * (m->sh_degree + 1);
+    sc->precision = m->precision;
+    const size_t groups = ((size_t)n + kShGroup - 1) / kShGroup;
+    const size_t shn = groups * kShGroup * (size_t)sc->sh_count;
+    int64_t bytes = (int64_t)n * 16 + (m->precision ? (int64_t)n * 8 : (int64_t)n * 16) +
+                    (m->sh_degree > 0 ? (int64_t)shn * (m->precision ? 2 : 4) : (int64_t)n * 16);
+    if (s->over_budget(bytes)) return sfail(GR_ERR_BUDGET, "scene does not fit the HBM budget");
+    std::atomic<int64_t>& acct = s->resident;
+    if (m->precision) {
+        std::vector<float> pos((size_t)4 * n);
+        std::vector<uint16_t> pk((size_t)4 * n);
+        std::vector<uint16_t> sh16(m->sh_degree > 0 ? shn : 1);
+        S_CHECK(gr_pack_precision_model(m, m->precision, pos.data(), pk.data(), sh16.data()));
+        S_CHECK(sc->position.ensure((size_t)n * 16, acct));
+        S_CHECK(sc->packed.ensure((size_t)n * 8, acct));
+        S_HIP(hipMemcpy(sc->position.p, pos.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+        S_HIP(hipMemcpy(sc->packed.p, pk.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+        if (m->sh_degree > 0) {
+            S_CHECK(sc->sh.ensure(shn * 2, acct));
+            S_HIP(hipMemcpy(sc->sh.p, sh16.data(), shn * 2, hipMemcpyHostToDevice));
+        }
+    } else {
+        S_CHECK(sc->position.ensure((size_t)n * 16, acct));
+        S_CHECK(sc->scale_rot.ensure((size_t)n * 16, acct));
+        S_HIP(hipMemcpy(sc->position.p, m->position, (size_t)n * 16, hipMemcpyHostToDevice));
+        S_HIP(hipMemcpy(sc->scale_rot.p, m->scale_and_rotation, (size_t)n * 16, hipMemcpyHostToDevice));
+        if (m->sh_degree > 0) {
+            std::vector<float> al(shn);
+            S_CHECK(gr_realign_spherical_harmonics(m->sh, al.data(), kShGroup, sc->sh_count, n));
+            S_CHECK(sc->sh.ensure(shn * 4, acct));
+            S_HIP(hipMemcpy(sc->sh.p, al.data(), shn * 4, hipMemcpyHostToDevice));
+        }
+    }
+    if (m->sh_degree == 0) {
+        S_CHECK(sc->color0.ensure((size_t)n * 16, acct));
+        S_HIP(hipMemcpy(sc->color0.p, m->color, (size_t)n * 16, hipMemcpyHostToDevice));
+    }
+    s->note_peak();
+    std::lock_guard<std::mutex> lk(s->mu);
+    sc->id = s->next_scene++;
+    sc->bytes = bytes;
+    *scene_id = sc->id;
+    s->scenes[sc->id] = sc.release();
+    return GR_OK;
+}
+
+gr_status gr_server_remove_scene(gr_server* s, int32_t id) {
+    if (!s) return sfail(GR_ERR_INVALID_ARGUMENT, "server is NULL");
+    Scene* sc = nullptr;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->scenes.find(id);
+        if (it == s->scenes.end()) return sfail(GR_ERR_NOT_FOUND, "no such scene");
+        sc = it->second;
+        sc->removing = true;
+        s->cv_done.wait(lk, [&] { return sc->inflight == 0; });
+        s->scenes.erase(it);
+    }
+    // graphs are keyed by scene id and ids are never reused, so stale graphs are never replayed
+    for (Dev* d : {&sc->position, &sc->scale_rot, &sc->packed, &sc->sh, &sc->color0}) d->release(s->resident);
+    delete sc;
+    return GR_OK;
+}
+
+gr_status gr_server_submit(gr_server* s, const gr_request* r, uint64_t* request_id) {
+    if (!s || !r || r->width <= 0 || r->height <= 0 || r->width > 16384 || r->height > 16384)
+        return sfail(GR_ERR_INVALID_ARGUMENT, "gr_server_submit: bad request");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->scenes.find(r->scene);
+    if (it == s->scenes.end() || it->second->removing) return sfail(GR_ERR_NOT_FOUND, "no such scene");
+    const FrameGeom g = make_geom(r->width, r->height);
+    if ((int64_t)g.tiles_x * g.tiles_y * s->cfg.max_views_per_batch >= (1 << 24))
+        return sfail(GR_ERR_INVALID_ARGUMENT, "frame too large for the batch tile space");
+    Pending p;
+    p.id = s->next_request++;
+    p.req = *r;
+    p.submit_ms = s->now_ms();
+    it->second->inflight += 1;
+    s->incoming.push_back(p);
+    if (request_id) *request_id = p.id;
+    s->cv_submit.notify_one();
+    return GR_OK;
+}
+
+gr_status gr_server_poll(gr_server* s, gr_completion* out, int32_t max, int32_t* n_out, int32_t wait_ms) {
+    if (!s || !n_out || (max > 0 && !out)) return sfail(GR_ERR_INVALID_ARGUMENT, "gr_server_poll: bad argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->done.empty() && wait_ms > 0)
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), [&] { return !s->done.empty(); });
+    int32_t n = 0;
+    while (n < max && !s->done.empty()) {
+        out[n++] = s->done.front();
+        s->done.pop_front();
+    }
+    *n_out = n;
+    return GR_OK;
+}
+
+gr_status gr_server_get_stats(gr_server* s, gr_server_stats* o) {
+    if (!s || !o) return sfail(GR_ERR_INVALID_ARGUMENT, "gr_server_get_stats: bad argument");
+    o->resident_bytes = s->resident.load();
+    o->workspace_bytes = s->workspace.load();
+    o->peak_bytes = s->peak.load();
+    o->batches = s->batches.load();
+    o->views = s->views.load();
+    o->graph_launches = s->graph_launches.load();
+    o->budget_waits = s->budget_waits.load();
+    return GR_OK;
+}
+
+double gr_server_clock_ms(gr_server* s) { return s ? s->now_ms() : 0.0; }
+
+}  // extern "C"

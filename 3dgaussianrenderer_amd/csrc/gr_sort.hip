@@ -92,9 +92,10 @@ constexpr int kHistSeg = 16;
 constexpr int kHistChunk = kHistThreads * kHistSeg;
 
 __global__ __launch_bounds__(kHistThreads) void sort_hist_kernel(const uint64_t* __restrict__ keys, int32_t n,
-                                                                 int begin_bit, int end_bit, int passes,
-                                                                 uint32_t* hist) {
+                                                                 const uint32_t* count_dev, int begin_bit,
+                                                                 int end_bit, int passes, uint32_t* hist) {
     __shared__ uint32_t s_h[kMaxPasses][kRadix];
+    if (count_dev) n = (int32_t)min((uint32_t)n, *count_dev);
     __shared__ uint64_t s_k[kHistChunk + kHistChunk / kHistSeg];
     for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += blockDim.x) (&s_h[0][0])[i] = 0u;
     for (int64_t base = (int64_t)blockIdx.x * kHistChunk; base < n; base += (int64_t)gridDim.x * kHistChunk) {
@@ -169,8 +170,8 @@ __device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
 
 __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-    int32_t* __restrict__ vout, int32_t n, int shift, int bits, const uint32_t* __restrict__ digit_off,
-    uint32_t* status, uint32_t* status_next, uint32_t* ticket) {
+    int32_t* __restrict__ vout, int32_t n, const uint32_t* count_dev, int shift, int bits,
+    const uint32_t* __restrict__ digit_off, uint32_t* status, uint32_t* status_next, uint32_t* ticket) {
     __shared__ uint32_t s_part;
     __shared__ uint32_t s_wcnt[kSortWaves][kRadix];
     __shared__ uint32_t s_dstart[kRadix];
@@ -184,6 +185,8 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     for (int i = t; i < kSortWaves * kRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0u;
     __syncthreads();
     const int part = (int)s_part;
+    if (count_dev) n = (int32_t)min((uint32_t)n, *count_dev);
+    if ((int64_t)part * kSortTile >= n) return;   // beyond the device-side count: whole workgroup
     if (status_next && t < kRadix) status_next[(size_t)part * kRadix + t] = 0u;
 
     const int64_t base = (int64_t)part * kSortTile;
@@ -295,8 +298,8 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
 }
 
 hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
-                            int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes,
-                            hipStream_t s) {
+                            const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                            size_t temp_bytes, hipStream_t s) {
     if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
     if ((uint32_t)count > kCountMask || begin_bit < 0 || end_bit > 64) return hipErrorInvalidValue;
     const int passes = (end_bit - begin_bit + 7) / 8;
@@ -318,14 +321,14 @@ hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel,
     int hist_blocks = (count + kHistChunk - 1) / kHistChunk;
     if (hist_blocks > 1024) hist_blocks = 1024;
     hipLaunchKernelGGL(sort_hist_kernel, dim3(hist_blocks), dim3(kHistThreads), 0, s, keys[*sel], count,
-                       begin_bit, end_bit, passes, hist);
+                       count_dev, begin_bit, end_bit, passes, hist);
     hipLaunchKernelGGL(sort_hist_scan_kernel, dim3(passes), dim3(256), 0, s, hist);
     for (int p = 0; p < passes; ++p) {
         const int shift = begin_bit + 8 * p;
         const int bits = min(8, end_bit - shift);
         const int cur = *sel;
         hipLaunchKernelGGL(onesweep_kernel, dim3(l.partitions), dim3(kSortThreads), 0, s, keys[cur],
-                           values[cur], keys[cur ^ 1], values[cur ^ 1], count, shift, bits,
+                           values[cur], keys[cur ^ 1], values[cur ^ 1], count, count_dev, shift, bits,
                            hist + p * kRadix, status[p & 1], (p + 1 < passes) ? status[(p + 1) & 1] : nullptr,
                            tickets + p);
         *sel = cur ^ 1;

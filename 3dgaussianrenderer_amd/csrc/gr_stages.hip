@@ -56,67 +56,72 @@ template <> __device__ __forceinline__ float h16<uint16_t>(uint32_t bits) {
 
 template <int DEG, typename T>
 __global__ __launch_bounds__(256) void sh_kernel(ShArgs a) {
+    constexpr int NC = 3 * (DEG + 1) * (DEG + 1);
     const int i = blockIdx.x * kShGroup + threadIdx.x;
     if (i >= a.count) return;
     const float4 p = a.position[i];
-    const v3 dir = normalize3(v3{a.camera_position.x - p.x, a.camera_position.y - p.y,
-                                 a.camera_position.z - p.z});
-    const float x = dir.x, y = dir.y, z = dir.z;
     const T* sh = static_cast<const T*>(a.sh) + (size_t)blockIdx.x * kShGroup * a.sh_count + threadIdx.x;
-    auto c = [&](int k) { return sh_ld<T>(sh + (size_t)k * kShGroup); };
-
-    float res[3];
+    float c[NC];
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch) res[ch] = 0.0f + 0.282094792f * c(ch);
-
-    if (DEG > 0) {
-        const float k1 = 0.488602512f * y, k2 = 0.488602512f * z, k3 = 0.488602512f * x;
+    for (int k = 0; k < NC; ++k) c[k] = sh_ld<T>(sh + (size_t)k * kShGroup);
+    float opacity;
+    if constexpr (sizeof(T) == 2) {
+        opacity = h16<T>(a.packed[i].y >> 16);   // opacity of the precision form
+    } else {
+        opacity = p.w;
+    }
+    for (int v = 0; v < a.nviews; ++v) {
+        const v3 cp = view_of(v, a.v0, a.views).cam.position;
+        const v3 dir = normalize3(v3{cp.x - p.x, cp.y - p.y, cp.z - p.z});
+        const float x = dir.x, y = dir.y, z = dir.z;
+        float res[3];
 #pragma unroll
-        for (int ch = 0; ch < 3; ++ch) {
-            const float t = (k1 * c(3 + ch) + k2 * c(6 + ch)) + k3 * c(9 + ch);
-            res[ch] = res[ch] + t;
-        }
-        if (DEG > 1) {
-            const float xx = x * x, yy = y * y;
-            const float k4 = (1.09254843f * x) * y;
-            const float k5 = (1.09254843f * y) * z;
-            const float k6 = ((-0.946174696f * xx) - 0.946174696f * yy) + 0.630783131f;
-            const float k7 = (1.09254843f * x) * z;
-            const float k8 = (0.546274215f * (x - y)) * (x + y);
+        for (int ch = 0; ch < 3; ++ch) res[ch] = 0.0f + 0.282094792f * c[ch];
+        if (DEG > 0) {
+            const float k1 = 0.488602512f * y, k2 = 0.488602512f * z, k3 = 0.488602512f * x;
 #pragma unroll
             for (int ch = 0; ch < 3; ++ch) {
-                const float t = (((k4 * c(12 + ch) + k5 * c(15 + ch)) + k6 * c(18 + ch)) +
-                                 k7 * c(21 + ch)) + k8 * c(24 + ch);
+                const float t = (k1 * c[3 + ch] + k2 * c[6 + ch]) + k3 * c[9 + ch];
                 res[ch] = res[ch] + t;
             }
-            if (DEG > 2) {
-                const float k9 = (0.295021795f * y) * (6.0f * xx - 2.0f * yy);
-                const float k10 = ((2.89061144f * x) * y) * z;
-                const float k11 = (3.6563664f * y) * ((-0.625f * xx - 0.625f * yy) + 0.5f);
-                const float k12 = (0.373176333f * z) * ((-5.0f * xx - 5.0f * yy) + 2.0f);
-                const float k13 = (0.457045799f * x) * ((-5.0f * xx - 5.0f * yy) + 4.0f);
-                const float k14 = ((1.44530572f * z) * (x - y)) * (x + y);
-                const float k15 = (0.59004359f * x) * (xx - 3.0f * yy);
+            if (DEG > 1) {
+                const float xx = x * x, yy = y * y;
+                const float k4 = (1.09254843f * x) * y;
+                const float k5 = (1.09254843f * y) * z;
+                const float k6 = ((-0.946174696f * xx) - 0.946174696f * yy) + 0.630783131f;
+                const float k7 = (1.09254843f * x) * z;
+                const float k8 = (0.546274215f * (x - y)) * (x + y);
 #pragma unroll
                 for (int ch = 0; ch < 3; ++ch) {
-                    const float t = (((((k9 * c(27 + ch) + k10 * c(30 + ch)) + k11 * c(33 + ch)) +
-                                       k12 * c(36 + ch)) + k13 * c(39 + ch)) + k14 * c(42 + ch)) +
-                                    k15 * c(45 + ch);
+                    const float t = (((k4 * c[12 + ch] + k5 * c[15 + ch]) + k6 * c[18 + ch]) + k7 * c[21 + ch]) +
+                                    k8 * c[24 + ch];
                     res[ch] = res[ch] + t;
+                }
+                if (DEG > 2) {
+                    const float k9 = (0.295021795f * y) * (6.0f * xx - 2.0f * yy);
+                    const float k10 = ((2.89061144f * x) * y) * z;
+                    const float k11 = (3.6563664f * y) * ((-0.625f * xx - 0.625f * yy) + 0.5f);
+                    const float k12 = (0.373176333f * z) * ((-5.0f * xx - 5.0f * yy) + 2.0f);
+                    const float k13 = (0.457045799f * x) * ((-5.0f * xx - 5.0f * yy) + 4.0f);
+                    const float k14 = ((1.44530572f * z) * (x - y)) * (x + y);
+                    const float k15 = (0.59004359f * x) * (xx - 3.0f * yy);
+#pragma unroll
+                    for (int ch = 0; ch < 3; ++ch) {
+                        const float t = (((((k9 * c[27 + ch] + k10 * c[30 + ch]) + k11 * c[33 + ch]) +
+                                           k12 * c[36 + ch]) + k13 * c[39 + ch]) + k14 * c[42 + ch]) +
+                                        k15 * c[45 + ch];
+                        res[ch] = res[ch] + t;
+                    }
                 }
             }
         }
+        float4 out;
+        out.x = gclamp(res[0] + 0.5f, 0.0f, 1.0f);
+        out.y = gclamp(res[1] + 0.5f, 0.0f, 1.0f);
+        out.z = gclamp(res[2] + 0.5f, 0.0f, 1.0f);
+        out.w = opacity;
+        a.color[(size_t)v * a.count + i] = out;
     }
-    float4 out;
-    out.x = gclamp(res[0] + 0.5f, 0.0f, 1.0f);
-    out.y = gclamp(res[1] + 0.5f, 0.0f, 1.0f);
-    out.z = gclamp(res[2] + 0.5f, 0.0f, 1.0f);
-    if constexpr (sizeof(T) == 2) {
-        out.w = h16<T>(a.packed[i].y >> 16);   // opacity of the precision form
-    } else {
-        out.w = p.w;
-    }
-    a.color[i] = out;
 }
 
 template <typename T>
@@ -175,10 +180,15 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     const m3 RS = mul33(quat_to_m3(rw, rx, ry, rz), S);
     const m3 sigma = mul33(RS, transpose3(RS));
 
-    const v4 vp4 = mul44v(a.cam.view, v4{pd.x, pd.y, pd.z, 1.0f});
+    for (int vi = 0; vi < a.nviews; ++vi) {
+    const ViewDesc& vd = view_of(vi, a.v0, a.views);
+    const CameraArgs& cam = vd.cam;
+    const FrameGeom& geom = vd.geom;
+    const size_t o = (size_t)vi * a.count + i;
+    const v4 vp4 = mul44v(cam.view, v4{pd.x, pd.y, pd.z, 1.0f});
     const v3 v = v3{vp4.x, vp4.y, vp4.z};
-    const v2 fc = a.cam.fov_cotangent;
-    const v2 dsb = a.cam.depth_scale_bias;
+    const v2 fc = cam.fov_cotangent;
+    const v2 dsb = cam.depth_scale_bias;
 
     const float zRcp = 1.0f / v.z;
     const float zRcpSqr = zRcp * zRcp;
@@ -196,13 +206,13 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     A.c[2] = v4{shearX, shearY, dsb.x, 0.0f};
     A.c[3] = v4{transX, transY, dsb.y, 1.0f};
 
-    const m3 VP = mul33(upper3(A), upper3(a.cam.view));
+    const m3 VP = mul33(upper3(A), upper3(cam.view));
     m3 cc = mul33(mul33(VP, transpose3(sigma)), transpose3(VP));
     const v4 clip4 = mul44v(A, v4{v.x, v.y, v.z, 1.0f});
     v3 clip = v3{clip4.x, clip4.y, clip4.z};
 
-    cc.c[0].x = cc.c[0].x + a.geom.trace_bump_x;
-    cc.c[1].y = cc.c[1].y + a.geom.trace_bump_y;
+    cc.c[0].x = cc.c[0].x + geom.trace_bump_x;
+    cc.c[1].y = cc.c[1].y + geom.trace_bump_y;
 
     const float ca = cc.c[0].x, cb = cc.c[1].x, ccc = cc.c[1].y;
     const float det = ca * ccc - cb * cb;
@@ -227,10 +237,11 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     extx = extx * vis;
     exty = exty * vis;
 
-    a.clip_xy[i] = make_float2(clip.x, clip.y);
-    a.clip_z[i] = clip.z;
-    a.ellipse[i] = make_float4(e0.x, e0.y, extx, exty);
-    a.conic[i] = make_float4(conx, cony, conz, 0.0f);
+    a.clip_xy[o] = make_float2(clip.x, clip.y);
+    a.clip_z[o] = clip.z;
+    a.ellipse[o] = make_float4(e0.x, e0.y, extx, exty);
+    a.conic[o] = make_float4(conx, cony, conz, 0.0f);
+    }
 }
 
 hipError_t launch_project(const ProjectArgs& a, hipStream_t s) {
@@ -381,17 +392,21 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
     const int first = batch * kBuildBatch;
     if (first >= a.count) return;   // wave-uniform; no block barrier below
     const int g = first + lane;
+    const int view = blockIdx.y;
+    const ViewDesc& vd = view_of(view, a.v0, a.views);
+    const FrameGeom& geom = vd.geom;
+    const size_t vo = (size_t)view * a.count;
 
     int cnt = 0;
     if (g < a.count) {
-        const float2 c = a.clip_xy[g];
-        const float4 e = a.ellipse[g];
-        const Footprint f = footprint(c.x, c.y, e.x, e.y, e.z, e.w, a.geom);
+        const float2 c = a.clip_xy[vo + g];
+        const float4 e = a.ellipse[vo + g];
+        const Footprint f = footprint(c.x, c.y, e.x, e.y, e.z, e.w, geom);
         // the reference keeps max(0, w*h); a rect with w <= 0 or h <= 0 only
         // arises from NaN corners and never passes the exact test
         cnt = (f.nx > 0 && f.ny > 0) ? f.nx * f.ny : 0;
         if (cnt > 0) {
-            s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, EMIT ? (int)depth_word(a.clip_z[g]) : 0);
+            s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, EMIT ? (int)depth_word(a.clip_z[vo + g]) : 0);
             s_ell[w][lane] = EllipseRec{c.x, c.y, e.x, e.y, e.z, e.w, 1.0f / e.z, 1.0f / e.w};
         }
     }
@@ -406,7 +421,8 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
     wave_sync();
 
     uint32_t running = 0;
-    const uint32_t out_base = EMIT ? a.batch_counts[batch] : 0u;
+    const size_t bslot = (size_t)view * a.batches + batch;
+    const uint32_t out_base = EMIT ? a.batch_counts[bslot] : 0u;
     for (int base = 0; base < total; base += 64) {
         const int c = base + lane;
         bool pass = false;
@@ -421,36 +437,36 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
             divmod_small(c - s_excl[w][o], rc.z, q, r);
             gx = rc.x + r;
             gy = rc.y + q;
-            pass = tile_overlap(gx, gy, s_ell[w][o], a.geom);
+            pass = tile_overlap(gx, gy, s_ell[w][o], geom);
         }
         const uint64_t m = __ballot(pass);
         if (EMIT && pass) {
             const uint32_t pos = out_base + running + (uint32_t)__popcll(m & lanemask_lt(lane));
             if (pos < (uint32_t)a.capacity) {
-                a.keys[pos] = ((uint64_t)(uint32_t)(gy * a.geom.tiles_x + gx) << 32) | (uint32_t)rc.w;
+                const uint32_t tile = (uint32_t)(vd.tile_base + gy * geom.tiles_x + gx);
+                a.keys[pos] = ((uint64_t)tile << 32) | (uint32_t)rc.w;
                 a.values[pos] = first + o;
             }
         }
         running += (uint32_t)__popcll(m);
     }
-    if (!EMIT && lane == 0) a.batch_counts[batch] = running;
+    if (!EMIT && lane == 0) a.batch_counts[bslot] = running;
 }
 
-static int build_grid(int count) {
-    const int batches = (count + kBuildBatch - 1) / kBuildBatch;
+static dim3 build_grid(const BuildArgs& a) {
     constexpr int W = kBuildThreads / 64;
-    return (batches + W - 1) / W;
+    return dim3((a.batches + W - 1) / W, a.nviews);
 }
 
 hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
-    if (a.count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(build_kernel<false>, dim3(build_grid(a.count)), dim3(kBuildThreads), 0, s, a);
+    if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
+    hipLaunchKernelGGL(build_kernel<false>, build_grid(a), dim3(kBuildThreads), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
-    if (a.count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(build_kernel<true>, dim3(build_grid(a.count)), dim3(kBuildThreads), 0, s, a);
+    if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
+    hipLaunchKernelGGL(build_kernel<true>, build_grid(a), dim3(kBuildThreads), 0, s, a);
     return hipGetLastError();
 }
 
@@ -578,12 +594,14 @@ hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t s) 
 
 __global__ __launch_bounds__(256) void ranges_kernel(RangeArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int size = a.size_dev ? (int)min((uint32_t)a.size, *a.size_dev) : a.size;
+    if (size <= 0) return;
     if (i == 0) {
         const uint32_t first = (uint32_t)(a.keys[0] >> 32);
         a.tile_range[(size_t)first * 2] = 0;
-        const uint32_t last = (uint32_t)(a.keys[a.size - 1] >> 32);
-        a.tile_range[(size_t)last * 2 + 1] = a.size;
-    } else if (i < a.size) {
+        const uint32_t last = (uint32_t)(a.keys[size - 1] >> 32);
+        a.tile_range[(size_t)last * 2 + 1] = size;
+    } else if (i < size) {
         const uint32_t prev = (uint32_t)(a.keys[i - 1] >> 32);
         const uint32_t cur = (uint32_t)(a.keys[i] >> 32);
         if (cur != prev) {
@@ -622,8 +640,15 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
     const int bid = blockIdx.x;
     const int tile = (bid & 7) * a.tiles_per_xcd + (bid >> 3);
     if (tile >= a.tiles) return;   // uniform over the workgroup
-    const FrameGeom& g = a.geom;
-    const int tx = tile % g.tiles_x, ty = tile / g.tiles_x;
+    int view = 0;
+    if (a.views) {
+        while (view + 1 < a.nviews && a.views[view + 1].tile_base <= tile) ++view;
+    }
+    const ViewDesc& vd = view_of(view, a.v0, a.views);
+    const FrameGeom& g = vd.geom;
+    const int local = tile - vd.tile_base;
+    const int tx = local % g.tiles_x, ty = local / g.tiles_x;
+    const size_t vo = (size_t)view * a.count;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int px = tx * kTileSize + (w & 1) * 8 + (lane & 7);
     const int py = ty * kTileSize + (w >> 1) * 8 + (lane >> 3);
@@ -642,7 +667,7 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
             const int nr = min(kRasterThreads, n - r0);
             __syncthreads();
             if ((int)threadIdx.x < nr) {
-                const int gi = a.values[start + r0 + threadIdx.x];
+                const size_t gi = vo + (size_t)a.values[start + r0 + threadIdx.x];
                 const float2 c = a.clip_xy[gi];
                 const float4 q = a.conic[gi];
                 const float4 col = a.color[gi];
@@ -687,7 +712,7 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
         } else {
             o = make_uchar4(0, 0, 0, 0);
         }
-        a.out[(size_t)py * g.width + px] = o;
+        vd.out[(size_t)py * g.width + px] = o;
     }
 }
 

@@ -158,6 +158,10 @@ GR_API const char* gr_version(void);
 /* ---- context / timer (replace __constant__ globals and CudaTimer) ----- */
 /* stream: a hipStream_t, or NULL to create (and own) a new one. */
 GR_API gr_status gr_context_create(gr_context** out, void* stream);
+/* Same, with flags: GR_CONTEXT_BORROW_STREAM uses `stream` as given even when
+ * it is NULL (the legacy default stream). */
+#define GR_CONTEXT_BORROW_STREAM 1
+GR_API gr_status gr_context_create_ex(gr_context** out, void* stream, int32_t flags);
 GR_API gr_status gr_context_destroy(gr_context* ctx);
 GR_API gr_status gr_context_synchronize(gr_context* ctx);
 
@@ -285,5 +289,74 @@ typedef struct gr_frame_view {
     int32_t splat_count;
 } gr_frame_view;
 GR_API gr_status gr_renderer_frame_view(gr_renderer* r, gr_frame_view* out);
+
+/* ---- serving: continuous batching over resident scenes -----------------
+ * No counterpart in the reference (one scene, one frame at a time,
+ * app/main.cpp:326-472).  A server keeps scenes resident in HBM and renders
+ * requests (scene, camera, width x height) submitted from any thread.  A
+ * scheduler thread forms batches of pending requests of one scene and renders
+ * each batch with ONE launch per stage (views fused: SH and projection read
+ * the scene once for all views, the tile lists of all views go through one
+ * sort), optionally replaying hipGraph-captured stage sequences.  Up to
+ * `slots` batches are in flight on their own streams, so requests that
+ * arrive while others run are admitted as soon as a slot and HBM budget are
+ * free, and their work overlaps on the device.  Memory admission: resident
+ * scenes + every slot's workspace (per-view buffers, tile keys, sort buffers)
+ * stay within hbm_budget_bytes; a batch waits when its workspace would not
+ * fit and fails with GR_ERR_BUDGET only when it cannot fit even alone.
+ * A view renders bit-identically alone, in a batch, and through
+ * gr_renderer_render (same precision). */
+typedef struct gr_server gr_server;
+
+typedef struct gr_server_config {
+    int32_t device;               /* -1: current device */
+    int32_t slots;                /* batches in flight (>= 1) */
+    int32_t max_views_per_batch;  /* >= 1 */
+    int32_t use_graphs;           /* capture + replay stage sequences with hipGraph */
+    int64_t hbm_budget_bytes;     /* resident scenes + workspaces; 0 = no limit */
+} gr_server_config;
+
+typedef struct gr_request {
+    int32_t scene;
+    int32_t width;
+    int32_t height;
+    int32_t pad;
+    gr_camera_data camera;
+    uint8_t* out_rgba8;           /* host memory, width*height*4 bytes; written before completion */
+    uint64_t user_tag;
+} gr_request;
+
+typedef struct gr_completion {
+    uint64_t request_id;
+    uint64_t user_tag;
+    int32_t status;               /* gr_status */
+    int32_t batch_views;          /* views fused in this request's batch */
+    double submit_ms;             /* server clock (gr_server_clock_ms) */
+    double start_ms;              /* batch launch */
+    double finish_ms;             /* result in out_rgba8 */
+    int64_t tile_list_size;       /* entries of the whole batch */
+} gr_completion;
+
+typedef struct gr_server_stats {
+    int64_t resident_bytes;       /* scenes */
+    int64_t workspace_bytes;      /* all slots */
+    int64_t peak_bytes;
+    int64_t batches;
+    int64_t views;
+    int64_t graph_launches;
+    int64_t budget_waits;         /* times a batch waited for memory */
+} gr_server_stats;
+
+GR_API gr_status gr_server_create(gr_server** out, const gr_server_config* config);
+GR_API gr_status gr_server_destroy(gr_server* server);
+/* Uploads a scene (precision field of the model selects FP32/FP16/BF16 storage). */
+GR_API gr_status gr_server_add_scene(gr_server* server, const gr_model_host* model, int32_t* scene_id);
+GR_API gr_status gr_server_remove_scene(gr_server* server, int32_t scene_id);
+GR_API gr_status gr_server_submit(gr_server* server, const gr_request* request, uint64_t* request_id);
+/* Returns up to max completions; waits up to wait_ms for the first one. */
+GR_API gr_status gr_server_poll(gr_server* server, gr_completion* out, int32_t max, int32_t* n_out,
+                                int32_t wait_ms);
+GR_API gr_status gr_server_get_stats(gr_server* server, gr_server_stats* out);
+GR_API double gr_server_clock_ms(gr_server* server);
 
 #endif /* GR_CAPI_H_ */
