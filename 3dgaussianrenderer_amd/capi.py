@@ -86,6 +86,31 @@ class FrameView(C.Structure):
                 ("tiles_x", C.c_int32), ("tiles_y", C.c_int32), ("splat_count", C.c_int32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("max_views_per_batch", C.c_int32),
+                ("use_graphs", C.c_int32), ("hbm_budget_bytes", C.c_int64)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("pad", C.c_int32),
+                ("camera", CameraData), ("out_rgba8", C.c_void_p), ("user_tag", C.c_uint64)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("user_tag", C.c_uint64), ("status", C.c_int32),
+                ("batch_views", C.c_int32), ("submit_ms", C.c_double), ("start_ms", C.c_double),
+                ("finish_ms", C.c_double), ("tile_list_size", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("resident_bytes", C.c_int64), ("workspace_bytes", C.c_int64), ("peak_bytes", C.c_int64),
+                ("batches", C.c_int64), ("views", C.c_int64), ("graph_launches", C.c_int64),
+                ("budget_waits", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
 # (name, restype, argtypes); every name here is declared in include/gr_capi.h
@@ -129,6 +154,14 @@ SIGNATURES = [
     ("gr_renderer_render", C.c_int, [_P, C.POINTER(CameraData), C.c_int32, C.c_int32, _P, C.c_int32,
                                      C.POINTER(RenderStats)]),
     ("gr_renderer_frame_view", C.c_int, [_P, C.POINTER(FrameView)]),
+    ("gr_server_create", C.c_int, [_PP, C.POINTER(ServerConfig)]),
+    ("gr_server_destroy", C.c_int, [_P]),
+    ("gr_server_add_scene", C.c_int, [_P, C.POINTER(ModelHost), C.POINTER(C.c_int32)]),
+    ("gr_server_remove_scene", C.c_int, [_P, C.c_int32]),
+    ("gr_server_submit", C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    ("gr_server_poll", C.c_int, [_P, C.POINTER(Completion), C.c_int32, C.POINTER(C.c_int32), C.c_int32]),
+    ("gr_server_get_stats", C.c_int, [_P, C.POINTER(ServerStats)]),
+    ("gr_server_clock_ms", C.c_double, [_P]),
 ]
 
 _lib = None
@@ -337,3 +370,82 @@ class Renderer:
         v = FrameView()
         check(lib().gr_renderer_frame_view(self.h, C.byref(v)))
         return v
+
+
+class Server:
+    """Continuous-batching render server (gr_server_*)."""
+
+    def __init__(self, slots: int = 3, max_views: int = 16, use_graphs: bool = True, budget_bytes: int = 0,
+                 device: int = -1):
+        self.h = C.c_void_p()
+        cfg = ServerConfig(device, slots, max_views, 1 if use_graphs else 0, budget_bytes)
+        check(lib().gr_server_create(C.byref(self.h), C.byref(cfg)))
+        self._keep = {}
+        self._scenes = {}
+
+    def close(self):
+        if self.h:
+            lib().gr_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_scene(self, scene: dict, precision: int = 0) -> int:
+        sid = C.c_int32()
+        m = model_host(scene, precision)
+        check(lib().gr_server_add_scene(self.h, C.byref(m), C.byref(sid)))
+        self._scenes[sid.value] = scene
+        return sid.value
+
+    def remove_scene(self, sid: int):
+        check(lib().gr_server_remove_scene(self.h, sid))
+        self._scenes.pop(sid, None)
+
+    def submit(self, sid: int, cam: CameraData, width: int, height: int, out: np.ndarray | None = None,
+               tag: int = 0) -> int:
+        """Submits a request; `out` (H, W, 4) uint8 receives the image before completion."""
+        if out is None:
+            out = np.empty((height, width, 4), np.uint8)
+        assert out.flags["C_CONTIGUOUS"] and out.nbytes == width * height * 4
+        r = Request(sid, width, height, 0, cam, _fp(out), tag)
+        rid = C.c_uint64()
+        check(lib().gr_server_submit(self.h, C.byref(r), C.byref(rid)))
+        self._keep[rid.value] = out
+        return rid.value
+
+    def poll(self, max_n: int = 256, wait_ms: int = 0):
+        """Returns a list of (completion dict, image) pairs."""
+        arr = (Completion * max_n)()
+        n = C.c_int32()
+        check(lib().gr_server_poll(self.h, arr, max_n, C.byref(n), wait_ms))
+        out = []
+        for i in range(n.value):
+            c = arr[i]
+            d = {k: getattr(c, k) for k, _ in Completion._fields_}
+            out.append((d, self._keep.pop(c.request_id, None)))
+        return out
+
+    def wait_all(self, ids, timeout_s: float = 60.0):
+        import time
+        want = set(ids)
+        got = {}
+        t_end = time.time() + timeout_s
+        while want - set(got) and time.time() < t_end:
+            for d, img in self.poll(256, 50):
+                got[d["request_id"]] = (d, img)
+        missing = want - set(got)
+        if missing:
+            raise TimeoutError(f"{len(missing)} requests did not complete")
+        return got
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().gr_server_get_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def clock_ms(self) -> float:
+        return lib().gr_server_clock_ms(self.h)

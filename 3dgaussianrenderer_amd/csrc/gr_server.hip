@@ -78,6 +78,11 @@ struct Dev {
         if (grew) *grew = true;
         return GR_OK;
     }
+    // Net change of the accounted bytes if ensure(need) ran now (the old block is freed first).
+    int64_t growth(size_t need) const {
+        if (need <= bytes && p) return 0;
+        return (int64_t)std::max<size_t>(need + need / 4, 256) - (int64_t)bytes;
+    }
     void release(std::atomic<int64_t>& acct) {
         if (p) {
             (void)hipFree(p);
@@ -181,6 +186,7 @@ struct gr_server {
     std::atomic<int64_t> peak{0}, batches{0}, views{0}, graph_launches{0}, budget_waits{0};
     std::vector<Slot> slots;
     std::map<int32_t, std::deque<Pending>> queues;   // scheduler-owned
+    std::map<int32_t, int> batch_limit;              // scheduler-owned: one-shot cap on a scene's next batch
     std::thread worker;
     std::atomic<bool> stop{false};
     std::string fatal;
@@ -240,34 +246,71 @@ void complete(gr_server* s, Slot& sl, int32_t status) {
 
 int64_t per_view_bytes(const Scene& sc) { return (int64_t)sc.n * (16 + 8 + 4 + 16 + 16); }
 
-// Bytes phase A would add to the slot's workspace for this batch.
-int64_t phase_a_growth(const Slot& sl, const Scene& sc, const std::vector<Pending>& b, int64_t img_bytes,
-                       int32_t tiles_total) {
-    const int64_t V = (int64_t)b.size(), N = sc.n;
+// Phase-A buffers of a slot and the bytes a batch needs in each.
+struct PhaseANeed {
+    Dev* dev;
+    int64_t bytes;
+};
+
+std::vector<PhaseANeed> phase_a_needs(Slot& sl, const Scene& sc, size_t nviews, int64_t img_bytes,
+                                      int32_t tiles_total) {
+    Workspace& w = sl.ws;
+    const int64_t V = (int64_t)nviews, N = sc.n;
     const int64_t batches = (N + kBuildBatch - 1) / kBuildBatch;
-    auto need = [](const Dev& d, int64_t bytes) { return d.p && (int64_t)d.bytes >= bytes ? 0 : bytes + bytes / 4; };
-    return need(sl.ws.color, V * N * 16) + need(sl.ws.xy, V * N * 8) + need(sl.ws.z, V * N * 4) +
-           need(sl.ws.ell, V * N * 16) + need(sl.ws.con, V * N * 16) +
-           need(sl.ws.counts, 4 * (V * batches + 2 + scan_scratch_words((int32_t)(V * batches)))) +
-           need(sl.ws.images, img_bytes) + need(sl.ws.ranges, 8 * (int64_t)tiles_total) +
-           need(sl.ws.views, V * (int64_t)sizeof(ViewDesc));
+    return {{&w.color, V * N * 16},
+            {&w.xy, V * N * 8},
+            {&w.z, V * N * 4},
+            {&w.ell, V * N * 16},
+            {&w.con, V * N * 16},
+            {&w.counts, 4 * (V * batches + 2 + scan_scratch_words((int32_t)(V * batches)))},
+            {&w.images, img_bytes},
+            {&w.ranges, 8 * (int64_t)tiles_total},
+            {&w.views, V * (int64_t)sizeof(ViewDesc)}};
+}
+
+// Bytes phase A would add to the slot's workspace for this batch.
+int64_t phase_a_growth(Slot& sl, const Scene& sc, size_t nviews, int64_t img_bytes, int32_t tiles_total) {
+    int64_t g = 0;
+    for (const PhaseANeed& n : phase_a_needs(sl, sc, nviews, img_bytes, tiles_total)) g += n.dev->growth((size_t)n.bytes);
+    return g;
+}
+
+int64_t fresh_bytes(int64_t need) { return (int64_t)std::max<size_t>((size_t)need + (size_t)need / 4, 256); }
+
+// Device bytes of a workspace allocated from nothing for the current batch with `entries` tile-list entries.
+int64_t fresh_workspace_bytes(Slot& sl, const Scene& sc, size_t entries) {
+    int64_t b = 0;
+    for (const PhaseANeed& n : phase_a_needs(sl, sc, sl.batch.size(), sl.img_bytes, sl.tiles_total))
+        b += fresh_bytes(n.bytes);
+    const int64_t cap = std::min<int64_t>(fresh_bytes((int64_t)entries * 8) / 8, fresh_bytes((int64_t)entries * 4) / 4);
+    b += 2 * fresh_bytes((int64_t)entries * 8) + 2 * fresh_bytes((int64_t)entries * 4);
+    b += fresh_bytes((int64_t)radix_sort_temp_bytes((int32_t)std::min<int64_t>(cap, (1 << 30) - 1), 0, sl.end_bit));
+    return b;
+}
+
+int64_t workspace_bytes(const Slot& sl) {
+    const Workspace& w = sl.ws;
+    int64_t b = 0;
+    for (const Dev* d : {&w.color, &w.xy, &w.z, &w.ell, &w.con, &w.counts, &w.keys0, &w.keys1, &w.vals0, &w.vals1,
+                         &w.sort_temp, &w.ranges, &w.images, &w.views})
+        b += (int64_t)d->bytes;
+    return b;
+}
+
+void release_workspace(gr_server* s, Slot& sl) {
+    Workspace& w = sl.ws;
+    for (Dev* d : {&w.color, &w.xy, &w.z, &w.ell, &w.con, &w.counts, &w.keys0, &w.keys1, &w.vals0, &w.vals1,
+                   &w.sort_temp, &w.ranges, &w.images, &w.views})
+        d->release(s->workspace);
+    ++w.generation;
 }
 
 gr_status ensure_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     Workspace& w = sl.ws;
-    const size_t V = sl.batch.size(), N = (size_t)sc.n;
-    const size_t batches = (N + kBuildBatch - 1) / kBuildBatch;
+    const size_t V = sl.batch.size();
     bool grew = false;
-    S_CHECK(w.color.ensure(V * N * 16, s->workspace, &grew));
-    S_CHECK(w.xy.ensure(V * N * 8, s->workspace, &grew));
-    S_CHECK(w.z.ensure(V * N * 4, s->workspace, &grew));
-    S_CHECK(w.ell.ensure(V * N * 16, s->workspace, &grew));
-    S_CHECK(w.con.ensure(V * N * 16, s->workspace, &grew));
-    S_CHECK(w.counts.ensure(4 * (V * batches + 2 + scan_scratch_words((int32_t)(V * batches))), s->workspace,
-                            &grew));
-    S_CHECK(w.images.ensure((size_t)sl.img_bytes, s->workspace, &grew));
-    S_CHECK(w.ranges.ensure(8 * (size_t)sl.tiles_total, s->workspace, &grew));
-    S_CHECK(w.views.ensure(V * sizeof(ViewDesc), s->workspace, &grew));
+    for (const PhaseANeed& n : phase_a_needs(sl, sc, V, sl.img_bytes, sl.tiles_total))
+        S_CHECK(n.dev->ensure((size_t)n.bytes, s->workspace, &grew));
     S_CHECK(w.views_host.ensure(V * sizeof(ViewDesc)));
     S_CHECK(w.k_host.ensure(64));
     S_CHECK(w.img_host.ensure((size_t)sl.img_bytes));
@@ -447,8 +490,14 @@ bool form_batch(gr_server* s, Slot& sl) {
     }
     if (best < 0) return false;
     auto& q = s->queues[best];
+    int limit = s->cfg.max_views_per_batch;
+    auto lim = s->batch_limit.find(best);
+    if (lim != s->batch_limit.end()) {
+        limit = std::min(limit, lim->second);
+        s->batch_limit.erase(lim);
+    }
     sl.batch.clear();
-    while (!q.empty() && (int)sl.batch.size() < s->cfg.max_views_per_batch) {
+    while (!q.empty() && (int)sl.batch.size() < limit) {
         sl.batch.push_back(q.front());
         q.pop_front();
     }
@@ -472,11 +521,7 @@ bool other_slots_busy(gr_server* s, const Slot& me) {
 void release_idle_workspaces(gr_server* s, const Slot& me) {
     for (Slot& o : s->slots) {
         if (&o == &me || o.state != SlotState::Idle) continue;
-        Workspace& w = o.ws;
-        for (Dev* d : {&w.color, &w.xy, &w.z, &w.ell, &w.con, &w.counts, &w.keys0, &w.keys1, &w.vals0,
-                       &w.vals1, &w.sort_temp, &w.ranges, &w.images, &w.views})
-            d->release(s->workspace);
-        ++w.generation;
+        release_workspace(s, o);
         drop_graphs(o);
     }
 }
@@ -509,7 +554,7 @@ bool start_batch(gr_server* s, Slot& sl) {
     sl.tiles_total = tiles;
     sl.img_bytes = img;
     sl.end_bit = 32 + tile_bits_for(tiles);
-    const int64_t growth = phase_a_growth(sl, *sc, sl.batch, img, tiles);
+    const int64_t growth = phase_a_growth(sl, *sc, sl.batch.size(), img, tiles);
     if (s->over_budget(growth)) {
         if (other_slots_busy(s, sl)) {
             ++s->budget_waits;
@@ -517,7 +562,13 @@ bool start_batch(gr_server* s, Slot& sl) {
             return false;
         }
         release_idle_workspaces(s, sl);
-        while (s->over_budget(phase_a_growth(sl, *sc, sl.batch, img, tiles)) && sl.batch.size() > 1) {
+        if (s->over_budget(phase_a_growth(sl, *sc, sl.batch.size(), img, tiles))) {
+            // this slot's buffers are rebuilt at this batch's size (grow-only buffers may be
+            // oversized from earlier, larger batches)
+            release_workspace(s, sl);
+            drop_graphs(sl);
+        }
+        while (s->over_budget(phase_a_growth(sl, *sc, sl.batch.size(), img, tiles)) && sl.batch.size() > 1) {
             // halve the batch: the tail goes back to the queue
             const size_t keep = sl.batch.size() / 2;
             auto& q = s->queues[sl.scene];
@@ -535,7 +586,7 @@ bool start_batch(gr_server* s, Slot& sl) {
             sl.img_bytes = img;
             sl.end_bit = 32 + tile_bits_for(tiles);
         }
-        if (s->over_budget(phase_a_growth(sl, *sc, sl.batch, img, tiles))) {
+        if (s->over_budget(phase_a_growth(sl, *sc, sl.batch.size(), img, tiles))) {
             gr_detail::set_last_error("request does not fit the HBM budget");
             sl.k = 0;
             sl.start_ms = s->now_ms();
@@ -581,50 +632,95 @@ bool start_phase_b(gr_server* s, Slot& sl) {
         sl.state = SlotState::Idle;
         return true;
     }
-    const size_t need = std::max<size_t>(sl.k, 1);
-    const bool have = w.keys0.p && w.keys0.bytes >= need * 8 && w.vals0.bytes >= need * 4;
-    if (!have) {
-        const size_t cap = need + need / 4;
-        const int64_t sort_bytes = (int64_t)radix_sort_temp_bytes((int32_t)cap, 0, sl.end_bit);
-        const int64_t grow = (int64_t)cap * 24 + sort_bytes;
-        if (s->over_budget(grow)) {
-            if (other_slots_busy(s, sl)) {
-                ++s->budget_waits;
-                sl.state = SlotState::WaitMemory;
-                return false;
+    // bytes each tile-list buffer holds after ensure(), and the net growth for `need` entries
+    auto after = [](const Dev& d, size_t want) { return d.p && d.bytes >= want ? d.bytes : std::max<size_t>(want + want / 4, 256); };
+    auto growth_for = [&](size_t need) {
+        const size_t cap_after = std::min(std::min(after(w.keys0, need * 8), after(w.keys1, need * 8)) / 8,
+                                          std::min(after(w.vals0, need * 4), after(w.vals1, need * 4)) / 4);
+        const size_t sort_after =
+            radix_sort_temp_bytes((int32_t)std::min<size_t>(cap_after, (1u << 30) - 1), 0, sl.end_bit);
+        return w.keys0.growth(need * 8) + w.keys1.growth(need * 8) + w.vals0.growth(need * 4) +
+               w.vals1.growth(need * 4) + w.sort_temp.growth(sort_after);
+    };
+    size_t need = std::max<size_t>(sl.k, 1);
+    int64_t grow = growth_for(need);
+    if (grow > 0 && s->over_budget(grow)) {
+        if (other_slots_busy(s, sl)) {
+            ++s->budget_waits;
+            sl.state = SlotState::WaitMemory;
+            return false;
+        }
+        release_idle_workspaces(s, sl);
+        // Split the batch: phase A already produced everything for its first views (view-major
+        // buffers, global tile ids in view order), and the scan's exclusive offset at the first
+        // dropped view is the tile-list size of the kept views.  The tail goes back to the queue.
+        const int batches = (sc->n + kBuildBatch - 1) / kBuildBatch;
+        while (s->over_budget(grow) && sl.batch.size() > 1) {
+            const size_t keep = sl.batch.size() / 2;
+            auto& q = s->queues[sl.scene];
+            for (size_t i = sl.batch.size(); i-- > keep;) q.push_front(sl.batch[i]);
+            sl.batch.resize(keep);
+            int32_t tiles = 0;
+            int64_t img = 0;
+            for (const Pending& pv : sl.batch) {
+                const FrameGeom g = make_geom(pv.req.width, pv.req.height);
+                tiles += g.tiles_x * g.tiles_y;
+                img += (int64_t)pv.req.width * pv.req.height * 4;
             }
-            release_idle_workspaces(s, sl);
-            if (s->over_budget(grow)) {
-                gr_detail::set_last_error("tile list does not fit the HBM budget");
-                complete(s, sl, GR_ERR_BUDGET);
+            sl.tiles_total = tiles;
+            sl.img_bytes = img;
+            sl.end_bit = 32 + tile_bits_for(tiles);
+            uint32_t k = 0;
+            if (hipMemcpyAsync(&k, w.counts.as<uint32_t>() + keep * (size_t)batches, 4, hipMemcpyDeviceToHost,
+                               sl.stream) != hipSuccess ||
+                hipStreamSynchronize(sl.stream) != hipSuccess) {
+                complete(s, sl, GR_ERR_HIP);
                 sl.batch.clear();
                 sl.state = SlotState::Idle;
                 return true;
             }
+            sl.k = k;
+            need = std::max<size_t>(sl.k, 1);
+            grow = growth_for(need);
         }
-        bool grew = false;
-        if (w.keys0.ensure(cap * 8, s->workspace, &grew) != GR_OK ||
-            w.keys1.ensure(cap * 8, s->workspace, &grew) != GR_OK ||
-            w.vals0.ensure(cap * 4, s->workspace, &grew) != GR_OK ||
-            w.vals1.ensure(cap * 4, s->workspace, &grew) != GR_OK) {
-            complete(s, sl, GR_ERR_OUT_OF_MEMORY);
+        if (s->over_budget(grow) && !s->over_budget(fresh_workspace_bytes(sl, *sc, need) - workspace_bytes(sl))) {
+            // phase-A buffers oversized by earlier, larger batches are what is in the way: run the
+            // batch again from phase A on buffers of its own size
+            release_workspace(s, sl);
+            drop_graphs(sl);
+            s->batch_limit[sl.scene] = (int)sl.batch.size();   // the same views again, no more
+            requeue_front(s, sl);
+            return true;
+        }
+        if (s->over_budget(grow)) {
+            gr_detail::set_last_error("tile list does not fit the HBM budget");
+            complete(s, sl, GR_ERR_BUDGET);
             sl.batch.clear();
             sl.state = SlotState::Idle;
             return true;
         }
-        if (grew) ++w.generation;
-        s->note_peak();
     }
-    sl.capacity = (uint32_t)std::min<size_t>(w.keys0.bytes / 8, w.vals0.bytes / 4);
+    bool grew = false;
+    if (w.keys0.ensure(need * 8, s->workspace, &grew) != GR_OK ||
+        w.keys1.ensure(need * 8, s->workspace, &grew) != GR_OK ||
+        w.vals0.ensure(need * 4, s->workspace, &grew) != GR_OK ||
+        w.vals1.ensure(need * 4, s->workspace, &grew) != GR_OK) {
+        complete(s, sl, GR_ERR_OUT_OF_MEMORY);
+        sl.batch.clear();
+        sl.state = SlotState::Idle;
+        return true;
+    }
+    sl.capacity = (uint32_t)std::min(std::min(w.keys0.bytes, w.keys1.bytes) / 8,
+                                     std::min(w.vals0.bytes, w.vals1.bytes) / 4);
     sl.capacity = std::min<uint32_t>(sl.capacity, (1u << 30) - 1);
     const size_t sort_bytes = radix_sort_temp_bytes((int32_t)sl.capacity, 0, sl.end_bit);
-    bool grew = false;
     if (w.sort_temp.ensure(sort_bytes, s->workspace, &grew) != GR_OK) {
         complete(s, sl, GR_ERR_OUT_OF_MEMORY);
         sl.batch.clear();
         sl.state = SlotState::Idle;
         return true;
     }
+    s->note_peak();
     if (grew) ++w.generation;
     if (run_phase(s, sl, *sc, true) != GR_OK || hipEventRecord(sl.ev_b, sl.stream) != hipSuccess) {
         complete(s, sl, GR_ERR_HIP);
@@ -787,7 +883,8 @@ gr_status gr_server_add_scene(gr_server* s, const gr_model_host* m, int32_t* sce
     sc->id = s->next_scene++;
     sc->bytes = bytes;
     *scene_id = sc->id;
-    s->scenes[sc->id] = sc.release();
+    const int32_t id = sc->id;   // (C++17 sequences the right operand of = first)
+    s->scenes[id] = sc.release();
     return GR_OK;
 }
 

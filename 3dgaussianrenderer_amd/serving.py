@@ -1,0 +1,150 @@
+"""Serving workloads and load generator for the continuous-batching server.
+
+Workloads (synthetic, seeded, 64-cluster generator, SH degree 3):
+
+* ``c4``: 8 resident scenes, Gaussian counts log-uniform in [200k, 5M], FP32;
+  Poisson arrivals; each request names a scene, a camera on a sphere of radius
+  4-10 looking at the scene centroid, and a resolution drawn from
+  {640x360, 1280x720, 1920x1080, 3840x2160}.
+* ``c5``: 16 resident scenes, counts log-uniform in [500k, 6M] (~40M total),
+  FP16 storage, HBM budget 200 GB including tile-key and sort buffers; bursty
+  arrivals mixing 64-view same-scene bursts with single 4K requests.
+
+``run_load`` drives a Server with a Poisson (or bursty) arrival process and
+reports views/s and latency percentiles; ``run_sequential`` renders the same
+requests one at a time with the plain Renderer, the baseline the server is
+compared with.
+"""
+from __future__ import annotations
+
+import threading
+import time
+
+import numpy as np
+
+from . import capi
+
+RESOLUTIONS = [(640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
+FOV50 = float(np.deg2rad(50.0))
+
+
+def make_scenes(workload: str, seed: int = 7):
+    rng = np.random.default_rng(seed)
+    if workload == "c4":
+        counts = np.exp(rng.uniform(np.log(2e5), np.log(5e6), 8)).astype(np.int64)
+    elif workload == "c5":
+        counts = np.exp(rng.uniform(np.log(5e5), np.log(6e6), 16)).astype(np.int64)
+    else:
+        raise ValueError(workload)
+    return [capi.generate_scene(2, int(n), 3, seed * 1000 + i) for i, n in enumerate(counts)]
+
+
+def make_requests(scenes, n: int, seed: int = 11, resolutions=RESOLUTIONS):
+    """(scene index, CameraData, W, H) tuples with random poses and resolutions."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        i = int(rng.integers(0, len(scenes)))
+        W, H = resolutions[int(rng.integers(0, len(resolutions)))]
+        c = scenes[i]["centroid"]
+        eye = capi.pose_eye(int(rng.integers(1 << 31)), 0, c, float(rng.uniform(4.0, 10.0)))
+        out.append((i, capi.camera_make(eye, c, FOV50, W, H), W, H))
+    return out
+
+
+def make_bursty(scenes, n_bursts: int, burst: int = 64, singles_per_burst: int = 4, seed: int = 13):
+    """Bursts of `burst` same-scene views plus single 4K requests (arrival offsets in seconds)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    t = 0.0
+    for b in range(n_bursts):
+        i = int(rng.integers(0, len(scenes)))
+        c = scenes[i]["centroid"]
+        W, H = RESOLUTIONS[int(rng.integers(0, 3))]
+        for k in range(burst):
+            eye = capi.pose_eye(int(rng.integers(1 << 31)), 0, c, float(rng.uniform(4.0, 10.0)))
+            out.append((t, i, capi.camera_make(eye, c, FOV50, W, H), W, H))
+        for k in range(singles_per_burst):
+            j = int(rng.integers(0, len(scenes)))
+            cj = scenes[j]["centroid"]
+            eye = capi.pose_eye(int(rng.integers(1 << 31)), 0, cj, float(rng.uniform(4.0, 10.0)))
+            out.append((t + 0.01 * (k + 1), j, capi.camera_make(eye, cj, FOV50, 3840, 2160), 3840, 2160))
+        t += float(rng.exponential(0.25))
+    return out
+
+
+def percentile(x, q):
+    return float(np.percentile(np.asarray(x, np.float64), q)) if len(x) else float("nan")
+
+
+def run_load(server: capi.Server, scene_ids, requests, rate: float | None = None, arrivals=None, seed: int = 3):
+    """Submits `requests` with Poisson arrivals at `rate` req/s (None: all at once, or the
+    explicit `arrivals` offsets in seconds) and waits for every completion."""
+    rng = np.random.default_rng(seed)
+    n = len(requests)
+    if arrivals is None:
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n)) if rate else np.zeros(n)
+    outs = [np.empty((H, W, 4), np.uint8) for (_, _, W, H) in requests]
+    ids = []
+    done = {}
+    stop = threading.Event()
+
+    def collector():
+        while not stop.is_set() or len(done) < len(ids):
+            for d, _ in server.poll(512, 20):
+                done[d["request_id"]] = d
+            if stop.is_set() and len(done) >= len(ids):
+                break
+
+    th = threading.Thread(target=collector, daemon=True)
+    th.start()
+    t0 = time.perf_counter()
+    for k, (i, cam, W, H) in enumerate(requests):
+        delay = t0 + float(arrivals[k]) - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(server.submit(scene_ids[i], cam, W, H, outs[k], tag=k))
+    stop.set()
+    deadline = time.perf_counter() + 600
+    while len(done) < len(ids) and time.perf_counter() < deadline:
+        time.sleep(0.005)
+    th.join(timeout=5)
+    wall = time.perf_counter() - t0
+    comps = [done[r] for r in ids if r in done]
+    lat = [c["finish_ms"] - c["submit_ms"] for c in comps]
+    ok = sum(1 for c in comps if c["status"] == 0)
+    first = min(c["submit_ms"] for c in comps)
+    last = max(c["finish_ms"] for c in comps)
+    return {
+        "requests": n,
+        "completed_ok": ok,
+        "views_per_s": ok / ((last - first) / 1000.0),
+        "wall_s": wall,
+        "p50_ms": percentile(lat, 50),
+        "p99_ms": percentile(lat, 99),
+        "mean_batch_views": float(np.mean([c["batch_views"] for c in comps])),
+        "images": outs,
+    }
+
+
+def run_sequential(scenes, requests, precision: int = 0):
+    """Plain renders one request at a time (one Renderer per scene, host output)."""
+    rs = {}
+    for i in sorted({i for (i, _, _, _) in requests}):
+        r = capi.Renderer()
+        r.upload(scenes[i], precision)
+        rs[i] = r
+    # warm-up per scene
+    for i, r in rs.items():
+        cam = next(c for (j, c, _, _) in requests if j == i)
+        r.render(cam, 640, 360)
+    t0 = time.perf_counter()
+    lat = []
+    for (i, cam, W, H) in requests:
+        t = time.perf_counter()
+        rs[i].render(cam, W, H)
+        lat.append((time.perf_counter() - t) * 1000.0)
+    wall = time.perf_counter() - t0
+    for r in rs.values():
+        r.close()
+    return {"views_per_s": len(requests) / wall, "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99)}

@@ -42,7 +42,44 @@ def parse():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--serving", default="c4", help="serving workload: c4, c5 or none")
+    ap.add_argument("--serving-requests", type=int, default=1200)
+    ap.add_argument("--serving-rate", type=float, default=400.0)
     return ap.parse_args()
+
+
+def serving_leg(gr, workload: str, n_req: int, rate: float):
+    """Continuous-batching server on the serving workload vs sequential plain renders."""
+    from importlib import import_module
+    sv = import_module("3dgaussianrenderer_amd.serving")
+    t0 = time.perf_counter()
+    scenes = sv.make_scenes(workload)
+    gen_s = time.perf_counter() - t0
+    precision = 1  # FP16 storage, FP32 arithmetic (the mixed-precision serving form)
+    budget = 200 * (1 << 30) if workload == "c5" else 0
+    srv = gr.Server(slots=3, max_views=64 if workload == "c5" else 16, use_graphs=True, budget_bytes=budget)
+    ids = [srv.add_scene(sc, precision) for sc in scenes]
+    reqs = sv.make_requests(scenes, n_req, seed=11)
+    warm = sv.make_requests(scenes, 64, seed=99)
+    sv.run_load(srv, ids, warm, rate=None)
+    out = {"workload": workload, "scenes": len(scenes), "gaussians": int(sum(s["count"] for s in scenes)),
+           "precision": "fp16 storage, fp32 arithmetic", "setup_generate_s": round(gen_s, 2)}
+    if workload == "c5":
+        arr = sv.make_bursty(scenes, n_bursts=max(1, n_req // 68))
+        res = sv.run_load(srv, ids, [(i, c, W, H) for (_, i, c, W, H) in arr], arrivals=[a[0] for a in arr])
+        out["bursty"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items() if k != "images"}
+    else:
+        res = sv.run_load(srv, ids, reqs, rate=rate)
+        out[f"poisson_{int(rate)}"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items()
+                                       if k != "images"}
+    sat = sv.run_load(srv, ids, sv.make_requests(scenes, n_req, seed=12), rate=None)
+    out["saturated"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in sat.items() if k != "images"}
+    out["server_stats"] = srv.stats()
+    srv.close()
+    seq = sv.run_sequential(scenes, reqs[: max(50, n_req // 8)], precision=0)
+    out["sequential_plain_fp32"] = {k: round(v, 3) for k, v in seq.items()}
+    out["speedup_vs_sequential"] = round(sat["views_per_s"] / seq["views_per_s"], 2)
+    return out, scenes, reqs
 
 
 def main():
@@ -95,6 +132,12 @@ def main():
     hbm_stage = max(alg, key=lambda k: mean[k])
     roof = {"bound": "hbm", "kernel": hbm_stage.replace("_ms", ""), "achieved": round(gbs[hbm_stage], 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs[hbm_stage] / HBM_PEAK_GBS, 4), "traffic": None}
+    serving = None
+    if args.serving != "none":
+        r.close()
+        serving, sv_scenes, sv_reqs = serving_leg(gr, args.serving, args.serving_requests, args.serving_rate)
+        r = gr.Renderer(0)
+        r.upload(scene)
     cpu = None
     if args.cpu_baseline:
         from oracle import orc
@@ -110,6 +153,28 @@ def main():
                "sample": f"1 frame of the same workload (pose 0), FP32 oracle, {threads} threads",
                "stage_ms": [round(x, 2) for x in res["stage_ms"].tolist()],
                "cpu_model": platform.processor() or platform.machine()}
+        # accuracy gates (the oracle as checker): the benchmarked frame vs the FP64 render,
+        # and served FP16-storage views vs the FP64 render of their FP32 scenes
+        img, _ = r.render(cam, W, H)
+        r64 = orc.render64(scene["position"], scene["scale_rot"], shal, scene["color"], N, 3, cam.packed(), W, H)
+        acc = {"frame_psnr_vs_fp64_db": round(orc.psnr(img, r64["image"]), 2),
+               "frame_max_abs_diff_lsb": int(np.abs(img[..., :3].astype(int) - r64["image"][..., :3].astype(int)).max())}
+        if serving is not None:
+            srv = gr.Server(slots=1, max_views=4, use_graphs=True)
+            ps = []
+            small = [q for q in sv_reqs if q[2] * q[3] <= 1280 * 720][:3]
+            for (i, c, Wv, Hv) in small:
+                sc = sv_scenes[i]
+                sid = srv.add_scene(sc, 1)
+                rid = srv.submit(sid, c, Wv, Hv)
+                _, simg = srv.wait_all([rid])[rid]
+                al = orc.realign_sh(sc["sh"], sc["count"], 48)
+                ref = orc.render64(sc["position"], sc["scale_rot"], al, sc["color"], sc["count"], 3, c.packed(), Wv, Hv)
+                ps.append(orc.psnr(simg, ref["image"]))
+                srv.remove_scene(sid)
+            srv.close()
+            acc["served_fp16_min_psnr_vs_fp64_db"] = round(min(ps), 2)
+        cpu["accuracy"] = acc
     line = {
         "metric": "1080p frames/s at 6M Gaussians",
         "value": round(fps, 3),
@@ -130,6 +195,7 @@ def main():
         "tile_list_entries": int(K),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serving,
         "setup_s": {"generate": round(gen_s, 2), "upload": round(upload_s, 2)},
     }
     r.close()

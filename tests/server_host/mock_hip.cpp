@@ -1,0 +1,305 @@
+// mock_hip.cpp -- host-only stand-in for the HIP runtime and for the stage
+// launchers, so that the serving scheduler (gr_server.hip, compiled with
+// --cuda-host-only) runs on a machine without a GPU, under AddressSanitizer.
+//
+// Test infrastructure only.  "Device" memory is host memory; every mocked
+// kernel reads or writes exactly the byte ranges the real kernel would, so an
+// undersized workspace, a stale pointer replayed from a graph or a buffer freed
+// while in use shows up as an ASAN report.  Graph capture records the enqueued
+// operations as closures and replays them on hipGraphLaunch, the way a real
+// graph replays its captured nodes (pointers and kernel arguments frozen at
+// capture time, memcpy sources re-read at replay).
+//
+// Mocked raster: every pixel of view v is set to (uint8)cam.position.x of that
+// view, so the test can check that each request received its own image.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gr_capi.h"
+#include "gr_kernels.hpp"
+
+namespace {
+
+struct MockStream {
+    bool capturing = false;
+    std::vector<std::function<void()>>* rec = nullptr;
+};
+
+struct MockGraph {
+    std::vector<std::function<void()>> ops;
+};
+
+void run_or_record(hipStream_t s, std::function<void()> f) {
+    MockStream* ms = reinterpret_cast<MockStream*>(s);
+    if (ms && ms->capturing) ms->rec->push_back(std::move(f));
+    else f();
+}
+
+void touch(void* p, size_t n) {
+    if (n) std::memset(p, 0, n);
+}
+
+void rd(const void* p, size_t n) {
+    if (n) {
+        volatile const char* c = static_cast<const char*>(p);
+        (void)c[0];
+        (void)c[n - 1];
+    }
+}
+
+std::mutex g_err_mu;
+std::string g_last_error;
+
+}  // namespace
+
+namespace gr_detail {
+void set_last_error(const std::string& msg) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_last_error = msg;
+}
+}  // namespace gr_detail
+
+extern "C" {
+
+hipError_t hipSetDevice(int) { return hipSuccess; }
+hipError_t hipGetDevice(int* d) {
+    *d = 0;
+    return hipSuccess;
+}
+const char* hipGetErrorString(hipError_t) { return "mock error"; }
+hipError_t hipMalloc(void** p, size_t n) {
+    *p = std::malloc(n);
+    std::memset(*p, 0xcd, n);
+    return hipSuccess;
+}
+hipError_t hipFree(void* p) {
+    std::free(p);
+    return hipSuccess;
+}
+hipError_t hipHostMalloc(void** p, size_t n, unsigned int) {
+    *p = std::malloc(n);
+    std::memset(*p, 0xcd, n);
+    return hipSuccess;
+}
+hipError_t hipHostFree(void* p) {
+    std::free(p);
+    return hipSuccess;
+}
+hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
+    std::memcpy(d, s, n);
+    return hipSuccess;
+}
+hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t st) {
+    run_or_record(st, [=] { std::memcpy(d, s, n); });
+    return hipSuccess;
+}
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned int) {
+    *s = reinterpret_cast<hipStream_t>(new MockStream());
+    return hipSuccess;
+}
+hipError_t hipStreamDestroy(hipStream_t s) {
+    delete reinterpret_cast<MockStream*>(s);
+    return hipSuccess;
+}
+hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned int) {
+    *e = reinterpret_cast<hipEvent_t>(new int(0));
+    return hipSuccess;
+}
+hipError_t hipEventDestroy(hipEvent_t e) {
+    delete reinterpret_cast<int*>(e);
+    return hipSuccess;
+}
+hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
+    MockStream* ms = reinterpret_cast<MockStream*>(s);
+    if (ms->capturing) return hipErrorInvalidValue;
+    ms->capturing = true;
+    ms->rec = new std::vector<std::function<void()>>();
+    return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g) {
+    MockStream* ms = reinterpret_cast<MockStream*>(s);
+    if (!ms->capturing) return hipErrorInvalidValue;
+    MockGraph* mg = new MockGraph();
+    mg->ops = std::move(*ms->rec);
+    delete ms->rec;
+    ms->rec = nullptr;
+    ms->capturing = false;
+    *g = reinterpret_cast<hipGraph_t>(mg);
+    return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* ex, hipGraph_t g, hipGraphNode_t*, char*, size_t) {
+    *ex = reinterpret_cast<hipGraphExec_t>(new MockGraph(*reinterpret_cast<MockGraph*>(g)));
+    return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t g) {
+    delete reinterpret_cast<MockGraph*>(g);
+    return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t g) {
+    delete reinterpret_cast<MockGraph*>(g);
+    return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t g, hipStream_t) {
+    for (auto& f : reinterpret_cast<MockGraph*>(g)->ops) f();
+    return hipSuccess;
+}
+
+// host helpers the server calls (the real ones live in gr_api.hip)
+gr_status gr_realign_spherical_harmonics(const float* src, float* dst, int32_t group_size, int32_t sh_count,
+                                         int32_t n) {
+    const int groups = (n + group_size - 1) / group_size;
+    for (int grp = 0; grp < groups; ++grp) {
+        const size_t start = (size_t)grp * group_size * sh_count;
+        const int here = std::min(group_size, n - grp * group_size);
+        for (int i = 0; i < sh_count; ++i) {
+            float* d = dst + start + (size_t)group_size * i;
+            for (int k = 0; k < here; ++k) d[k] = src[start + (size_t)sh_count * k + i];
+            for (int k = here; k < group_size; ++k) d[k] = 0.0f;
+        }
+    }
+    return GR_OK;
+}
+
+gr_status gr_pack_precision_model(const gr_model_host* m, int32_t, float* pos, uint16_t* packed, uint16_t* sh) {
+    const size_t n = (size_t)m->num_gaussians;
+    std::memset(pos, 0, n * 16);
+    std::memset(packed, 0, n * 8);
+    if (m->sh_degree > 0) {
+        const size_t groups = (n + 255) / 256;
+        std::memset(sh, 0, groups * 256 * 3 * (m->sh_degree + 1) * (m->sh_degree + 1) * 2);
+    }
+    return GR_OK;
+}
+
+const char* mock_last_error() { return g_last_error.c_str(); }
+
+}  // extern "C"
+
+namespace gr {
+
+hipError_t launch_sh(const ShArgs& a, int, int precision, hipStream_t s) {
+    run_or_record(s, [=] {
+        rd(a.position, (size_t)a.count * 16);
+        if (precision) rd(a.packed, (size_t)a.count * 8);
+        for (int v = 0; v < a.nviews; ++v) rd(&view_of(v, a.v0, a.views), sizeof(ViewDesc));
+        touch(a.color, (size_t)a.nviews * a.count * 16);
+    });
+    return hipSuccess;
+}
+
+hipError_t launch_project(const ProjectArgs& a, hipStream_t s) {
+    run_or_record(s, [=] {
+        const size_t n = (size_t)a.nviews * a.count;
+        rd(a.position, (size_t)a.count * 16);
+        touch(a.clip_xy, n * 8);
+        touch(a.clip_z, n * 4);
+        touch(a.ellipse, n * 16);
+        touch(a.conic, n * 16);
+    });
+    return hipSuccess;
+}
+
+// 37 entries per build batch of every view
+hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
+    run_or_record(s, [=] {
+        const size_t n = (size_t)a.nviews * a.count;
+        rd(a.clip_xy, n * 8);
+        rd(a.clip_z, n * 4);
+        rd(a.ellipse, n * 16);
+        for (int i = 0; i < a.nviews * a.batches; ++i) a.batch_counts[i] = 37;
+    });
+    return hipSuccess;
+}
+
+int scan_scratch_words(int32_t n) { return (n + 1023) / 1024 + 64; }
+
+hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_t* scratch, hipStream_t s) {
+    run_or_record(s, [=] {
+        uint32_t acc = 0;
+        for (int i = 0; i < n; ++i) {
+            const uint32_t c = data[i];
+            data[i] = acc;
+            acc += c;
+        }
+        *total = acc;
+        touch(scratch, (size_t)scan_scratch_words(n) * 4);
+    });
+    return hipSuccess;
+}
+
+hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
+    run_or_record(s, [=] {
+        const uint32_t K = a.batch_counts[a.nviews * a.batches];
+        const uint32_t n = std::min<uint32_t>(K, (uint32_t)a.capacity);
+        touch(a.keys, (size_t)n * 8);
+        touch(a.values, (size_t)n * 4);
+    });
+    return hipSuccess;
+}
+
+hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t s) {
+    run_or_record(s, [=] {
+        for (int64_t i = 0; i < n; ++i) p[i] = value;
+    });
+    return hipSuccess;
+}
+
+hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
+    run_or_record(s, [=] {
+        const uint32_t n = a.size_dev ? std::min<uint32_t>(*a.size_dev, (uint32_t)a.size) : (uint32_t)a.size;
+        rd(a.keys, (size_t)n * 8);
+        touch(a.tile_range, (size_t)a.tiles * 8);
+    });
+    return hipSuccess;
+}
+
+hipError_t launch_raster(const RasterArgs& a, hipStream_t s) {
+    run_or_record(s, [=] {
+        rd(a.tile_range, (size_t)a.tiles * 8);
+        int tiles = 0;
+        for (int v = 0; v < a.nviews; ++v) {
+            const ViewDesc& d = view_of(v, a.v0, a.views);
+            if (d.tile_base != tiles) std::abort();   // views must tile the global tile space in order
+            tiles += d.tiles;
+            std::memset(d.out, (int)(uint8_t)d.cam.position.x, (size_t)d.geom.width * d.geom.height * 4);
+        }
+        if (tiles != a.tiles) std::abort();
+    });
+    return hipSuccess;
+}
+
+size_t radix_sort_temp_bytes(int32_t count, int32_t, int32_t end_bit) {
+    return (size_t)count * 4 + 4096 + (size_t)end_bit;
+}
+
+hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                            const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                            size_t temp_bytes, hipStream_t s) {
+    if (temp_bytes < radix_sort_temp_bytes(count, begin_bit, end_bit)) return hipErrorInvalidValue;
+    uint64_t* k0 = keys[*sel];
+    uint64_t* k1 = keys[*sel ^ 1];
+    int32_t* v0 = values[*sel];
+    int32_t* v1 = values[*sel ^ 1];
+    const size_t tb = radix_sort_temp_bytes(count, begin_bit, end_bit);
+    run_or_record(s, [=] {
+        const uint32_t n = count_dev ? std::min<uint32_t>(*count_dev, (uint32_t)count) : (uint32_t)count;
+        std::memcpy(k1, k0, (size_t)n * 8);
+        std::memcpy(v1, v0, (size_t)n * 4);
+        touch(temp, tb);
+    });
+    *sel ^= 1;
+    return hipSuccess;
+}
+
+}  // namespace gr
