@@ -45,7 +45,23 @@ def parse():
     ap.add_argument("--serving", default="c4", help="serving workload: c4, c5 or none")
     ap.add_argument("--serving-requests", type=int, default=1200)
     ap.add_argument("--serving-rate", type=float, default=400.0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (5 launches), to scale "
+                         "FETCH_SIZE/WRITE_SIZE counter readings")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int):
+    """A plain elementwise copy (one read + one write of `mib` MiB per launch)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
+                      "launches": 5}), flush=True)
 
 
 def serving_leg(gr, workload: str, n_req: int, rate: float):
@@ -90,8 +106,11 @@ def main():
         sys.exit(2)
     import torch
 
-    gr = importlib.import_module("3dgaussianrenderer_amd")
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib)
+        return
+    gr = importlib.import_module("3dgaussianrenderer_amd")
     W, H, N = args.width, args.height, args.count
     t0 = time.perf_counter()
     scene = gr.generate_scene(2, N, 3, args.seed)
