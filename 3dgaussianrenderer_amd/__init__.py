@@ -7,4 +7,5 @@ build recipe.  The directory name starts with a digit, so import it with
 """
 from .capi import (CameraData, Context, DoubleBufferI32, DoubleBufferU64, GlobalArgs, GrError, ModelHost,  # noqa: F401
                    Renderer, RenderStats, SortTemp, TileListArgs, camera_make, encode_vec4, generate_scene, lib,
-                   pose_eye, realign_sh, tile_grid, LIB_PATH, SIGNATURES, PRECISION, model_host, pack_precision, Server)
+                   pose_eye, realign_sh, tile_grid, LIB_PATH, SIGNATURES, PRECISION, model_host, pack_precision, Server,
+                   ply_probe, load_ply, save_ply, write_image)

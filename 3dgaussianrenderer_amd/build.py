@@ -29,7 +29,7 @@ HEADLESS = PKG / "gr_headless"
 ORACLE_DIR = ROOT / "oracle"
 
 HIP_SOURCES = ["gr_stages.hip", "gr_sort.hip", "gr_api.hip", "gr_renderer.hip", "gr_server.hip"]
-CXX_SOURCES = ["gr_scene.cpp", "gauss_render.cpp"]
+CXX_SOURCES = ["gr_scene.cpp", "gauss_render.cpp", "gr_ply.cpp", "gr_image.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

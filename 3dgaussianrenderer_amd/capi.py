@@ -146,6 +146,10 @@ SIGNATURES = [
     ("gr_tile_grid", C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                C.POINTER(C.c_int32)]),
     ("gr_generate_scene", C.c_int, [C.POINTER(SceneParams), _P, _P, _P, _P, C.POINTER(C.c_float)]),
+    ("gr_ply_probe", C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("gr_ply_read", C.c_int, [C.c_char_p, C.c_int32, _P, _P, _P, _P, _P, _P, _P, C.POINTER(C.c_int32)]),
+    ("gr_ply_write", C.c_int, [C.c_char_p, C.c_int32, _P, _P, _P]),
+    ("gr_image_write", C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32, C.c_int32]),
     ("gr_pose_eye", C.c_int, [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_float), C.c_float,
                               C.POINTER(C.c_float)]),
     ("gr_renderer_create", C.c_int, [_PP, C.c_int32, _P]),
@@ -236,6 +240,52 @@ def pack_precision(scene: dict, precision: int):
     m = model_host(scene)
     check(lib().gr_pack_precision_model(C.byref(m), precision, _fp(pos), _fp(packed), _fp(sh)))
     return pos, packed, (sh if d else None)
+
+
+def ply_probe(path) -> tuple[int, int, int]:
+    """(vertex count, SH degree, sh_count) from a PLY header."""
+    n, d, c = C.c_int32(), C.c_int32(), C.c_int32()
+    check(lib().gr_ply_probe(str(path).encode(), C.byref(n), C.byref(d), C.byref(c)))
+    return n.value, d.value, c.value
+
+
+def load_ply(path, want_rotation: bool = False) -> dict:
+    """A 3DGS binary PLY as a scene dict (the layout generate_scene returns)."""
+    n, degree, shc = ply_probe(path)
+    nn = max(n, 1)
+    pos = np.empty((nn, 4), np.float32)
+    sr = np.empty((nn, 4), np.float32)
+    col = np.empty((nn, 4), np.float32)
+    sh = np.empty((nn, shc), np.float32) if degree > 0 else None
+    rot = np.empty((nn, 4), np.float32) if want_rotation else None
+    mn = (C.c_float * 3)()
+    mx = (C.c_float * 3)()
+    got = C.c_int32()
+    check(lib().gr_ply_read(str(path).encode(), nn, _fp(pos), _fp(sr), _fp(col), _fp(sh), _fp(rot),
+                            C.cast(mn, C.c_void_p), C.cast(mx, C.c_void_p), C.byref(got)))
+    n = got.value
+    out = {"position": pos[:n], "scale_rot": sr[:n], "color": col[:n], "sh": None if sh is None else sh[:n],
+           "degree": degree, "count": n, "bounds_min": np.array(list(mn), np.float32),
+           "bounds_max": np.array(list(mx), np.float32)}
+    out["centroid"] = (out["position"][:, :3].mean(axis=0) if n else np.zeros(3)).astype(np.float32)
+    if want_rotation:
+        out["rotation_wxyz"] = rot[:n]
+    return out
+
+
+def save_ply(path, scene: dict) -> None:
+    """savePly's 14-property file of a scene dict (position, scale_rot, color)."""
+    n = scene["count"]
+    check(lib().gr_ply_write(str(path).encode(), n, _fp(np.ascontiguousarray(scene["position"], np.float32)),
+                             _fp(np.ascontiguousarray(scene["scale_rot"], np.float32)),
+                             _fp(np.ascontiguousarray(scene["color"], np.float32))))
+
+
+def write_image(path, rgba: np.ndarray, bottom_up: bool = True) -> None:
+    """(H, W, 4) uint8 back buffer to .png / .ppm (gr_image_write)."""
+    rgba = np.ascontiguousarray(rgba, np.uint8)
+    h, w = rgba.shape[:2]
+    check(lib().gr_image_write(str(path).encode(), _fp(rgba), w, h, 1 if bottom_up else 0))
 
 
 def generate_scene(kind: int, count: int, degree: int = 3, seed: int = 1) -> dict:

@@ -60,16 +60,6 @@ gr_status grow(T*& p, size_t& cap, size_t need, double headroom = 1.0) {
     return GR_OK;
 }
 
-CameraArgs camera_args(const gr_camera_data& c) {
-    CameraArgs a;
-    a.view = load_m4(c.view);
-    a.fov_cotangent = v2{c.fov_cotangent[0], c.fov_cotangent[1]};
-    a.depth_scale_bias = v2{c.depth_scale_bias[0], c.depth_scale_bias[1]};
-    a.position = v3{c.position[0], c.position[1], c.position[2]};
-    a.aspect = c.aspect;
-    return a;
-}
-
 }  // namespace
 
 namespace gr_detail {
@@ -233,7 +223,7 @@ gr_status gr_evaluate_splat_clip_data(gr_context* ctx, gr_timer* timer, int32_t 
     if (count > 0 && (!g.position || (!packed && !g.scale_and_rotation) || (packed && !g.packed_scale_opacity) ||
                       !g.position_clip_xy || !g.position_clip_z || !g.screen_ellipse || !g.conic))
         return fail(GR_ERR_INVALID_ARGUMENT, "projection stage is missing a buffer");
-    ProjectArgs a;
+    ProjectArgs a{};
     a.position = reinterpret_cast<const float4*>(g.position);
     a.scale_rotation = reinterpret_cast<const float4*>(g.scale_and_rotation);
     a.packed = reinterpret_cast<const uint2*>(g.packed_scale_opacity);
@@ -262,7 +252,7 @@ gr_status gr_build_tile_list(gr_context* ctx, gr_timer* timer, int32_t num_block
     const int n = g.splat_count;
     const int batches = (n + kBuildBatch - 1) / kBuildBatch;
     GR_CHECK(grow(ctx->batch_counts, ctx->batch_cap, (size_t)batches + 1 + scan_scratch_words(batches)));
-    BuildArgs a;
+    BuildArgs a{};
     a.clip_xy = reinterpret_cast<const float2*>(g.position_clip_xy);
     a.clip_z = g.position_clip_z;
     a.ellipse = reinterpret_cast<const float4*>(g.screen_ellipse);

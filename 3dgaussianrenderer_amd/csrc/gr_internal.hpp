@@ -6,6 +6,7 @@
 #include <string>
 
 #include "../../include/gr_capi.h"
+#include "gr_kernels.hpp"
 
 struct gr_timer {
     hipEvent_t start = nullptr, stop = nullptr;
@@ -28,3 +29,14 @@ namespace gr_detail {
 // Records the message returned by gr_last_error() on this thread.
 void set_last_error(const std::string& msg);
 }  // namespace gr_detail
+
+// Camera block of the C-ABI -> kernel camera arguments.
+inline gr::CameraArgs camera_args(const gr_camera_data& c) {
+    gr::CameraArgs a;
+    a.view = gr::load_m4(c.view);
+    a.fov_cotangent = gr::v2{c.fov_cotangent[0], c.fov_cotangent[1]};
+    a.depth_scale_bias = gr::v2{c.depth_scale_bias[0], c.depth_scale_bias[1]};
+    a.position = gr::v3{c.position[0], c.position[1], c.position[2]};
+    a.aspect = c.aspect;
+    return a;
+}

@@ -62,10 +62,39 @@ struct ProjectArgs {
     float* clip_z;
     float4* ellipse;
     float4* conic;
+    uint64_t* vis_bits;             // optional [nviews][batches]: bit l of word b = Gaussian 64b+l has a
+                                    // non-empty tile rect (NULL: not written)
+    uint32_t* vis_counts;           // [nviews][batches] popcounts of vis_bits
+    int32_t batches;                // ceil(count / 64)
     int32_t count;
     int32_t nviews;
     ViewDesc v0;
     const ViewDesc* views;
+};
+
+// Depth-ordered tile lists (the renderer's path): the visible Gaussians of
+// every view are compacted in (view, index) order into build records and
+// (depth word, record index) pairs, the pairs are sorted stably by depth
+// word, and the build then walks the records in that order.  Entries of one
+// tile therefore come out ordered by (depth word, index) -- the order the
+// stable sort of (tile << 32 | depth) keys emitted in index order produces --
+// and only the tile bits remain to be sorted.
+//
+// Build record (32 bytes, one cache-line gather per Gaussian in the build):
+//   [0] centre x, centre y, cos, sin     [1] extent x, extent y,
+//   depth word (bits), view * count + index (bits)
+struct CompactArgs {
+    const uint64_t* vis_bits;       // [nviews][batches]
+    const uint32_t* vis_offsets;    // exclusive scan of vis_counts over all views
+    const float2* clip_xy;          // [nviews][count]
+    const float* clip_z;
+    const float4* ellipse;
+    float4* records;                // [visible][2]
+    uint32_t* depth_keys;           // [visible]
+    int32_t* order;                 // [visible] record index (identity before the sort)
+    int32_t count;
+    int32_t batches;
+    int32_t nviews;
 };
 
 struct BuildArgs {
@@ -75,9 +104,12 @@ struct BuildArgs {
     uint32_t* batch_counts;      // [nviews][batches]: counts (count pass) / exclusive offsets (emit pass)
     uint64_t* keys;
     int32_t* values;             // Gaussian index within the scene
+    const int32_t* order;        // depth-ordered walk: record index per position (NULL: index order)
+    const float4* records;       // build records of the depth-ordered walk
+    const uint32_t* order_count; // number of valid positions in `order` (device)
     int32_t count;
     int32_t capacity;
-    int32_t batches;             // ceil(count / 64)
+    int32_t batches;             // index order: ceil(count / 64) per view; depth order: ceil(nviews * count / 64)
     int32_t nviews;
     ViewDesc v0;
     const ViewDesc* views;
@@ -119,6 +151,7 @@ int scan_scratch_words(int32_t n);
 hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_t* scratch,
                              hipStream_t stream);
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t stream);
+hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t stream);
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t stream);
 hipError_t launch_ranges(const RangeArgs& a, hipStream_t stream);
 hipError_t launch_raster(const RasterArgs& a, hipStream_t stream);
@@ -133,5 +166,9 @@ size_t radix_sort_temp_bytes(int32_t count, int32_t begin_bit, int32_t end_bit);
 hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                             const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                             size_t temp_bytes, hipStream_t stream);
+// The same for 32-bit keys (depth words); same temp layout and size.
+hipError_t radix_sort_pairs_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                                const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                                size_t temp_bytes, hipStream_t stream);
 
 }  // namespace gr

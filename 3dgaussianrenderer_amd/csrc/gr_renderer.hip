@@ -5,11 +5,19 @@
 // Frame (one HIP stream, no host sync except the 4-byte K readback that sizes
 // the tile list, as the reference's buildTileList does at
 // core/gaussian_render.cu:581):
-//   SH (degree > 0) -> project -> build count/scan, read K -> grow tile list
-//   (grow-only, 1.25x headroom) -> emit -> radix sort -> ranges -> raster.
-// The reference instead truncates a full tile list and doubles the capacity
-// for the next frame (app/main.cpp:330-338,405); here a frame is never
-// truncated.
+//   SH (degree > 0) -> project (+ per-64 visibility ballot)
+//   -> depth order: scan of the ballots, compaction of the visible Gaussians
+//      into (depth word, index) pairs, 32-bit onesweep sort by depth word
+//   -> build count over the depth-ordered Gaussians, scan, read K -> grow the
+//      tile list (grow-only, 1.25x headroom) -> emit
+//   -> onesweep sort of the tile bits only (bits [32, 32 + tile bits))
+//   -> ranges -> raster.
+// Entries are emitted in (depth word, Gaussian index) order, so the stable
+// tile-bit sort yields exactly the order of the reference's full-key sort of
+// (tile << 32 | depth word) keys emitted in index order, with 2 passes over
+// the K entries instead of 6 (4 passes run over the visible Gaussians only).
+// The reference truncates a full tile list and doubles the capacity for the
+// next frame (app/main.cpp:330-338,405); here a frame is never truncated.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -75,7 +83,7 @@ gr_status rfail(gr_status st, const std::string& msg) {
 struct gr_renderer {
     int device = 0;
     gr_context* ctx = nullptr;
-    hipEvent_t ev[7] = {};
+    hipEvent_t ev[8] = {};
     int32_t n = 0, degree = 0, sh_count = 3, precision = GR_PRECISION_FP32;
     DevBuf<float4> position, scale_rot, color, conic, ellipse;
     DevBuf<uint2> packed;        // precision form: scale + opacity, 16-bit
@@ -86,6 +94,12 @@ struct gr_renderer {
     DevBuf<uint64_t> keys[2];
     DevBuf<int32_t> values[2];
     DevBuf<uchar4> out;
+    DevBuf<uint64_t> vis_bits;      // per 64 Gaussians: non-empty tile rect ballot
+    DevBuf<uint32_t> vis_counts;    // popcounts -> exclusive offsets, total, scan scratch
+    DevBuf<uint32_t> batch_counts;  // build counts over the depth order -> offsets, total, scratch
+    DevBuf<uint32_t> dkeys[2];      // depth words of the visible Gaussians
+    DevBuf<int32_t> dorder[2];      // their build-record indices
+    DevBuf<float4> records;         // build records of the visible Gaussians (2 x float4 each)
     void* sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
     int32_t sel = 0;
@@ -119,6 +133,9 @@ gr_status gr_renderer_destroy(gr_renderer* r) {
     r->packed.release(); r->sh16.release();
     r->tile_range.release(); r->keys[0].release(); r->keys[1].release();
     r->values[0].release(); r->values[1].release(); r->out.release();
+    r->vis_bits.release(); r->vis_counts.release(); r->batch_counts.release();
+    r->dkeys[0].release(); r->dkeys[1].release(); r->dorder[0].release(); r->dorder[1].release();
+    r->records.release();
     if (r->sort_temp) (void)hipFree(r->sort_temp);
     for (auto& e : r->ev) if (e) (void)hipEventDestroy(e);
     gr_context_destroy(r->ctx);
@@ -190,6 +207,17 @@ gr_status gr_renderer_upload_model(gr_renderer* r, const gr_model_host* m) {
     return GR_OK;
 }
 
+static gr_status ensure_temp(void*& p, size_t& have, size_t need) {
+    if (p && have >= need) return GR_OK;
+    if (p) R_HIP(hipFree(p));
+    p = nullptr;
+    have = 0;
+    const size_t want = need + need / 4;
+    R_HIP(hipMalloc(&p, want));
+    have = want;
+    return GR_OK;
+}
+
 gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t width, int32_t height,
                              uint8_t* out_rgba8, int32_t out_is_device, gr_render_stats* stats) {
     if (!r || !cam || !out_rgba8) return rfail(GR_ERR_INVALID_ARGUMENT, "gr_renderer_render: NULL argument");
@@ -199,97 +227,179 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     hipStream_t stream = ctx->stream;
     const FrameGeom geom = make_geom(width, height);
     const int tiles = geom.tiles_x * geom.tiles_y;
+    if (tiles >= (1 << 20)) return rfail(GR_ERR_INVALID_ARGUMENT, "frame has too many tiles");
     R_CHECK(r->tile_range.ensure((size_t)2 * tiles));
     uchar4* outp = reinterpret_cast<uchar4*>(out_rgba8);
     if (!out_is_device) {
         R_CHECK(r->out.ensure((size_t)width * height));
         outp = r->out.p;
     }
-    gr_global_args ga{};
-    ga.camera_data = *cam;
-    ga.position = reinterpret_cast<const float*>(r->position.p);
-    ga.scale_and_rotation = reinterpret_cast<const float*>(r->scale_rot.p);
-    ga.color = reinterpret_cast<float*>(r->color.p);
-    ga.spherical_harmonics_degree = r->degree;
-    ga.spherical_harmonics_count = r->sh_count;
-    ga.spherical_harmonics = r->precision != GR_PRECISION_FP32 ? (const void*)r->sh16.p : (const void*)r->sh.p;
-    ga.packed_scale_opacity = r->precision != GR_PRECISION_FP32 ? (const void*)r->packed.p : nullptr;
-    ga.position_clip_xy = reinterpret_cast<float*>(r->clip_xy.p);
-    ga.position_clip_z = r->clip_z.p;
-    ga.conic = reinterpret_cast<float*>(r->conic.p);
-    ga.screen_ellipse = reinterpret_cast<float*>(r->ellipse.p);
-    ga.splat_count = r->n;
-    ga.back_buffer = reinterpret_cast<uint8_t*>(outp);
-    ga.tile_range = r->tile_range.p;
-    ga.width = width;
-    ga.height = height;
-    ga.sh_precision = r->precision;
-    R_CHECK(gr_set_global_args(ctx, &ga));
+    const int n = r->n;
+    const int batches = (n + kBuildBatch - 1) / kBuildBatch;
+    const size_t nn = (size_t)std::max(n, 1);
+    R_CHECK(r->vis_bits.ensure((size_t)std::max(batches, 1)));
+    R_CHECK(r->vis_counts.ensure((size_t)batches + 2 + scan_scratch_words(batches)));
+    R_CHECK(r->batch_counts.ensure((size_t)batches + 2 + scan_scratch_words(batches)));
+    for (int k = 0; k < 2; ++k) {
+        R_CHECK(r->dkeys[k].ensure(nn));
+        R_CHECK(r->dorder[k].ensure(nn));
+    }
+    R_CHECK(r->records.ensure(2 * nn));
+    R_CHECK(ensure_temp(r->sort_temp, r->sort_temp_bytes, radix_sort_temp_bytes((int32_t)nn, 0, 32)));
 
-    gr_tile_list_args tl{};
-    tl.keys = r->keys[0].p;
-    tl.values = r->values[0].p;
-    tl.capacity = (int32_t)std::min<size_t>(r->keys[0].cap, (size_t)0x7fffffff);
-    R_CHECK(gr_set_tile_list_args(ctx, &tl));
+    ViewDesc vd{};
+    vd.cam = camera_args(*cam);
+    vd.geom = geom;
+    vd.tile_base = 0;
+    vd.tiles = tiles;
+    vd.out = outp;
+    uint32_t* vis_total = r->vis_counts.p + batches;
+    uint32_t* k_total = r->batch_counts.p + batches;
 
     R_HIP(hipEventRecord(r->ev[0], stream));
-    if (r->degree > 0) R_CHECK(gr_evaluate_spherical_harmonics(ctx, nullptr, r->n));
+    if (r->degree > 0 && n > 0) {
+        ShArgs a{};
+        a.position = r->position.p;
+        a.sh = r->precision != GR_PRECISION_FP32 ? (const void*)r->sh16.p : (const void*)r->sh.p;
+        a.packed = r->precision != GR_PRECISION_FP32 ? r->packed.p : nullptr;
+        a.color = r->color.p;
+        a.count = n;
+        a.sh_count = r->sh_count;
+        a.nviews = 1;
+        a.v0 = vd;
+        R_HIP(launch_sh(a, r->degree, r->precision, stream));
+    }
     R_HIP(hipEventRecord(r->ev[1], stream));
-    R_CHECK(gr_evaluate_splat_clip_data(ctx, nullptr, r->n));
+    {
+        ProjectArgs a{};
+        a.position = r->position.p;
+        a.scale_rotation = r->scale_rot.p;
+        a.packed = r->packed.p;
+        a.precision = r->precision;
+        a.clip_xy = r->clip_xy.p;
+        a.clip_z = r->clip_z.p;
+        a.ellipse = r->ellipse.p;
+        a.conic = r->conic.p;
+        a.vis_bits = r->vis_bits.p;
+        a.vis_counts = r->vis_counts.p;
+        a.batches = batches;
+        a.count = n;
+        a.nviews = 1;
+        a.v0 = vd;
+        R_HIP(launch_project(a, stream));
+    }
     R_HIP(hipEventRecord(r->ev[2], stream));
-
-    int32_t size = 0;
-    int64_t need = 0;
-    R_CHECK(gr_build_tile_list(ctx, nullptr, 0, tl.capacity, &size, &need));
-    if (need > tl.capacity) {
-        if (need >= (int64_t)(1 << 30))
-            return rfail(GR_ERR_CAPACITY, "tile list of " + std::to_string(need) + " entries exceeds 2^30");
-        const size_t want = (size_t)need;
-        R_CHECK(r->keys[0].ensure(want, 1.25));
-        R_CHECK(r->keys[1].ensure(want, 1.25));
-        R_CHECK(r->values[0].ensure(want, 1.25));
-        R_CHECK(r->values[1].ensure(want, 1.25));
-        tl.keys = r->keys[0].p;
-        tl.values = r->values[0].p;
-        tl.capacity = (int32_t)std::min<size_t>(r->keys[0].cap, (size_t)0x7fffffff);
-        R_CHECK(gr_set_tile_list_args(ctx, &tl));
-        R_CHECK(gr_build_tile_list(ctx, nullptr, 0, tl.capacity, &size, &need));
+    // depth order of the visible Gaussians
+    R_HIP(launch_build_scan(r->vis_counts.p, batches, vis_total, vis_total + 1, stream));
+    int32_t dsel = 0;
+    {
+        CompactArgs a{};
+        a.vis_bits = r->vis_bits.p;
+        a.vis_offsets = r->vis_counts.p;
+        a.clip_xy = r->clip_xy.p;
+        a.clip_z = r->clip_z.p;
+        a.ellipse = r->ellipse.p;
+        a.records = r->records.p;
+        a.depth_keys = r->dkeys[0].p;
+        a.order = r->dorder[0].p;
+        a.count = n;
+        a.batches = batches;
+        a.nviews = 1;
+        R_HIP(launch_compact_depth(a, stream));
+        uint32_t* dk[2] = {r->dkeys[0].p, r->dkeys[1].p};
+        int32_t* dv[2] = {r->dorder[0].p, r->dorder[1].p};
+        R_HIP(radix_sort_pairs_u32(dk, dv, &dsel, n, vis_total, 0, 32, r->sort_temp, r->sort_temp_bytes, stream));
     }
     R_HIP(hipEventRecord(r->ev[3], stream));
-
-    gr_double_buffer_u64 kb{{r->keys[0].p, r->keys[1].p}, 0};
-    gr_double_buffer_i32 vb{{r->values[0].p, r->values[1].p}, 0};
-    R_CHECK(gr_sort_tile_list(ctx, nullptr, size, &r->sort_temp, &r->sort_temp_bytes, &kb, &vb));
+    // tile list over the depth order
+    BuildArgs b{};
+    b.clip_xy = r->clip_xy.p;
+    b.clip_z = r->clip_z.p;
+    b.ellipse = r->ellipse.p;
+    b.batch_counts = r->batch_counts.p;
+    b.keys = r->keys[0].p;
+    b.values = r->values[0].p;
+    b.order = r->dorder[dsel].p;
+    b.records = r->records.p;
+    b.order_count = vis_total;
+    b.count = n;
+    b.capacity = (int32_t)std::min<size_t>(r->keys[0].cap, (size_t)0x7fffffff);
+    b.batches = batches;
+    b.nviews = 1;
+    b.v0 = vd;
+    R_HIP(launch_build_count(b, stream));
+    R_HIP(launch_build_scan(r->batch_counts.p, batches, k_total, k_total + 1, stream));
+    R_HIP(hipMemcpyAsync(ctx->pinned, k_total, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    R_HIP(hipStreamSynchronize(stream));
+    const int64_t need = (int64_t)ctx->pinned[0];
+    if (need >= (int64_t)(1 << 30))
+        return rfail(GR_ERR_CAPACITY, "tile list of " + std::to_string(need) + " entries exceeds 2^30");
+    if (need > b.capacity || !r->keys[0].p) {
+        const size_t want = (size_t)std::max<int64_t>(need, 1);
+        for (int k = 0; k < 2; ++k) {
+            R_CHECK(r->keys[k].ensure(want, 1.25));
+            R_CHECK(r->values[k].ensure(want, 1.25));
+        }
+        b.keys = r->keys[0].p;
+        b.values = r->values[0].p;
+        b.capacity = (int32_t)std::min<size_t>(r->keys[0].cap, (size_t)0x7fffffff);
+    }
+    const int32_t size = (int32_t)need;
+    if (size > 0) R_HIP(launch_build_emit(b, stream));
     R_HIP(hipEventRecord(r->ev[4], stream));
-    r->sel = kb.selector;
-    tl.keys = kb.buffers[kb.selector];
-    tl.values = vb.buffers[vb.selector];
-    tl.size = size;
-    R_CHECK(gr_set_tile_list_args(ctx, &tl));
-    R_CHECK(gr_evaluate_tile_range(ctx, nullptr, size));
-    R_CHECK(gr_rasterize_tile(ctx, nullptr));
+    // tile bits
+    int32_t sel = 0;
+    if (size > 1) {
+        R_CHECK(ensure_temp(r->sort_temp, r->sort_temp_bytes, radix_sort_temp_bytes(size, 32, 32 + tile_bits_for(tiles))));
+        uint64_t* kb[2] = {r->keys[0].p, r->keys[1].p};
+        int32_t* vb[2] = {r->values[0].p, r->values[1].p};
+        R_HIP(radix_sort_pairs(kb, vb, &sel, size, nullptr, 32, 32 + tile_bits_for(tiles), r->sort_temp,
+                               r->sort_temp_bytes, stream));
+    }
     R_HIP(hipEventRecord(r->ev[5], stream));
+    r->sel = sel;
+    R_HIP(launch_fill_i32(r->tile_range.p, -1, 2 * (int64_t)tiles, stream));
+    if (size > 0) {
+        RangeArgs ra{};
+        ra.keys = r->keys[sel].p;
+        ra.tile_range = r->tile_range.p;
+        ra.size_dev = nullptr;
+        ra.size = size;
+        ra.tiles = tiles;
+        R_HIP(launch_ranges(ra, stream));
+    }
+    {
+        RasterArgs a{};
+        a.tile_range = r->tile_range.p;
+        a.values = r->values[sel].p;
+        a.clip_xy = r->clip_xy.p;
+        a.conic = r->conic.p;
+        a.color = r->color.p;
+        a.count = n;
+        a.tiles = tiles;
+        a.tiles_per_xcd = (tiles + 7) / 8;
+        a.nviews = 1;
+        a.v0 = vd;
+        R_HIP(launch_raster(a, stream));
+    }
+    R_HIP(hipEventRecord(r->ev[6], stream));
     if (!out_is_device)
         R_HIP(hipMemcpyAsync(out_rgba8, outp, (size_t)width * height * 4, hipMemcpyDeviceToHost, stream));
-    R_HIP(hipEventRecord(r->ev[6], stream));
-    R_HIP(hipEventSynchronize(r->ev[6]));
+    R_HIP(hipEventRecord(r->ev[7], stream));
+    R_HIP(hipEventSynchronize(r->ev[7]));
     r->last_k = need;
     r->last_w = width;
     r->last_h = height;
     if (stats) {
-        float t[6];
-        R_HIP(hipEventElapsedTime(&t[0], r->ev[0], r->ev[1]));
-        R_HIP(hipEventElapsedTime(&t[1], r->ev[1], r->ev[2]));
-        R_HIP(hipEventElapsedTime(&t[2], r->ev[2], r->ev[3]));
-        R_HIP(hipEventElapsedTime(&t[3], r->ev[3], r->ev[4]));
-        R_HIP(hipEventElapsedTime(&t[4], r->ev[4], r->ev[5]));
-        R_HIP(hipEventElapsedTime(&t[5], r->ev[0], r->ev[5]));
+        float t[7];
+        for (int k = 0; k < 6; ++k) R_HIP(hipEventElapsedTime(&t[k], r->ev[k], r->ev[k + 1]));
+        R_HIP(hipEventElapsedTime(&t[6], r->ev[0], r->ev[6]));
         stats->sh_eval_ms = t[0];
         stats->project_ms = t[1];
-        stats->build_tile_ms = t[2];
-        stats->sort_ms = t[3];
-        stats->raster_ms = t[4];
-        stats->total_ms = t[5];
+        stats->sort_ms = t[2] + t[4];       // depth order + tile-bit sort
+        stats->build_tile_ms = t[3];        // count, scan, K readback, emit
+        stats->raster_ms = t[5];            // ranges + raster
+        stats->total_ms = t[6];
         stats->tile_list_size = need;
     }
     return GR_OK;

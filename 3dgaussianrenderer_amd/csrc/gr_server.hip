@@ -6,10 +6,14 @@
 // thread owns every HIP call of the render path.  Each slot is a stream plus
 // a grow-only workspace; a batch (up to max_views requests of one scene, any
 // mix of resolutions) runs on one slot in two phases:
-//   A  camera block H2D -> SH (all views) -> projection (all views) ->
-//      build count -> scan -> 4-byte K readback
-//   B  emit -> onesweep sort (device-side K) -> ranges -> raster (all views)
-//      -> images D2H into pinned staging
+//   A  camera block H2D -> SH (all views) -> projection (all views, with the
+//      visibility ballot) -> compaction of every view's visible Gaussians and
+//      one 32-bit sort of them all by depth word -> build count over that
+//      depth order -> scan -> 4-byte K readback
+//   B  emit (depth order) -> onesweep sort of the tile bits (device-side K)
+//      -> ranges -> raster (all views) -> images D2H into pinned staging
+// Tile ids are global over the batch's views, so one sort serves all views,
+// and within a tile entries keep (depth word, index) order.
 // Between A and B the host reads K and grows the slot's tile-list buffers
 // within the HBM budget (exact admission).  Both phases are captured once
 // per (scene, batch shape, workspace) as hipGraphs and replayed; the camera
@@ -134,7 +138,11 @@ enum class SlotState { Idle, PhaseA, WaitMemory, PhaseB };
 
 struct Workspace {
     Dev color, xy, z, ell, con;      // [V][N]
-    Dev counts;                       // V*batches + 1 + scan scratch
+    Dev vis_bits, vis_counts;         // [V][batches] ballots; popcounts -> offsets, total, scan scratch
+    Dev dkeys0, dkeys1, dorder0, dorder1;   // [<= V*N] depth words and record indices of visible Gaussians
+    Dev records;                      // [<= V*N][2] float4 build records
+    Dev dsort_temp;                   // depth sort scratch
+    Dev counts;                       // build counts over the depth order (ceil(V*N/64)) + total + scratch
     Dev keys0, keys1, vals0, vals1;   // tile lists
     Dev sort_temp;
     Dev ranges;                       // 2 * tiles
@@ -142,6 +150,10 @@ struct Workspace {
     Dev views;                        // ViewDesc[V]
     Pinned views_host, k_host, img_host;
     uint64_t generation = 0;          // bumps when any device pointer changes
+    std::vector<Dev*> devs() {
+        return {&color, &xy, &z, &ell, &con, &vis_bits, &vis_counts, &dkeys0, &dkeys1, &dorder0, &dorder1,
+                &records, &dsort_temp, &counts, &keys0, &keys1, &vals0, &vals1, &sort_temp, &ranges, &images, &views};
+    }
 };
 
 struct GraphKey {
@@ -208,16 +220,6 @@ struct gr_server {
 
 namespace {
 
-CameraArgs cam_args(const gr_camera_data& c) {
-    CameraArgs a;
-    a.view = load_m4(c.view);
-    a.fov_cotangent = v2{c.fov_cotangent[0], c.fov_cotangent[1]};
-    a.depth_scale_bias = v2{c.depth_scale_bias[0], c.depth_scale_bias[1]};
-    a.position = v3{c.position[0], c.position[1], c.position[2]};
-    a.aspect = c.aspect;
-    return a;
-}
-
 Scene* scene_ptr(gr_server* s, int32_t id) {
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->scenes.find(id);
@@ -257,12 +259,21 @@ std::vector<PhaseANeed> phase_a_needs(Slot& sl, const Scene& sc, size_t nviews, 
     Workspace& w = sl.ws;
     const int64_t V = (int64_t)nviews, N = sc.n;
     const int64_t batches = (N + kBuildBatch - 1) / kBuildBatch;
+    const int64_t ob = (V * N + kBuildBatch - 1) / kBuildBatch;
     return {{&w.color, V * N * 16},
             {&w.xy, V * N * 8},
             {&w.z, V * N * 4},
             {&w.ell, V * N * 16},
             {&w.con, V * N * 16},
-            {&w.counts, 4 * (V * batches + 2 + scan_scratch_words((int32_t)(V * batches)))},
+            {&w.vis_bits, V * batches * 8},
+            {&w.vis_counts, 4 * (V * batches + 2 + scan_scratch_words((int32_t)(V * batches)))},
+            {&w.dkeys0, V * N * 4},
+            {&w.dkeys1, V * N * 4},
+            {&w.dorder0, V * N * 4},
+            {&w.dorder1, V * N * 4},
+            {&w.records, V * N * 32},
+            {&w.dsort_temp, (int64_t)radix_sort_temp_bytes((int32_t)std::max<int64_t>(V * N, 1), 0, 32)},
+            {&w.counts, 4 * (ob + 2 + scan_scratch_words((int32_t)ob))},
             {&w.images, img_bytes},
             {&w.ranges, 8 * (int64_t)tiles_total},
             {&w.views, V * (int64_t)sizeof(ViewDesc)}};
@@ -288,19 +299,17 @@ int64_t fresh_workspace_bytes(Slot& sl, const Scene& sc, size_t entries) {
     return b;
 }
 
-int64_t workspace_bytes(const Slot& sl) {
-    const Workspace& w = sl.ws;
+int64_t workspace_bytes(Slot& sl) {
+    Workspace& w = sl.ws;
     int64_t b = 0;
-    for (const Dev* d : {&w.color, &w.xy, &w.z, &w.ell, &w.con, &w.counts, &w.keys0, &w.keys1, &w.vals0, &w.vals1,
-                         &w.sort_temp, &w.ranges, &w.images, &w.views})
+    for (const Dev* d : w.devs())
         b += (int64_t)d->bytes;
     return b;
 }
 
 void release_workspace(gr_server* s, Slot& sl) {
     Workspace& w = sl.ws;
-    for (Dev* d : {&w.color, &w.xy, &w.z, &w.ell, &w.con, &w.counts, &w.keys0, &w.keys1, &w.vals0, &w.vals1,
-                   &w.sort_temp, &w.ranges, &w.images, &w.views})
+    for (Dev* d : w.devs())
         d->release(s->workspace);
     ++w.generation;
 }
@@ -317,6 +326,31 @@ gr_status ensure_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     if (grew) ++w.generation;
     s->note_peak();
     return GR_OK;
+}
+
+// Build over the depth order of the batch (count in phase A, emit in phase B).
+BuildArgs ordered_build_args(Slot& sl, const Scene& sc) {
+    Workspace& w = sl.ws;
+    const int V = (int)sl.batch.size();
+    const int N = sc.n;
+    const int nbv = V * ((N + kBuildBatch - 1) / kBuildBatch);
+    BuildArgs b{};
+    b.clip_xy = w.xy.as<float2>();
+    b.clip_z = w.z.as<float>();
+    b.ellipse = w.ell.as<float4>();
+    b.batch_counts = w.counts.as<uint32_t>();
+    b.keys = w.keys0.as<uint64_t>();
+    b.values = w.vals0.as<int32_t>();
+    b.order = w.dorder0.as<int32_t>();   // 4 passes of 8 bits: the sorted order is back in buffer 0
+    b.records = w.records.as<float4>();
+    b.order_count = w.vis_counts.as<uint32_t>() + nbv;
+    b.count = N;
+    b.capacity = (int32_t)sl.capacity;
+    b.batches = (int32_t)(((int64_t)V * N + kBuildBatch - 1) / kBuildBatch);
+    b.nviews = V;
+    b.v0 = w.views_host.as<ViewDesc>()[0];
+    b.views = w.views.as<ViewDesc>();
+    return b;
 }
 
 // Launch sequences (shared by eager runs and graph capture).
@@ -345,6 +379,8 @@ gr_status enqueue_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
             S_HIP(hipMemcpyAsync(w.color.as<float4>() + (size_t)v * N, sc.color0.p, (size_t)N * 16,
                                  hipMemcpyDeviceToDevice, sl.stream));
     }
+    const int nbv = V * batches;
+    uint32_t* vis = w.vis_counts.as<uint32_t>();
     ProjectArgs p{};
     p.position = sc.position.as<float4>();
     p.scale_rotation = sc.scale_rot.as<float4>();
@@ -354,27 +390,41 @@ gr_status enqueue_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     p.clip_z = w.z.as<float>();
     p.ellipse = w.ell.as<float4>();
     p.conic = w.con.as<float4>();
+    p.vis_bits = w.vis_bits.as<uint64_t>();
+    p.vis_counts = vis;
+    p.batches = batches;
     p.count = N;
     p.nviews = V;
     p.v0 = vhost[0];
     p.views = vdev;
     S_HIP(launch_project(p, sl.stream));
-    BuildArgs b{};
-    b.clip_xy = w.xy.as<float2>();
-    b.clip_z = w.z.as<float>();
-    b.ellipse = w.ell.as<float4>();
-    b.batch_counts = w.counts.as<uint32_t>();
-    b.count = N;
-    b.capacity = 0;
-    b.batches = batches;
-    b.nviews = V;
-    b.v0 = vhost[0];
-    b.views = vdev;
+    // depth order of all views' visible Gaussians
+    S_HIP(launch_build_scan(vis, nbv, vis + nbv, vis + nbv + 2, sl.stream));
+    CompactArgs c{};
+    c.vis_bits = w.vis_bits.as<uint64_t>();
+    c.vis_offsets = vis;
+    c.clip_xy = w.xy.as<float2>();
+    c.clip_z = w.z.as<float>();
+    c.ellipse = w.ell.as<float4>();
+    c.records = w.records.as<float4>();
+    c.depth_keys = w.dkeys0.as<uint32_t>();
+    c.order = w.dorder0.as<int32_t>();
+    c.count = N;
+    c.batches = batches;
+    c.nviews = V;
+    S_HIP(launch_compact_depth(c, sl.stream));
+    uint32_t* dk[2] = {w.dkeys0.as<uint32_t>(), w.dkeys1.as<uint32_t>()};
+    int32_t* dv[2] = {w.dorder0.as<int32_t>(), w.dorder1.as<int32_t>()};
+    int32_t dsel = 0;
+    S_HIP(radix_sort_pairs_u32(dk, dv, &dsel, V * N, vis + nbv, 0, 32, w.dsort_temp.p, w.dsort_temp.bytes,
+                               sl.stream));
+    if (dsel != 0) return sfail(GR_ERR_HIP, "depth sort left its result in the second buffer");
+    BuildArgs b = ordered_build_args(sl, sc);
     S_HIP(launch_build_count(b, sl.stream));
     uint32_t* counts = w.counts.as<uint32_t>();
-    const int nb = V * batches;
-    S_HIP(launch_build_scan(counts, nb, counts + nb, counts + nb + 2, sl.stream));
-    S_HIP(hipMemcpyAsync(w.k_host.p, counts + nb, 4, hipMemcpyDeviceToHost, sl.stream));
+    const int ob = b.batches;
+    S_HIP(launch_build_scan(counts, ob, counts + ob, counts + ob + 2, sl.stream));
+    S_HIP(hipMemcpyAsync(w.k_host.p, counts + ob, 4, hipMemcpyDeviceToHost, sl.stream));
     return GR_OK;
 }
 
@@ -382,29 +432,16 @@ gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
     Workspace& w = sl.ws;
     const int V = (int)sl.batch.size();
     const int N = sc.n;
-    const int batches = (N + kBuildBatch - 1) / kBuildBatch;
     const ViewDesc* vhost = w.views_host.as<ViewDesc>();
     ViewDesc* vdev = w.views.as<ViewDesc>();
     uint32_t* counts = w.counts.as<uint32_t>();
-    const int nb = V * batches;
-    BuildArgs b{};
-    b.clip_xy = w.xy.as<float2>();
-    b.clip_z = w.z.as<float>();
-    b.ellipse = w.ell.as<float4>();
-    b.batch_counts = counts;
-    b.keys = w.keys0.as<uint64_t>();
-    b.values = w.vals0.as<int32_t>();
-    b.count = N;
-    b.capacity = (int32_t)sl.capacity;
-    b.batches = batches;
-    b.nviews = V;
-    b.v0 = vhost[0];
-    b.views = vdev;
+    BuildArgs b = ordered_build_args(sl, sc);
+    const int nb = b.batches;
     S_HIP(launch_build_emit(b, sl.stream));
     uint64_t* kb[2] = {w.keys0.as<uint64_t>(), w.keys1.as<uint64_t>()};
     int32_t* vb[2] = {w.vals0.as<int32_t>(), w.vals1.as<int32_t>()};
     int32_t sel = 0;
-    S_HIP(radix_sort_pairs(kb, vb, &sel, (int32_t)sl.capacity, counts + nb, 0, sl.end_bit, w.sort_temp.p,
+    S_HIP(radix_sort_pairs(kb, vb, &sel, (int32_t)sl.capacity, counts + nb, 32, sl.end_bit, w.sort_temp.p,
                            w.sort_temp.bytes, sl.stream));
     S_HIP(launch_fill_i32(w.ranges.as<int32_t>(), -1, 2 * (int64_t)sl.tiles_total, sl.stream));
     RangeArgs r{};
@@ -543,7 +580,7 @@ bool start_batch(gr_server* s, Slot& sl) {
     for (size_t v = 0; v < sl.batch.size(); ++v) {
         const gr_request& r = sl.batch[v].req;
         ViewDesc& d = vds[v];
-        d.cam = cam_args(r.camera);
+        d.cam = camera_args(r.camera);
         d.geom = make_geom(r.width, r.height);
         d.tile_base = tiles;
         d.tiles = d.geom.tiles_x * d.geom.tiles_y;
@@ -651,37 +688,14 @@ bool start_phase_b(gr_server* s, Slot& sl) {
             return false;
         }
         release_idle_workspaces(s, sl);
-        // Split the batch: phase A already produced everything for its first views (view-major
-        // buffers, global tile ids in view order), and the scan's exclusive offset at the first
-        // dropped view is the tile-list size of the kept views.  The tail goes back to the queue.
-        const int batches = (sc->n + kBuildBatch - 1) / kBuildBatch;
-        while (s->over_budget(grow) && sl.batch.size() > 1) {
-            const size_t keep = sl.batch.size() / 2;
-            auto& q = s->queues[sl.scene];
-            for (size_t i = sl.batch.size(); i-- > keep;) q.push_front(sl.batch[i]);
-            sl.batch.resize(keep);
-            int32_t tiles = 0;
-            int64_t img = 0;
-            for (const Pending& pv : sl.batch) {
-                const FrameGeom g = make_geom(pv.req.width, pv.req.height);
-                tiles += g.tiles_x * g.tiles_y;
-                img += (int64_t)pv.req.width * pv.req.height * 4;
-            }
-            sl.tiles_total = tiles;
-            sl.img_bytes = img;
-            sl.end_bit = 32 + tile_bits_for(tiles);
-            uint32_t k = 0;
-            if (hipMemcpyAsync(&k, w.counts.as<uint32_t>() + keep * (size_t)batches, 4, hipMemcpyDeviceToHost,
-                               sl.stream) != hipSuccess ||
-                hipStreamSynchronize(sl.stream) != hipSuccess) {
-                complete(s, sl, GR_ERR_HIP);
-                sl.batch.clear();
-                sl.state = SlotState::Idle;
-                return true;
-            }
-            sl.k = k;
-            need = std::max<size_t>(sl.k, 1);
-            grow = growth_for(need);
+        if (s->over_budget(grow) && sl.batch.size() > 1) {
+            // run again with half as many views: phase A is redone for them (the entries of the
+            // batch's views interleave in depth order, so no prefix of this list is theirs)
+            release_workspace(s, sl);
+            drop_graphs(sl);
+            s->batch_limit[sl.scene] = (int)sl.batch.size() / 2;
+            requeue_front(s, sl);
+            return true;
         }
         if (s->over_budget(grow) && !s->over_budget(fresh_workspace_bytes(sl, *sc, need) - workspace_bytes(sl))) {
             // phase-A buffers oversized by earlier, larger batches are what is in the way: run the
@@ -812,8 +826,7 @@ gr_status gr_server_destroy(gr_server* s) {
         (void)hipStreamSynchronize(sl.stream);
         drop_graphs(sl);
         Workspace& w = sl.ws;
-        for (Dev* d : {&w.color, &w.xy, &w.z, &w.ell, &w.con, &w.counts, &w.keys0, &w.keys1, &w.vals0,
-                       &w.vals1, &w.sort_temp, &w.ranges, &w.images, &w.views})
+        for (Dev* d : w.devs())
             d->release(s->workspace);
         w.views_host.release();
         w.k_host.release();

@@ -1,5 +1,6 @@
-// gr_sort.hip -- stable LSD radix sort of (uint64 key, int32 value) pairs for
-// gfx950, replacing cub::DeviceRadixSort::SortPairs (core/gaussian_render.cu:591-632).
+// gr_sort.hip -- stable LSD radix sort of (uint64 or uint32 key, int32 value)
+// pairs for gfx950, replacing cub::DeviceRadixSort::SortPairs
+// (core/gaussian_render.cu:591-632).
 //
 // Onesweep structure (Adinets & Merrill 2022, "Onesweep: A Faster Least
 // Significant Digit Radix Sort for GPUs"), written for wave64:
@@ -91,13 +92,15 @@ constexpr int kHistThreads = 256;
 constexpr int kHistSeg = 16;
 constexpr int kHistChunk = kHistThreads * kHistSeg;
 
-__global__ __launch_bounds__(kHistThreads) void sort_hist_kernel(const uint64_t* __restrict__ keys, int32_t n,
+template <typename KeyT>
+__global__ __launch_bounds__(kHistThreads) void sort_hist_kernel(const KeyT* __restrict__ keys, int32_t n,
                                                                  const uint32_t* count_dev, int begin_bit,
                                                                  int end_bit, int passes, uint32_t* hist) {
-    __shared__ uint32_t s_h[kMaxPasses][kRadix];
+    constexpr int kP = (int)sizeof(KeyT);   // at most one 8-bit pass per key byte
+    __shared__ uint32_t s_h[kP][kRadix];
     if (count_dev) n = (int32_t)min((uint32_t)n, *count_dev);
-    __shared__ uint64_t s_k[kHistChunk + kHistChunk / kHistSeg];
-    for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += blockDim.x) (&s_h[0][0])[i] = 0u;
+    __shared__ KeyT s_k[kHistChunk + kHistChunk / kHistSeg];
+    for (int i = threadIdx.x; i < kP * kRadix; i += blockDim.x) (&s_h[0][0])[i] = 0u;
     for (int64_t base = (int64_t)blockIdx.x * kHistChunk; base < n; base += (int64_t)gridDim.x * kHistChunk) {
         const int m = (int)min((int64_t)kHistChunk, (int64_t)n - base);
         __syncthreads();
@@ -107,17 +110,17 @@ __global__ __launch_bounds__(kHistThreads) void sort_hist_kernel(const uint64_t*
             if (i < m) s_k[i + i / kHistSeg] = keys[base + i];
         }
         __syncthreads();
-        uint32_t cur[kMaxPasses], cnt[kMaxPasses];
+        uint32_t cur[kP], cnt[kP];
 #pragma unroll
-        for (int p = 0; p < kMaxPasses; ++p) {
+        for (int p = 0; p < kP; ++p) {
             cur[p] = 0u;
             cnt[p] = 0u;
         }
         const int my = threadIdx.x * kHistSeg;
         for (int j = 0; j < kHistSeg && my + j < m; ++j) {
-            const uint64_t k = s_k[my + j + threadIdx.x];
+            const KeyT k = s_k[my + j + threadIdx.x];
 #pragma unroll
-            for (int p = 0; p < kMaxPasses; ++p) {
+            for (int p = 0; p < kP; ++p) {
                 if (p < passes) {
                     const int shift = begin_bit + 8 * p;
                     const int bits = min(8, end_bit - shift);
@@ -132,7 +135,7 @@ __global__ __launch_bounds__(kHistThreads) void sort_hist_kernel(const uint64_t*
             }
         }
 #pragma unroll
-        for (int p = 0; p < kMaxPasses; ++p)
+        for (int p = 0; p < kP; ++p)
             if (p < passes && cnt[p]) atomicAdd(&s_h[p][cur[p]], cnt[p]);
     }
     __syncthreads();
@@ -168,8 +171,9 @@ __device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+template <typename KeyT>
 __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
-    const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+    const KeyT* __restrict__ kin, const int32_t* __restrict__ vin, KeyT* __restrict__ kout,
     int32_t* __restrict__ vout, int32_t n, const uint32_t* count_dev, int shift, int bits,
     const uint32_t* __restrict__ digit_off, uint32_t* status, uint32_t* status_next, uint32_t* ticket) {
     __shared__ uint32_t s_part;
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     __shared__ uint32_t s_dstart[kRadix];
     __shared__ uint32_t s_gbase[kRadix];
     __shared__ uint32_t s_wsum[kRadix / 64];
-    __shared__ uint64_t s_keys[kSortTile];
+    __shared__ KeyT s_keys[kSortTile];
     __shared__ int32_t s_vals[kSortTile];
 
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -192,13 +196,13 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     const int64_t base = (int64_t)part * kSortTile;
     const int64_t wbase = base + (int64_t)w * (64 * kSortItems) + lane;
     const uint32_t dmask = (1u << bits) - 1u;
-    uint64_t key[kSortItems];
+    KeyT key[kSortItems];
     int32_t val[kSortItems];
     uint32_t rank[kSortItems];
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
         const int64_t idx = wbase + i * 64;
-        key[i] = idx < n ? kin[idx] : 0ull;
+        key[i] = idx < n ? kin[idx] : (KeyT)0;
         val[i] = idx < n ? vin[idx] : 0;
     }
     // per-wave stable ranking: peers = lanes with the same digit (8 ballots)
@@ -289,7 +293,7 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     __syncthreads();
     const int nvalid = (int)min((int64_t)kSortTile, (int64_t)n - base);
     for (int j = t; j < nvalid; j += kSortThreads) {
-        const uint64_t k = s_keys[j];
+        const KeyT k = s_keys[j];
         const uint32_t d = (uint32_t)(k >> shift) & dmask;
         const uint32_t gpos = s_gbase[d] + (uint32_t)j;
         kout[gpos] = k;
@@ -297,11 +301,13 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     }
 }
 
-hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
-                            const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
-                            size_t temp_bytes, hipStream_t s) {
+template <typename KeyT>
+static hipError_t sort_pairs_t(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                               const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                               size_t temp_bytes, hipStream_t s) {
+    constexpr int kKeyBits = 8 * (int)sizeof(KeyT);
     if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
-    if ((uint32_t)count > kCountMask || begin_bit < 0 || end_bit > 64) return hipErrorInvalidValue;
+    if ((uint32_t)count > kCountMask || begin_bit < 0 || end_bit > kKeyBits) return hipErrorInvalidValue;
     const int passes = (end_bit - begin_bit + 7) / 8;
     if (passes > kMaxPasses) return hipErrorInvalidValue;
     const SortLayout l = sort_layout(count);
@@ -320,20 +326,32 @@ hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel,
                        status[0], status_n);
     int hist_blocks = (count + kHistChunk - 1) / kHistChunk;
     if (hist_blocks > 1024) hist_blocks = 1024;
-    hipLaunchKernelGGL(sort_hist_kernel, dim3(hist_blocks), dim3(kHistThreads), 0, s, keys[*sel], count,
+    hipLaunchKernelGGL(sort_hist_kernel<KeyT>, dim3(hist_blocks), dim3(kHistThreads), 0, s, keys[*sel], count,
                        count_dev, begin_bit, end_bit, passes, hist);
     hipLaunchKernelGGL(sort_hist_scan_kernel, dim3(passes), dim3(256), 0, s, hist);
     for (int p = 0; p < passes; ++p) {
         const int shift = begin_bit + 8 * p;
         const int bits = min(8, end_bit - shift);
         const int cur = *sel;
-        hipLaunchKernelGGL(onesweep_kernel, dim3(l.partitions), dim3(kSortThreads), 0, s, keys[cur],
+        hipLaunchKernelGGL(onesweep_kernel<KeyT>, dim3(l.partitions), dim3(kSortThreads), 0, s, keys[cur],
                            values[cur], keys[cur ^ 1], values[cur ^ 1], count, count_dev, shift, bits,
                            hist + p * kRadix, status[p & 1], (p + 1 < passes) ? status[(p + 1) & 1] : nullptr,
                            tickets + p);
         *sel = cur ^ 1;
     }
     return hipGetLastError();
+}
+
+hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                            const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                            size_t temp_bytes, hipStream_t s) {
+    return sort_pairs_t<uint64_t>(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
+}
+
+hipError_t radix_sort_pairs_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                                const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                                size_t temp_bytes, hipStream_t s) {
+    return sort_pairs_t<uint32_t>(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
 }
 
 }  // namespace gr

@@ -147,6 +147,47 @@ hipError_t launch_sh(const ShArgs& a, int degree, int precision, hipStream_t s) 
     }
 }
 
+// Footprint: getAABBRect core/gaussian_render.cu:274-298 and the tile rect of
+// :379-389 (used by the projection's visibility ballot and by the build).
+struct Footprint {
+    int x0, y0, nx, ny;
+};
+
+__device__ __forceinline__ Footprint footprint(float cx, float cy, float csx, float csy, float ex,
+                                               float ey, const FrameGeom& g) {
+    const float rx = csx * ex, ry = csy * ex;        // principal axis
+    const float ux = csy * ey, uy = (-csx) * ey;     // minor axis
+    float mnx = 1e12f, mny = 1e12f, mxx = -1e12f, mxy = -1e12f;
+#pragma unroll
+    for (int j = 0; j != 4; ++j) {
+        const int bottom = j & 1, top = j >> 1;
+        const float r = (float)(bottom ^ top) * 2.0f - 1.0f;
+        const float u = (float)top * 2.0f - 1.0f;
+        const float vx = rx * r + ux * u;
+        const float vy = ry * r + uy * u;
+        mnx = gmin(mnx, vx);
+        mny = gmin(mny, vy);
+        mxx = gmax(mxx, vx);
+        mxy = gmax(mxy, vy);
+    }
+    mnx = mnx + cx;
+    mny = mny + cy;
+    mxx = mxx + cx;
+    mxy = mxy + cy;
+    const float tfx = ((mnx + 1.0f) * 0.5f) * g.tiles_per_axis_x;
+    const float tfy = ((mny + 1.0f) * 0.5f) * g.tiles_per_axis_y;
+    const float tfz = ((mxx + 1.0f) * 0.5f) * g.tiles_per_axis_x;
+    const float tfw = ((mxy + 1.0f) * 0.5f) * g.tiles_per_axis_y;
+    Footprint f;
+    f.x0 = iclamp(f2i_sat(floorf(tfx)), 0, g.tiles_x);
+    f.y0 = iclamp(f2i_sat(floorf(tfy)), 0, g.tiles_y);
+    const int x1 = iclamp(f2i_sat(ceilf(tfz)), 0, g.tiles_x);
+    const int y1 = iclamp(f2i_sat(ceilf(tfw)), 0, g.tiles_y);
+    f.nx = x1 - f.x0;
+    f.ny = y1 - f.y0;
+    return f;
+}
+
 // ---------------------------------------------------------------------------
 // EWA projection: core/gaussian_render.cu:135-215 (quaternion decode :45-48).
 // The eigenvector (cos, sin) is written directly instead of going through
@@ -241,6 +282,16 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     a.clip_z[o] = clip.z;
     a.ellipse[o] = make_float4(e0.x, e0.y, extx, exty);
     a.conic[o] = make_float4(conx, cony, conz, 0.0f);
+    if (a.vis_bits) {
+        // lanes of a wave are Gaussians 64b .. 64b+63 (block of 256, i contiguous)
+        const Footprint f = footprint(clip.x, clip.y, e0.x, e0.y, extx, exty, geom);
+        const uint64_t m = __ballot(f.nx > 0 && f.ny > 0);
+        if ((threadIdx.x & 63) == 0) {
+            const size_t b = (size_t)vi * a.batches + (i >> 6);
+            a.vis_bits[b] = m;
+            a.vis_counts[b] = (uint32_t)__popcll(m);
+        }
+    }
     }
 }
 
@@ -263,47 +314,11 @@ hipError_t launch_project(const ProjectArgs& a, hipStream_t s) {
 // convertToEllipseCoordinates :234-239 and intersectsUnitCircle :241-248.
 // Key: getKey :306-310.  The reference lets atomics decide entry order
 // (:485, :498); here entries follow Gaussian index, then row-major candidate
-// order, so the stable sort that follows is deterministic.
+// order, so the stable sort that follows is deterministic.  In the
+// depth-ordered walk (ORDERED) positions follow the depth-sorted Gaussian
+// list instead, so entries come out in (depth word, index) order and a wave's
+// 64 Gaussians may belong to different views.
 // ---------------------------------------------------------------------------
-struct Footprint {
-    int x0, y0, nx, ny;
-};
-
-__device__ __forceinline__ Footprint footprint(float cx, float cy, float csx, float csy, float ex,
-                                               float ey, const FrameGeom& g) {
-    const float rx = csx * ex, ry = csy * ex;        // principal axis
-    const float ux = csy * ey, uy = (-csx) * ey;     // minor axis
-    float mnx = 1e12f, mny = 1e12f, mxx = -1e12f, mxy = -1e12f;
-#pragma unroll
-    for (int j = 0; j != 4; ++j) {
-        const int bottom = j & 1, top = j >> 1;
-        const float r = (float)(bottom ^ top) * 2.0f - 1.0f;
-        const float u = (float)top * 2.0f - 1.0f;
-        const float vx = rx * r + ux * u;
-        const float vy = ry * r + uy * u;
-        mnx = gmin(mnx, vx);
-        mny = gmin(mny, vy);
-        mxx = gmax(mxx, vx);
-        mxy = gmax(mxy, vy);
-    }
-    mnx = mnx + cx;
-    mny = mny + cy;
-    mxx = mxx + cx;
-    mxy = mxy + cy;
-    const float tfx = ((mnx + 1.0f) * 0.5f) * g.tiles_per_axis_x;
-    const float tfy = ((mny + 1.0f) * 0.5f) * g.tiles_per_axis_y;
-    const float tfz = ((mxx + 1.0f) * 0.5f) * g.tiles_per_axis_x;
-    const float tfw = ((mxy + 1.0f) * 0.5f) * g.tiles_per_axis_y;
-    Footprint f;
-    f.x0 = iclamp(f2i_sat(floorf(tfx)), 0, g.tiles_x);
-    f.y0 = iclamp(f2i_sat(floorf(tfy)), 0, g.tiles_y);
-    const int x1 = iclamp(f2i_sat(ceilf(tfz)), 0, g.tiles_x);
-    const int y1 = iclamp(f2i_sat(ceilf(tfw)), 0, g.tiles_y);
-    f.nx = x1 - f.x0;
-    f.ny = y1 - f.y0;
-    return f;
-}
-
 // IEEE quotient a / b from rb = RN(1/b): q0 = RN(a*rb), e = a - b*q0 (exact
 // through the FMA), q = RN(q0 + e*rb) is the correctly rounded quotient when
 // nothing underflows or overflows (Markstein 1990, "Computation of elementary
@@ -379,35 +394,74 @@ __device__ __forceinline__ void divmod_small(int t, int n, int& q, int& r) {
     else if (r >= n) { ++q; r -= n; }
 }
 
-template <bool EMIT>
+template <bool EMIT, bool ORDERED>
 __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
     constexpr int W = kBuildThreads / 64;
     __shared__ int s_excl[W][64];
     __shared__ int4 s_rect[W][64];          // x0, y0, nx, depth word
     __shared__ EllipseRec s_ell[W][64];
+    __shared__ int4 s_tile[ORDERED ? W : 1][64];   // ORDERED: tile_base, tiles_x, tile clip x/y bits
+    __shared__ int s_gidx[ORDERED ? W : 1][64];    // ORDERED: Gaussian index within the scene
 
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int batch = blockIdx.x * W + w;
     const int first = batch * kBuildBatch;
-    if (first >= a.count) return;   // wave-uniform; no block barrier below
-    const int g = first + lane;
-    const int view = blockIdx.y;
+    int limit = a.count;
+    if constexpr (ORDERED) {
+        limit = (int)min(*a.order_count, (uint32_t)a.nviews * (uint32_t)a.count);
+        if (first >= limit) {   // wave-uniform; no block barrier below
+            if (!EMIT && lane == 0 && batch < a.batches) a.batch_counts[batch] = 0u;
+            return;
+        }
+    } else {
+        if (first >= a.count) return;
+    }
+    const int pos = first + lane;
+    int view, gi = -1;
+    float2 c = make_float2(0.0f, 0.0f);
+    float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t dw = 0u;
+    if constexpr (ORDERED) {
+        view = 0;
+        if (pos < limit) {
+            const int rp = a.order[pos];
+            const float4 r0 = a.records[2 * (size_t)rp];
+            const float4 r1 = a.records[2 * (size_t)rp + 1];
+            const int pg = __float_as_int(r1.w);
+            view = pg / a.count;
+            gi = pg - view * a.count;
+            c = make_float2(r0.x, r0.y);
+            e = make_float4(r0.z, r0.w, r1.x, r1.y);
+            dw = __float_as_uint(r1.z);
+        }
+    } else {
+        view = blockIdx.y;
+        if (pos < a.count) {
+            gi = pos;
+            const size_t vo = (size_t)view * a.count;
+            c = a.clip_xy[vo + gi];
+            e = a.ellipse[vo + gi];
+            if (EMIT) dw = depth_word(a.clip_z[vo + gi]);
+        }
+    }
     const ViewDesc& vd = view_of(view, a.v0, a.views);
     const FrameGeom& geom = vd.geom;
-    const size_t vo = (size_t)view * a.count;
 
     int cnt = 0;
-    if (g < a.count) {
-        const float2 c = a.clip_xy[vo + g];
-        const float4 e = a.ellipse[vo + g];
+    if (gi >= 0) {
         const Footprint f = footprint(c.x, c.y, e.x, e.y, e.z, e.w, geom);
         // the reference keeps max(0, w*h); a rect with w <= 0 or h <= 0 only
         // arises from NaN corners and never passes the exact test
         cnt = (f.nx > 0 && f.ny > 0) ? f.nx * f.ny : 0;
         if (cnt > 0) {
-            s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, EMIT ? (int)depth_word(a.clip_z[vo + g]) : 0);
+            s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, (int)dw);
             s_ell[w][lane] = EllipseRec{c.x, c.y, e.x, e.y, e.z, e.w, 1.0f / e.z, 1.0f / e.w};
+            if constexpr (ORDERED) {
+                s_tile[w][lane] = make_int4(vd.tile_base, geom.tiles_x, __float_as_int(geom.tile_clip_x),
+                                            __float_as_int(geom.tile_clip_y));
+                s_gidx[w][lane] = gi;
+            }
         }
     }
     int incl = cnt;
@@ -421,13 +475,14 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
     wave_sync();
 
     uint32_t running = 0;
-    const size_t bslot = (size_t)view * a.batches + batch;
+    const size_t bslot = ORDERED ? (size_t)batch : (size_t)view * a.batches + batch;
     const uint32_t out_base = EMIT ? a.batch_counts[bslot] : 0u;
     for (int base = 0; base < total; base += 64) {
         const int c = base + lane;
         bool pass = false;
         int o = 0, gx = 0, gy = 0;
         int4 rc = make_int4(0, 0, 1, 0);
+        uint32_t tile = 0;
         if (c < total) {
 #pragma unroll
             for (int step = 32; step >= 1; step >>= 1)
@@ -437,15 +492,25 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
             divmod_small(c - s_excl[w][o], rc.z, q, r);
             gx = rc.x + r;
             gy = rc.y + q;
-            pass = tile_overlap(gx, gy, s_ell[w][o], geom);
+            if constexpr (ORDERED) {
+                const int4 tv = s_tile[w][o];
+                FrameGeom og;
+                og.tile_clip_x = __int_as_float(tv.z);
+                og.tile_clip_y = __int_as_float(tv.w);
+                pass = tile_overlap(gx, gy, s_ell[w][o], og);
+                tile = (uint32_t)(tv.x + gy * tv.y + gx);
+            } else {
+                pass = tile_overlap(gx, gy, s_ell[w][o], geom);
+                tile = (uint32_t)(vd.tile_base + gy * geom.tiles_x + gx);
+            }
         }
         const uint64_t m = __ballot(pass);
         if (EMIT && pass) {
-            const uint32_t pos = out_base + running + (uint32_t)__popcll(m & lanemask_lt(lane));
-            if (pos < (uint32_t)a.capacity) {
-                const uint32_t tile = (uint32_t)(vd.tile_base + gy * geom.tiles_x + gx);
-                a.keys[pos] = ((uint64_t)tile << 32) | (uint32_t)rc.w;
-                a.values[pos] = first + o;
+            const uint32_t p = out_base + running + (uint32_t)__popcll(m & lanemask_lt(lane));
+            if (p < (uint32_t)a.capacity) {
+                a.keys[p] = ((uint64_t)tile << 32) | (uint32_t)rc.w;
+                if constexpr (ORDERED) a.values[p] = s_gidx[w][o];
+                else a.values[p] = first + o;
             }
         }
         running += (uint32_t)__popcll(m);
@@ -455,18 +520,54 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
 
 static dim3 build_grid(const BuildArgs& a) {
     constexpr int W = kBuildThreads / 64;
-    return dim3((a.batches + W - 1) / W, a.nviews);
+    return dim3((a.batches + W - 1) / W, a.order ? 1 : a.nviews);
 }
 
 hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
-    hipLaunchKernelGGL(build_kernel<false>, build_grid(a), dim3(kBuildThreads), 0, s, a);
+    if (a.order)
+        hipLaunchKernelGGL((build_kernel<false, true>), build_grid(a), dim3(kBuildThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL((build_kernel<false, false>), build_grid(a), dim3(kBuildThreads), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
-    hipLaunchKernelGGL(build_kernel<true>, build_grid(a), dim3(kBuildThreads), 0, s, a);
+    if (a.order)
+        hipLaunchKernelGGL((build_kernel<true, true>), build_grid(a), dim3(kBuildThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL((build_kernel<true, false>), build_grid(a), dim3(kBuildThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+// Compaction of the visible Gaussians (non-empty tile rect) of every view into
+// build records and (depth word, record index) pairs, in (view, index) order:
+// one wave per 64-Gaussian batch, position = scanned batch offset + rank in
+// the ballot.
+__global__ __launch_bounds__(256) void compact_depth_kernel(CompactArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int v = blockIdx.y;
+    if (b >= a.batches) return;
+    const size_t slot = (size_t)v * a.batches + b;
+    const uint64_t bits = a.vis_bits[slot];
+    if (!((bits >> lane) & 1ull)) return;
+    const int i = b * kBuildBatch + lane;
+    const size_t o = (size_t)v * a.count + i;
+    const uint32_t p = a.vis_offsets[slot] + (uint32_t)__popcll(bits & lanemask_lt(lane));
+    const float2 c = a.clip_xy[o];
+    const float4 e = a.ellipse[o];
+    const uint32_t dw = depth_word(a.clip_z[o]);
+    a.records[2 * (size_t)p] = make_float4(c.x, c.y, e.x, e.y);
+    a.records[2 * (size_t)p + 1] = make_float4(e.z, e.w, __uint_as_float(dw), __int_as_float(v * a.count + i));
+    a.depth_keys[p] = dw;
+    a.order[p] = (int32_t)p;
+}
+
+hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t s) {
+    if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
+    hipLaunchKernelGGL(compact_depth_kernel, dim3((a.batches + 3) / 4, a.nviews), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

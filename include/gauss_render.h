@@ -12,6 +12,7 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -148,5 +149,78 @@ public:
 private:
     gr_renderer* r_ = nullptr;
 };
+
+// app/camera_controller.h:12-40 / app/camera_controller.cpp: orbit about the
+// scene centre.  yaw/pitch/distance drive the pose: direction = normalize(
+// (cos yaw cos pitch, sin pitch, sin yaw cos pitch)), position = centre -
+// direction * distance; pitch is clamped to +-89 degrees; SetBounds centres on
+// the box and sets distance = 0.8 |max - min|.  SetOrbit is the headless
+// driver (an orbit path sets yaw per frame instead of mouse deltas).
+class CameraController {
+public:
+    explicit CameraController(Camera* camera);
+    void Update(float delta_time);
+    void OnMouseMove(float dx, float dy);
+    void OnScroll(float dy);
+    void SetBounds(const vec3& min_bounds, const vec3& max_bounds);
+    void SetOrbit(float yaw, float pitch, float distance);
+    float yaw() const { return yaw_; }
+    float pitch() const { return pitch_; }
+    float distance() const { return distance_; }
+    vec3 center() const { return center_; }
+
+private:
+    void UpdateCameraPose();
+    Camera* camera_;
+    float yaw_ = 0.0f;
+    float pitch_ = 0.0f;
+    float distance_ = 2.0f;
+    vec3 center_{};
+    vec3 bounds_min_{-5.0f, -5.0f, -5.0f};
+    vec3 bounds_max_{5.0f, 5.0f, 5.0f};
+    float sensitivity_ = 0.01f;
+    float scroll_speed_ = 0.1f;
+};
+
+// core/camera_controls.cpp:130-143 (CameraControls::setBounds): the default
+// viewpoint for a scene box -- centre + normalize(0, s/2, s/2) * dist with
+// s the largest box side and dist = sqrt(2) * s/2 / tan(fov_y/2), looking at
+// the centre.
+void FitCameraToBounds(Camera& camera, const vec3& min_bounds, const vec3& max_bounds);
+
+// Back buffer (RGBA8, row 0 at the bottom) to a .png or .ppm file, top row
+// first (gr_image_write).  Replaces the GL display, app/main.cpp:435-466.
+bool WriteImage(const std::string& path, const uint8_t* rgba8, int width, int height);
+
+// Per-frame RenderStats reporting (app/main.cpp:483-503 prints the averages;
+// app/perf_overlay.cpp:115-161 draws FPS and the stage times over the image):
+// one CSV row per frame, a JSON summary (mean, median and p99 of each stage
+// and of the wall time per frame, FPS), and the reference's printed averages.
+class StatsLog {
+public:
+    void AddFrame(const RenderStats& stats, double wall_ms);
+    size_t frames() const { return frames_.size(); }
+    bool WriteCsv(const std::string& path) const;
+    std::string SummaryJson(int num_gaussians) const;
+    void PrintAverages(std::FILE* out) const;
+
+private:
+    struct Frame {
+        RenderStats s;
+        double wall_ms;
+    };
+    std::vector<Frame> frames_;
+};
+
+// core/ply_loader.h:13 (LoadPly), with the parsing rules of
+// core/ply_parser.cpp (gr_ply_read): fills gaussians (activated values,
+// rotation (w, x, y, z) normalised), sh_coefficients ([N][3(d+1)^2], 3j + c),
+// sh_degree, num_gaussians and the bounds, then PreprocessSh().  Returns the
+// Gaussian count, or 0 after printing the reason (as the reference does); the
+// reason also stays in gr_last_error().
+int LoadPly(const std::string& path, GaussianModel& model);
+
+// savePly (core/ply_parser.cpp:143-219) of the model's Gaussians; false on error.
+bool SavePly(const std::string& path, const GaussianModel& model);
 
 }  // namespace gauss_render

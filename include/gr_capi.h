@@ -262,6 +262,38 @@ GR_API gr_status gr_generate_scene(const gr_scene_params* p, float* position, fl
 GR_API gr_status gr_pose_eye(uint64_t seed, int32_t index, int32_t orbit, int32_t frames, const float center[3],
                              float radius, float eye[3]);
 
+/* ---- PLY scenes: core/ply_parser.cpp (parseHeader :15-100, parsePly :102-131
+ * and :241-393, savePly :143-219) ------------------------------------------
+ * Binary little-endian PLY with float properties x y z rot_0..3 (rot_0 = w)
+ * scale_0..2 f_dc_0..2 opacity [f_rest_0 ..]; the f_rest count fixes the SH
+ * degree (0..3).  gr_ply_probe reads the header only: vertex count, degree
+ * and sh_count = 3 + f_rest count.  gr_ply_read fills the packed model
+ * arrays of parsePly: position float4[N] (x, y, z, sigmoid(opacity)),
+ * scale_and_rotation float4[N] (exp(scale), encodeVec4 bits of the
+ * normalised quaternion), color float4[N] (f_dc * SH_C0 + 0.5, opacity),
+ * spherical_harmonics [N][sh_count] with coefficient j, channel c at 3j + c
+ * (NULL allowed when the degree is 0), rotation_wxyz float4[N] the
+ * normalised quaternion as floats (optional), bounds of the positions
+ * (optional; the maximum starts at -FLT_MAX).  capacity is the number of
+ * vertices the arrays hold.  Errors: GR_ERR_NOT_FOUND (cannot open),
+ * GR_ERR_INVALID_ARGUMENT with the reference's message for a bad header,
+ * missing property or wrong f_rest count, GR_ERR_CAPACITY.  gr_ply_write
+ * writes savePly's 14-property file (rotation written in decodeVec4's
+ * x, y, z, w order under rot_0..rot_3, as the reference does). */
+GR_API gr_status gr_ply_probe(const char* path, int32_t* count, int32_t* sh_degree, int32_t* sh_count);
+GR_API gr_status gr_ply_read(const char* path, int32_t capacity, float* position, float* scale_and_rotation,
+                             float* color, float* spherical_harmonics, float* rotation_wxyz, float* bounds_min,
+                             float* bounds_max, int32_t* count);
+GR_API gr_status gr_ply_write(const char* path, int32_t count, const float* position,
+                              const float* scale_and_rotation, const float* color);
+
+/* ---- image output (replaces the GL display, app/main.cpp:435-466) -------
+ * RGBA8 back buffer to a .png (RGBA, uncompressed deflate) or .ppm (RGB)
+ * file chosen by the extension; bottom_up != 0 flips the rows so that row 0
+ * of the buffer (the bottom of the image, clip y = -1) is written last. */
+GR_API gr_status gr_image_write(const char* path, const uint8_t* rgba8, int32_t width, int32_t height,
+                                int32_t bottom_up);
+
 /* ---- object level: gauss_render::Renderer (core/renderer.h:13-50) ---- */
 /* device < 0 keeps the current device; stream NULL creates one. */
 GR_API gr_status gr_renderer_create(gr_renderer** out, int32_t device, void* stream);

@@ -145,3 +145,18 @@ def gpu_radix_sort(torch, keys: np.ndarray, vals: np.ndarray, begin_bit: int, en
     temp.free()
     ctx.close()
     return out_k, out_v
+
+
+def frame_list(renderer):
+    """Sorted tile-list keys, values and tile ranges of the renderer's last frame (host copies)."""
+    v = renderer.frame_view()
+    k = int(v.tile_list_size)
+    hip = C.CDLL("libamdhip64.so")
+    keys = np.empty(max(k, 1), np.uint64)
+    vals = np.empty(max(k, 1), np.int32)
+    rng = np.empty(2 * v.tiles_x * v.tiles_y, np.int32)
+    for dst, src, nbytes in ((keys, v.sorted_keys, 8 * k), (vals, v.sorted_values, 4 * k),
+                             (rng, v.tile_range, rng.nbytes)):
+        if nbytes:
+            assert hip.hipMemcpy(C.c_void_p(dst.ctypes.data), C.c_void_p(src), C.c_size_t(nbytes), 2) == 0
+    return keys[:k], vals[:k], rng

@@ -206,6 +206,35 @@ hipError_t launch_project(const ProjectArgs& a, hipStream_t s) {
         touch(a.clip_z, n * 4);
         touch(a.ellipse, n * 16);
         touch(a.conic, n * 16);
+        if (a.vis_bits) {   // every Gaussian visible
+            for (int v = 0; v < a.nviews; ++v)
+                for (int b = 0; b < a.batches; ++b) {
+                    const int here = std::min(64, a.count - 64 * b);
+                    a.vis_bits[(size_t)v * a.batches + b] = here == 64 ? ~0ull : ((1ull << here) - 1ull);
+                    a.vis_counts[(size_t)v * a.batches + b] = (uint32_t)here;
+                }
+        }
+    });
+    return hipSuccess;
+}
+
+hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t s) {
+    run_or_record(s, [=] {
+        for (int v = 0; v < a.nviews; ++v)
+            for (int b = 0; b < a.batches; ++b) {
+                const size_t slot = (size_t)v * a.batches + b;
+                uint32_t p = a.vis_offsets[slot];
+                for (int l = 0; l < 64; ++l)
+                    if ((a.vis_bits[slot] >> l) & 1ull) {
+                        rd(&a.clip_z[(size_t)v * a.count + 64 * b + l], 4);
+                        rd(&a.clip_xy[(size_t)v * a.count + 64 * b + l], 8);
+                        rd(&a.ellipse[(size_t)v * a.count + 64 * b + l], 16);
+                        touch(&a.records[2 * (size_t)p], 32);
+                        a.depth_keys[p] = 7u;
+                        a.order[p] = (int32_t)p;
+                        ++p;
+                    }
+            }
     });
     return hipSuccess;
 }
@@ -217,7 +246,13 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
         rd(a.clip_xy, n * 8);
         rd(a.clip_z, n * 4);
         rd(a.ellipse, n * 16);
-        for (int i = 0; i < a.nviews * a.batches; ++i) a.batch_counts[i] = 37;
+        const uint32_t nvis = a.order ? std::min<uint32_t>(*a.order_count, (uint32_t)n) : (uint32_t)n;
+        if (a.order) {
+            rd(a.order, (size_t)nvis * 4);
+            for (uint32_t q = 0; q < nvis; ++q) rd(&a.records[2 * (size_t)a.order[q]], 32);
+        }
+        const int nb = a.order ? a.batches : a.nviews * a.batches;
+        for (int i = 0; i < nb; ++i) a.batch_counts[i] = 37;
     });
     return hipSuccess;
 }
@@ -240,7 +275,7 @@ hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_
 
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     run_or_record(s, [=] {
-        const uint32_t K = a.batch_counts[a.nviews * a.batches];
+        const uint32_t K = a.batch_counts[a.order ? a.batches : a.nviews * a.batches];
         const uint32_t n = std::min<uint32_t>(K, (uint32_t)a.capacity);
         touch(a.keys, (size_t)n * 8);
         touch(a.values, (size_t)n * 4);
@@ -279,6 +314,10 @@ hipError_t launch_raster(const RasterArgs& a, hipStream_t s) {
     return hipSuccess;
 }
 
+template <typename KeyT>
+hipError_t mock_sort(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
+                     int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes, hipStream_t s);
+
 size_t radix_sort_temp_bytes(int32_t count, int32_t, int32_t end_bit) {
     return (size_t)count * 4 + 4096 + (size_t)end_bit;
 }
@@ -286,19 +325,38 @@ size_t radix_sort_temp_bytes(int32_t count, int32_t, int32_t end_bit) {
 hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                             const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                             size_t temp_bytes, hipStream_t s) {
+    return mock_sort(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
+}
+
+hipError_t radix_sort_pairs_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                                const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                                size_t temp_bytes, hipStream_t s) {
+    return mock_sort(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
+}
+
+// moves the data like an even (end-begin <= 32 bits: 4 passes) or odd number of passes would
+template <typename KeyT>
+hipError_t mock_sort(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
+                     int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes, hipStream_t s) {
+    if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
     if (temp_bytes < radix_sort_temp_bytes(count, begin_bit, end_bit)) return hipErrorInvalidValue;
-    uint64_t* k0 = keys[*sel];
-    uint64_t* k1 = keys[*sel ^ 1];
+    const int passes = (end_bit - begin_bit + 7) / 8;
+    KeyT* k0 = keys[*sel];
+    KeyT* k1 = keys[*sel ^ 1];
     int32_t* v0 = values[*sel];
     int32_t* v1 = values[*sel ^ 1];
     const size_t tb = radix_sort_temp_bytes(count, begin_bit, end_bit);
     run_or_record(s, [=] {
         const uint32_t n = count_dev ? std::min<uint32_t>(*count_dev, (uint32_t)count) : (uint32_t)count;
-        std::memcpy(k1, k0, (size_t)n * 8);
+        std::memcpy(k1, k0, (size_t)n * sizeof(KeyT));
         std::memcpy(v1, v0, (size_t)n * 4);
+        if (passes % 2 == 0) {
+            std::memcpy(k0, k1, (size_t)n * sizeof(KeyT));
+            std::memcpy(v0, v1, (size_t)n * 4);
+        }
         touch(temp, tb);
     });
-    *sel ^= 1;
+    if (passes % 2) *sel ^= 1;
     return hipSuccess;
 }
 
