@@ -74,13 +74,18 @@ struct ProjectArgs {
 
 // Depth-ordered tile lists (the renderer's path): the visible Gaussians of
 // every view are compacted in (view, index) order into build records and
-// (depth word, record index) pairs, the pairs are sorted stably by depth
-// word, and the build then walks the records in that order.  Entries of one
-// tile therefore come out ordered by (depth word, index) -- the order the
-// stable sort of (tile << 32 | depth) keys emitted in index order produces --
-// and only the tile bits remain to be sorted.
+// (depth word, record index) pairs.  The count pass walks the records in that
+// order and writes each record's exact entry count; the pairs are sorted
+// stably by depth word; a scan of the counts taken in depth order gives every
+// record its output offset; the emit pass walks the records in (view, index)
+// order again and writes each record's entries at its offset.  Entries of one
+// tile therefore sit in (depth word, index) order -- the order the stable
+// sort of (tile << 32 | depth) keys emitted in index order produces -- and
+// only the tile bits remain to be sorted.  Both build passes keep the
+// balanced index-order walk: depth order would put the largest (nearest)
+// splats into the same few waves.
 //
-// Build record (32 bytes, one cache-line gather per Gaussian in the build):
+// Build record (32 bytes):
 //   [0] centre x, centre y, cos, sin     [1] extent x, extent y,
 //   depth word (bits), view * count + index (bits)
 struct CompactArgs {
@@ -104,12 +109,13 @@ struct BuildArgs {
     uint32_t* batch_counts;      // [nviews][batches]: counts (count pass) / exclusive offsets (emit pass)
     uint64_t* keys;
     int32_t* values;             // Gaussian index within the scene
-    const int32_t* order;        // depth-ordered walk: record index per position (NULL: index order)
-    const float4* records;       // build records of the depth-ordered walk
-    const uint32_t* order_count; // number of valid positions in `order` (device)
+    const float4* records;       // record walk (NULL: the per-view index-order walk over clip_xy/ellipse)
+    const uint32_t* record_count;// number of valid records (device)
+    uint32_t* record_counts;     // count pass: exact entries per record
+    const uint32_t* record_offsets; // emit pass: output offset per record
     int32_t count;
     int32_t capacity;
-    int32_t batches;             // index order: ceil(count / 64) per view; depth order: ceil(nviews * count / 64)
+    int32_t batches;             // ceil(count / 64) per view; record walk: ceil(nviews * count / 64)
     int32_t nviews;
     ViewDesc v0;
     const ViewDesc* views;
@@ -152,6 +158,12 @@ hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_
                              hipStream_t stream);
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t stream);
 hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t stream);
+// tmp[j] = counts[order[j]] for j < *valid, 0 for valid <= j < n
+hipError_t launch_gather_counts(const uint32_t* counts, const int32_t* order, const uint32_t* valid, uint32_t* tmp,
+                                int32_t n, hipStream_t stream);
+// offsets[order[j]] = scanned[j] for j < *valid
+hipError_t launch_scatter_offsets(const uint32_t* scanned, const int32_t* order, const uint32_t* valid,
+                                  uint32_t* offsets, int32_t n, hipStream_t stream);
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t stream);
 hipError_t launch_ranges(const RangeArgs& a, hipStream_t stream);
 hipError_t launch_raster(const RasterArgs& a, hipStream_t stream);

@@ -6,16 +6,19 @@
 // the tile list, as the reference's buildTileList does at
 // core/gaussian_render.cu:581):
 //   SH (degree > 0) -> project (+ per-64 visibility ballot)
-//   -> depth order: scan of the ballots, compaction of the visible Gaussians
-//      into (depth word, index) pairs, 32-bit onesweep sort by depth word
-//   -> build count over the depth-ordered Gaussians, scan, read K -> grow the
-//      tile list (grow-only, 1.25x headroom) -> emit
+//   -> compaction of the visible Gaussians into build records and (depth
+//      word, record) pairs -> build count: exact entries per record
+//   -> 32-bit onesweep sort of the pairs by depth word -> the counts taken
+//      in depth order are scanned and scattered back as per-record output
+//      offsets; read K -> grow the tile list (grow-only, 1.25x headroom)
+//   -> emit: each record's entries at its offset
 //   -> onesweep sort of the tile bits only (bits [32, 32 + tile bits))
 //   -> ranges -> raster.
-// Entries are emitted in (depth word, Gaussian index) order, so the stable
-// tile-bit sort yields exactly the order of the reference's full-key sort of
-// (tile << 32 | depth word) keys emitted in index order, with 2 passes over
-// the K entries instead of 6 (4 passes run over the visible Gaussians only).
+// Entries sit in (depth word, Gaussian index) order before the last sort, so
+// the stable tile-bit sort yields exactly the order of the reference's
+// full-key sort of (tile << 32 | depth word) keys emitted in index order,
+// with 2 passes over the K entries instead of 6 (the 4 depth passes run over
+// the visible Gaussians only).
 // The reference truncates a full tile list and doubles the capacity for the
 // next frame (app/main.cpp:330-338,405); here a frame is never truncated.
 #include <hip/hip_runtime.h>
@@ -83,7 +86,7 @@ gr_status rfail(gr_status st, const std::string& msg) {
 struct gr_renderer {
     int device = 0;
     gr_context* ctx = nullptr;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[9] = {};
     int32_t n = 0, degree = 0, sh_count = 3, precision = GR_PRECISION_FP32;
     DevBuf<float4> position, scale_rot, color, conic, ellipse;
     DevBuf<uint2> packed;        // precision form: scale + opacity, 16-bit
@@ -96,7 +99,9 @@ struct gr_renderer {
     DevBuf<uchar4> out;
     DevBuf<uint64_t> vis_bits;      // per 64 Gaussians: non-empty tile rect ballot
     DevBuf<uint32_t> vis_counts;    // popcounts -> exclusive offsets, total, scan scratch
-    DevBuf<uint32_t> batch_counts;  // build counts over the depth order -> offsets, total, scratch
+    DevBuf<uint32_t> batch_counts;  // record counts in depth order -> offsets, total, scan scratch
+    DevBuf<uint32_t> rec_counts;    // entries per build record
+    DevBuf<uint32_t> rec_offsets;   // output offset per build record
     DevBuf<uint32_t> dkeys[2];      // depth words of the visible Gaussians
     DevBuf<int32_t> dorder[2];      // their build-record indices
     DevBuf<float4> records;         // build records of the visible Gaussians (2 x float4 each)
@@ -135,7 +140,7 @@ gr_status gr_renderer_destroy(gr_renderer* r) {
     r->values[0].release(); r->values[1].release(); r->out.release();
     r->vis_bits.release(); r->vis_counts.release(); r->batch_counts.release();
     r->dkeys[0].release(); r->dkeys[1].release(); r->dorder[0].release(); r->dorder[1].release();
-    r->records.release();
+    r->records.release(); r->rec_counts.release(); r->rec_offsets.release();
     if (r->sort_temp) (void)hipFree(r->sort_temp);
     for (auto& e : r->ev) if (e) (void)hipEventDestroy(e);
     gr_context_destroy(r->ctx);
@@ -239,7 +244,9 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     const size_t nn = (size_t)std::max(n, 1);
     R_CHECK(r->vis_bits.ensure((size_t)std::max(batches, 1)));
     R_CHECK(r->vis_counts.ensure((size_t)batches + 2 + scan_scratch_words(batches)));
-    R_CHECK(r->batch_counts.ensure((size_t)batches + 2 + scan_scratch_words(batches)));
+    R_CHECK(r->batch_counts.ensure(nn + 2 + scan_scratch_words((int32_t)nn)));
+    R_CHECK(r->rec_counts.ensure(nn));
+    R_CHECK(r->rec_offsets.ensure(nn));
     for (int k = 0; k < 2; ++k) {
         R_CHECK(r->dkeys[k].ensure(nn));
         R_CHECK(r->dorder[k].ensure(nn));
@@ -254,7 +261,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     vd.tiles = tiles;
     vd.out = outp;
     uint32_t* vis_total = r->vis_counts.p + batches;
-    uint32_t* k_total = r->batch_counts.p + batches;
+    uint32_t* k_total = r->batch_counts.p + n;
 
     R_HIP(hipEventRecord(r->ev[0], stream));
     if (r->degree > 0 && n > 0) {
@@ -289,9 +296,8 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         R_HIP(launch_project(a, stream));
     }
     R_HIP(hipEventRecord(r->ev[2], stream));
-    // depth order of the visible Gaussians
+    // visible Gaussians -> build records + (depth word, record) pairs
     R_HIP(launch_build_scan(r->vis_counts.p, batches, vis_total, vis_total + 1, stream));
-    int32_t dsel = 0;
     {
         CompactArgs a{};
         a.vis_bits = r->vis_bits.p;
@@ -306,29 +312,31 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         a.batches = batches;
         a.nviews = 1;
         R_HIP(launch_compact_depth(a, stream));
-        uint32_t* dk[2] = {r->dkeys[0].p, r->dkeys[1].p};
-        int32_t* dv[2] = {r->dorder[0].p, r->dorder[1].p};
-        R_HIP(radix_sort_pairs_u32(dk, dv, &dsel, n, vis_total, 0, 32, r->sort_temp, r->sort_temp_bytes, stream));
     }
-    R_HIP(hipEventRecord(r->ev[3], stream));
-    // tile list over the depth order
+    // exact entry count of every record (index order)
     BuildArgs b{};
-    b.clip_xy = r->clip_xy.p;
-    b.clip_z = r->clip_z.p;
-    b.ellipse = r->ellipse.p;
-    b.batch_counts = r->batch_counts.p;
-    b.keys = r->keys[0].p;
-    b.values = r->values[0].p;
-    b.order = r->dorder[dsel].p;
     b.records = r->records.p;
-    b.order_count = vis_total;
+    b.record_count = vis_total;
+    b.record_counts = r->rec_counts.p;
+    b.record_offsets = r->rec_offsets.p;
     b.count = n;
     b.capacity = (int32_t)std::min<size_t>(r->keys[0].cap, (size_t)0x7fffffff);
     b.batches = batches;
     b.nviews = 1;
     b.v0 = vd;
     R_HIP(launch_build_count(b, stream));
-    R_HIP(launch_build_scan(r->batch_counts.p, batches, k_total, k_total + 1, stream));
+    R_HIP(hipEventRecord(r->ev[3], stream));
+    // depth order -> record offsets (count scan in depth order)
+    int32_t dsel = 0;
+    {
+        uint32_t* dk[2] = {r->dkeys[0].p, r->dkeys[1].p};
+        int32_t* dv[2] = {r->dorder[0].p, r->dorder[1].p};
+        R_HIP(radix_sort_pairs_u32(dk, dv, &dsel, n, vis_total, 0, 32, r->sort_temp, r->sort_temp_bytes, stream));
+    }
+    R_HIP(hipEventRecord(r->ev[4], stream));
+    R_HIP(launch_gather_counts(r->rec_counts.p, r->dorder[dsel].p, vis_total, r->batch_counts.p, n, stream));
+    R_HIP(launch_build_scan(r->batch_counts.p, n, k_total, k_total + 1, stream));
+    R_HIP(launch_scatter_offsets(r->batch_counts.p, r->dorder[dsel].p, vis_total, r->rec_offsets.p, n, stream));
     R_HIP(hipMemcpyAsync(ctx->pinned, k_total, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     R_HIP(hipStreamSynchronize(stream));
     const int64_t need = (int64_t)ctx->pinned[0];
@@ -340,13 +348,13 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
             R_CHECK(r->keys[k].ensure(want, 1.25));
             R_CHECK(r->values[k].ensure(want, 1.25));
         }
-        b.keys = r->keys[0].p;
-        b.values = r->values[0].p;
         b.capacity = (int32_t)std::min<size_t>(r->keys[0].cap, (size_t)0x7fffffff);
     }
+    b.keys = r->keys[0].p;
+    b.values = r->values[0].p;
     const int32_t size = (int32_t)need;
     if (size > 0) R_HIP(launch_build_emit(b, stream));
-    R_HIP(hipEventRecord(r->ev[4], stream));
+    R_HIP(hipEventRecord(r->ev[5], stream));
     // tile bits
     int32_t sel = 0;
     if (size > 1) {
@@ -356,7 +364,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         R_HIP(radix_sort_pairs(kb, vb, &sel, size, nullptr, 32, 32 + tile_bits_for(tiles), r->sort_temp,
                                r->sort_temp_bytes, stream));
     }
-    R_HIP(hipEventRecord(r->ev[5], stream));
+    R_HIP(hipEventRecord(r->ev[6], stream));
     r->sel = sel;
     R_HIP(launch_fill_i32(r->tile_range.p, -1, 2 * (int64_t)tiles, stream));
     if (size > 0) {
@@ -382,24 +390,26 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         a.v0 = vd;
         R_HIP(launch_raster(a, stream));
     }
-    R_HIP(hipEventRecord(r->ev[6], stream));
+    R_HIP(hipEventRecord(r->ev[7], stream));
     if (!out_is_device)
         R_HIP(hipMemcpyAsync(out_rgba8, outp, (size_t)width * height * 4, hipMemcpyDeviceToHost, stream));
-    R_HIP(hipEventRecord(r->ev[7], stream));
-    R_HIP(hipEventSynchronize(r->ev[7]));
+    R_HIP(hipEventRecord(r->ev[8], stream));
+    R_HIP(hipEventSynchronize(r->ev[8]));
     r->last_k = need;
     r->last_w = width;
     r->last_h = height;
     if (stats) {
-        float t[7];
-        for (int k = 0; k < 6; ++k) R_HIP(hipEventElapsedTime(&t[k], r->ev[k], r->ev[k + 1]));
-        R_HIP(hipEventElapsedTime(&t[6], r->ev[0], r->ev[6]));
+        // events: 0 SH 1 project 2 compaction + count 3 depth sort 4 offsets, K readback, emit
+        // 5 tile-bit sort 6 ranges + raster 7
+        float t[8];
+        for (int k = 0; k < 7; ++k) R_HIP(hipEventElapsedTime(&t[k], r->ev[k], r->ev[k + 1]));
+        R_HIP(hipEventElapsedTime(&t[7], r->ev[0], r->ev[7]));
         stats->sh_eval_ms = t[0];
         stats->project_ms = t[1];
-        stats->sort_ms = t[2] + t[4];       // depth order + tile-bit sort
-        stats->build_tile_ms = t[3];        // count, scan, K readback, emit
-        stats->raster_ms = t[5];            // ranges + raster
-        stats->total_ms = t[6];
+        stats->build_tile_ms = t[2] + t[4];   // compaction, count, offsets, K readback, emit
+        stats->sort_ms = t[3] + t[5];         // depth sort + tile-bit sort
+        stats->raster_ms = t[6];              // ranges + raster
+        stats->total_ms = t[7];
         stats->tile_list_size = need;
     }
     return GR_OK;

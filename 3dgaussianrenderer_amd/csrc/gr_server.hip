@@ -7,11 +7,13 @@
 // a grow-only workspace; a batch (up to max_views requests of one scene, any
 // mix of resolutions) runs on one slot in two phases:
 //   A  camera block H2D -> SH (all views) -> projection (all views, with the
-//      visibility ballot) -> compaction of every view's visible Gaussians and
-//      one 32-bit sort of them all by depth word -> build count over that
-//      depth order -> scan -> 4-byte K readback
-//   B  emit (depth order) -> onesweep sort of the tile bits (device-side K)
-//      -> ranges -> raster (all views) -> images D2H into pinned staging
+//      visibility ballot) -> compaction of every view's visible Gaussians
+//      into build records -> build count per record -> one 32-bit sort of
+//      all views' records by depth word -> record offsets from the counts in
+//      depth order -> 4-byte K readback
+//   B  emit (each record at its offset) -> onesweep sort of the tile bits
+//      (device-side K) -> ranges -> raster (all views) -> images D2H into
+//      pinned staging
 // Tile ids are global over the batch's views, so one sort serves all views,
 // and within a tile entries keep (depth word, index) order.
 // Between A and B the host reads K and grows the slot's tile-list buffers
@@ -142,7 +144,8 @@ struct Workspace {
     Dev dkeys0, dkeys1, dorder0, dorder1;   // [<= V*N] depth words and record indices of visible Gaussians
     Dev records;                      // [<= V*N][2] float4 build records
     Dev dsort_temp;                   // depth sort scratch
-    Dev counts;                       // build counts over the depth order (ceil(V*N/64)) + total + scratch
+    Dev counts;                       // record counts in depth order -> offsets [V*N] + total + scratch
+    Dev rec_counts, rec_offsets;      // [V*N] entries and output offset per record
     Dev keys0, keys1, vals0, vals1;   // tile lists
     Dev sort_temp;
     Dev ranges;                       // 2 * tiles
@@ -152,7 +155,7 @@ struct Workspace {
     uint64_t generation = 0;          // bumps when any device pointer changes
     std::vector<Dev*> devs() {
         return {&color, &xy, &z, &ell, &con, &vis_bits, &vis_counts, &dkeys0, &dkeys1, &dorder0, &dorder1,
-                &records, &dsort_temp, &counts, &keys0, &keys1, &vals0, &vals1, &sort_temp, &ranges, &images, &views};
+                &records, &rec_counts, &rec_offsets, &dsort_temp, &counts, &keys0, &keys1, &vals0, &vals1, &sort_temp, &ranges, &images, &views};
     }
 };
 
@@ -259,7 +262,6 @@ std::vector<PhaseANeed> phase_a_needs(Slot& sl, const Scene& sc, size_t nviews, 
     Workspace& w = sl.ws;
     const int64_t V = (int64_t)nviews, N = sc.n;
     const int64_t batches = (N + kBuildBatch - 1) / kBuildBatch;
-    const int64_t ob = (V * N + kBuildBatch - 1) / kBuildBatch;
     return {{&w.color, V * N * 16},
             {&w.xy, V * N * 8},
             {&w.z, V * N * 4},
@@ -273,7 +275,9 @@ std::vector<PhaseANeed> phase_a_needs(Slot& sl, const Scene& sc, size_t nviews, 
             {&w.dorder1, V * N * 4},
             {&w.records, V * N * 32},
             {&w.dsort_temp, (int64_t)radix_sort_temp_bytes((int32_t)std::max<int64_t>(V * N, 1), 0, 32)},
-            {&w.counts, 4 * (ob + 2 + scan_scratch_words((int32_t)ob))},
+            {&w.counts, 4 * (V * N + 2 + scan_scratch_words((int32_t)(V * N)))},
+            {&w.rec_counts, V * N * 4},
+            {&w.rec_offsets, V * N * 4},
             {&w.images, img_bytes},
             {&w.ranges, 8 * (int64_t)tiles_total},
             {&w.views, V * (int64_t)sizeof(ViewDesc)}};
@@ -328,22 +332,19 @@ gr_status ensure_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     return GR_OK;
 }
 
-// Build over the depth order of the batch (count in phase A, emit in phase B).
-BuildArgs ordered_build_args(Slot& sl, const Scene& sc) {
+// Record walk of the batch (count in phase A, emit in phase B).
+BuildArgs record_build_args(Slot& sl, const Scene& sc) {
     Workspace& w = sl.ws;
     const int V = (int)sl.batch.size();
     const int N = sc.n;
     const int nbv = V * ((N + kBuildBatch - 1) / kBuildBatch);
     BuildArgs b{};
-    b.clip_xy = w.xy.as<float2>();
-    b.clip_z = w.z.as<float>();
-    b.ellipse = w.ell.as<float4>();
-    b.batch_counts = w.counts.as<uint32_t>();
     b.keys = w.keys0.as<uint64_t>();
     b.values = w.vals0.as<int32_t>();
-    b.order = w.dorder0.as<int32_t>();   // 4 passes of 8 bits: the sorted order is back in buffer 0
     b.records = w.records.as<float4>();
-    b.order_count = w.vis_counts.as<uint32_t>() + nbv;
+    b.record_count = w.vis_counts.as<uint32_t>() + nbv;
+    b.record_counts = w.rec_counts.as<uint32_t>();
+    b.record_offsets = w.rec_offsets.as<uint32_t>();
     b.count = N;
     b.capacity = (int32_t)sl.capacity;
     b.batches = (int32_t)(((int64_t)V * N + kBuildBatch - 1) / kBuildBatch);
@@ -413,18 +414,19 @@ gr_status enqueue_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     c.batches = batches;
     c.nviews = V;
     S_HIP(launch_compact_depth(c, sl.stream));
+    BuildArgs b = record_build_args(sl, sc);
+    S_HIP(launch_build_count(b, sl.stream));
     uint32_t* dk[2] = {w.dkeys0.as<uint32_t>(), w.dkeys1.as<uint32_t>()};
     int32_t* dv[2] = {w.dorder0.as<int32_t>(), w.dorder1.as<int32_t>()};
     int32_t dsel = 0;
     S_HIP(radix_sort_pairs_u32(dk, dv, &dsel, V * N, vis + nbv, 0, 32, w.dsort_temp.p, w.dsort_temp.bytes,
                                sl.stream));
-    if (dsel != 0) return sfail(GR_ERR_HIP, "depth sort left its result in the second buffer");
-    BuildArgs b = ordered_build_args(sl, sc);
-    S_HIP(launch_build_count(b, sl.stream));
     uint32_t* counts = w.counts.as<uint32_t>();
-    const int ob = b.batches;
-    S_HIP(launch_build_scan(counts, ob, counts + ob, counts + ob + 2, sl.stream));
-    S_HIP(hipMemcpyAsync(w.k_host.p, counts + ob, 4, hipMemcpyDeviceToHost, sl.stream));
+    const int vn = V * N;
+    S_HIP(launch_gather_counts(w.rec_counts.as<uint32_t>(), dv[dsel], vis + nbv, counts, vn, sl.stream));
+    S_HIP(launch_build_scan(counts, vn, counts + vn, counts + vn + 2, sl.stream));
+    S_HIP(launch_scatter_offsets(counts, dv[dsel], vis + nbv, w.rec_offsets.as<uint32_t>(), vn, sl.stream));
+    S_HIP(hipMemcpyAsync(w.k_host.p, counts + vn, 4, hipMemcpyDeviceToHost, sl.stream));
     return GR_OK;
 }
 
@@ -435,8 +437,8 @@ gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
     const ViewDesc* vhost = w.views_host.as<ViewDesc>();
     ViewDesc* vdev = w.views.as<ViewDesc>();
     uint32_t* counts = w.counts.as<uint32_t>();
-    BuildArgs b = ordered_build_args(sl, sc);
-    const int nb = b.batches;
+    BuildArgs b = record_build_args(sl, sc);
+    const int nb = V * N;   // the device-side K sits after the V*N scanned counts
     S_HIP(launch_build_emit(b, sl.stream));
     uint64_t* kb[2] = {w.keys0.as<uint64_t>(), w.keys1.as<uint64_t>()};
     int32_t* vb[2] = {w.vals0.as<int32_t>(), w.vals1.as<int32_t>()};

@@ -314,10 +314,11 @@ hipError_t launch_project(const ProjectArgs& a, hipStream_t s) {
 // convertToEllipseCoordinates :234-239 and intersectsUnitCircle :241-248.
 // Key: getKey :306-310.  The reference lets atomics decide entry order
 // (:485, :498); here entries follow Gaussian index, then row-major candidate
-// order, so the stable sort that follows is deterministic.  In the
-// depth-ordered walk (ORDERED) positions follow the depth-sorted Gaussian
-// list instead, so entries come out in (depth word, index) order and a wave's
-// 64 Gaussians may belong to different views.
+// order, so the stable sort that follows is deterministic.  The record walk
+// (RECORDS) runs over the compacted visible Gaussians of all views (a wave's
+// 64 records may belong to different views); its count pass writes each
+// record's entry count and its emit pass writes each record's entries at an
+// offset assigned in depth order.
 // ---------------------------------------------------------------------------
 // IEEE quotient a / b from rb = RN(1/b): q0 = RN(a*rb), e = a - b*q0 (exact
 // through the FMA), q = RN(q0 + e*rb) is the correctly rounded quotient when
@@ -394,40 +395,36 @@ __device__ __forceinline__ void divmod_small(int t, int n, int& q, int& r) {
     else if (r >= n) { ++q; r -= n; }
 }
 
-template <bool EMIT, bool ORDERED>
+template <bool EMIT, bool RECORDS>
 __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
     constexpr int W = kBuildThreads / 64;
     __shared__ int s_excl[W][64];
     __shared__ int4 s_rect[W][64];          // x0, y0, nx, depth word
     __shared__ EllipseRec s_ell[W][64];
-    __shared__ int4 s_tile[ORDERED ? W : 1][64];   // ORDERED: tile_base, tiles_x, tile clip x/y bits
-    __shared__ int s_gidx[ORDERED ? W : 1][64];    // ORDERED: Gaussian index within the scene
+    __shared__ int4 s_tile[RECORDS ? W : 1][64];   // RECORDS: tile_base, tiles_x, tile clip x/y bits
+    __shared__ int s_gidx[RECORDS ? W : 1][64];    // RECORDS: Gaussian index within the scene
+    __shared__ uint32_t s_start[RECORDS ? W : 1][64];  // RECORDS: entries of the wave before each owner
+    __shared__ uint32_t s_off[(RECORDS && EMIT) ? W : 1][64];  // RECORDS emit: output offset of each owner
 
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int batch = blockIdx.x * W + w;
     const int first = batch * kBuildBatch;
     int limit = a.count;
-    if constexpr (ORDERED) {
-        limit = (int)min(*a.order_count, (uint32_t)a.nviews * (uint32_t)a.count);
-        if (first >= limit) {   // wave-uniform; no block barrier below
-            if (!EMIT && lane == 0 && batch < a.batches) a.batch_counts[batch] = 0u;
-            return;
-        }
-    } else {
-        if (first >= a.count) return;
+    if constexpr (RECORDS) {
+        limit = (int)min(*a.record_count, (uint32_t)a.nviews * (uint32_t)a.count);
     }
+    if (first >= limit) return;   // wave-uniform; no block barrier below
     const int pos = first + lane;
     int view, gi = -1;
     float2 c = make_float2(0.0f, 0.0f);
     float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t dw = 0u;
-    if constexpr (ORDERED) {
+    if constexpr (RECORDS) {
         view = 0;
         if (pos < limit) {
-            const int rp = a.order[pos];
-            const float4 r0 = a.records[2 * (size_t)rp];
-            const float4 r1 = a.records[2 * (size_t)rp + 1];
+            const float4 r0 = a.records[2 * (size_t)pos];
+            const float4 r1 = a.records[2 * (size_t)pos + 1];
             const int pg = __float_as_int(r1.w);
             view = pg / a.count;
             gi = pg - view * a.count;
@@ -457,12 +454,16 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
         if (cnt > 0) {
             s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, (int)dw);
             s_ell[w][lane] = EllipseRec{c.x, c.y, e.x, e.y, e.z, e.w, 1.0f / e.z, 1.0f / e.w};
-            if constexpr (ORDERED) {
+            if constexpr (RECORDS) {
                 s_tile[w][lane] = make_int4(vd.tile_base, geom.tiles_x, __float_as_int(geom.tile_clip_x),
                                             __float_as_int(geom.tile_clip_y));
                 s_gidx[w][lane] = gi;
             }
         }
+    }
+    if constexpr (RECORDS) {
+        s_start[w][lane] = 0u;
+        if constexpr (EMIT) s_off[w][lane] = pos < limit ? a.record_offsets[pos] : 0u;
     }
     int incl = cnt;
 #pragma unroll
@@ -475,24 +476,24 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
     wave_sync();
 
     uint32_t running = 0;
-    const size_t bslot = ORDERED ? (size_t)batch : (size_t)view * a.batches + batch;
-    const uint32_t out_base = EMIT ? a.batch_counts[bslot] : 0u;
+    const size_t bslot = (size_t)view * a.batches + batch;
+    const uint32_t out_base = (EMIT && !RECORDS) ? a.batch_counts[bslot] : 0u;
     for (int base = 0; base < total; base += 64) {
-        const int c = base + lane;
+        const int cc = base + lane;
         bool pass = false;
         int o = 0, gx = 0, gy = 0;
         int4 rc = make_int4(0, 0, 1, 0);
         uint32_t tile = 0;
-        if (c < total) {
+        if (cc < total) {
 #pragma unroll
             for (int step = 32; step >= 1; step >>= 1)
-                if (s_excl[w][o + step] <= c) o += step;
+                if (s_excl[w][o + step] <= cc) o += step;
             rc = s_rect[w][o];
             int q, r;
-            divmod_small(c - s_excl[w][o], rc.z, q, r);
+            divmod_small(cc - s_excl[w][o], rc.z, q, r);
             gx = rc.x + r;
             gy = rc.y + q;
-            if constexpr (ORDERED) {
+            if constexpr (RECORDS) {
                 const int4 tv = s_tile[w][o];
                 FrameGeom og;
                 og.tile_clip_x = __int_as_float(tv.z);
@@ -505,27 +506,49 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
             }
         }
         const uint64_t m = __ballot(pass);
-        if (EMIT && pass) {
-            const uint32_t p = out_base + running + (uint32_t)__popcll(m & lanemask_lt(lane));
+        const uint32_t before = running + (uint32_t)__popcll(m & lanemask_lt(lane));
+        if constexpr (RECORDS) {
+            // the lane of an owner's first candidate records the wave's entry count there
+            if (cc < total && cc == s_excl[w][o]) s_start[w][o] = before;
+            wave_sync();
+            if (EMIT && pass) {
+                const uint32_t p = s_off[w][o] + (before - s_start[w][o]);
+                if (p < (uint32_t)a.capacity) {
+                    a.keys[p] = ((uint64_t)tile << 32) | (uint32_t)rc.w;
+                    a.values[p] = s_gidx[w][o];
+                }
+            }
+        } else if (EMIT && pass) {
+            const uint32_t p = out_base + before;
             if (p < (uint32_t)a.capacity) {
                 a.keys[p] = ((uint64_t)tile << 32) | (uint32_t)rc.w;
-                if constexpr (ORDERED) a.values[p] = s_gidx[w][o];
-                else a.values[p] = first + o;
+                a.values[p] = first + o;
             }
         }
         running += (uint32_t)__popcll(m);
     }
-    if (!EMIT && lane == 0) a.batch_counts[bslot] = running;
+    if constexpr (RECORDS) {
+        if (!EMIT && pos < limit) {
+            // owners are consecutive and every record has at least one candidate, so an owner's
+            // entries are the difference of consecutive start counts
+            wave_sync();
+            const int last = min(63, limit - 1 - first);
+            const uint32_t next = lane < last ? s_start[w][lane + 1] : running;
+            a.record_counts[pos] = next - s_start[w][lane];
+        }
+    } else {
+        if (!EMIT && lane == 0) a.batch_counts[bslot] = running;
+    }
 }
 
 static dim3 build_grid(const BuildArgs& a) {
     constexpr int W = kBuildThreads / 64;
-    return dim3((a.batches + W - 1) / W, a.order ? 1 : a.nviews);
+    return dim3((a.batches + W - 1) / W, a.records ? 1 : a.nviews);
 }
 
 hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
-    if (a.order)
+    if (a.records)
         hipLaunchKernelGGL((build_kernel<false, true>), build_grid(a), dim3(kBuildThreads), 0, s, a);
     else
         hipLaunchKernelGGL((build_kernel<false, false>), build_grid(a), dim3(kBuildThreads), 0, s, a);
@@ -534,7 +557,7 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
 
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
-    if (a.order)
+    if (a.records)
         hipLaunchKernelGGL((build_kernel<true, true>), build_grid(a), dim3(kBuildThreads), 0, s, a);
     else
         hipLaunchKernelGGL((build_kernel<true, false>), build_grid(a), dim3(kBuildThreads), 0, s, a);
@@ -563,6 +586,39 @@ __global__ __launch_bounds__(256) void compact_depth_kernel(CompactArgs a) {
     a.records[2 * (size_t)p + 1] = make_float4(e.z, e.w, __uint_as_float(dw), __int_as_float(v * a.count + i));
     a.depth_keys[p] = dw;
     a.order[p] = (int32_t)p;
+}
+
+__global__ void gather_counts_kernel(const uint32_t* counts, const int32_t* order, const uint32_t* valid,
+                                     uint32_t* tmp, int32_t n) {
+    const uint32_t nv = min(*valid, (uint32_t)n);
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+        tmp[j] = (uint32_t)j < nv ? counts[order[j]] : 0u;
+}
+
+__global__ void scatter_offsets_kernel(const uint32_t* scanned, const int32_t* order, const uint32_t* valid,
+                                       uint32_t* offsets, int32_t n) {
+    const uint32_t nv = min(*valid, (uint32_t)n);
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nv; j += (int64_t)gridDim.x * blockDim.x)
+        offsets[order[j]] = scanned[j];
+}
+
+static unsigned grid_for(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
+}
+
+hipError_t launch_gather_counts(const uint32_t* counts, const int32_t* order, const uint32_t* valid, uint32_t* tmp,
+                                int32_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_counts_kernel, dim3(grid_for(n)), dim3(256), 0, s, counts, order, valid, tmp, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_offsets(const uint32_t* scanned, const int32_t* order, const uint32_t* valid,
+                                  uint32_t* offsets, int32_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(scatter_offsets_kernel, dim3(grid_for(n)), dim3(256), 0, s, scanned, order, valid, offsets, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t s) {

@@ -239,20 +239,38 @@ hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t s) {
     return hipSuccess;
 }
 
-// 37 entries per build batch of every view
+// index-order walk: 37 entries per build batch of every view; record walk: 3 entries per record
 hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
     run_or_record(s, [=] {
+        if (a.records) {
+            const uint32_t nvis = std::min<uint32_t>(*a.record_count, (uint32_t)(a.nviews * a.count));
+            rd(a.records, (size_t)nvis * 32);
+            for (uint32_t q = 0; q < nvis; ++q) a.record_counts[q] = 3u;
+            return;
+        }
         const size_t n = (size_t)a.nviews * a.count;
         rd(a.clip_xy, n * 8);
         rd(a.clip_z, n * 4);
         rd(a.ellipse, n * 16);
-        const uint32_t nvis = a.order ? std::min<uint32_t>(*a.order_count, (uint32_t)n) : (uint32_t)n;
-        if (a.order) {
-            rd(a.order, (size_t)nvis * 4);
-            for (uint32_t q = 0; q < nvis; ++q) rd(&a.records[2 * (size_t)a.order[q]], 32);
-        }
-        const int nb = a.order ? a.batches : a.nviews * a.batches;
-        for (int i = 0; i < nb; ++i) a.batch_counts[i] = 37;
+        for (int i = 0; i < a.nviews * a.batches; ++i) a.batch_counts[i] = 37;
+    });
+    return hipSuccess;
+}
+
+hipError_t launch_gather_counts(const uint32_t* counts, const int32_t* order, const uint32_t* valid, uint32_t* tmp,
+                                int32_t n, hipStream_t s) {
+    run_or_record(s, [=] {
+        const uint32_t nv = std::min<uint32_t>(*valid, (uint32_t)n);
+        for (int32_t j = 0; j < n; ++j) tmp[j] = (uint32_t)j < nv ? counts[order[j]] : 0u;
+    });
+    return hipSuccess;
+}
+
+hipError_t launch_scatter_offsets(const uint32_t* scanned, const int32_t* order, const uint32_t* valid,
+                                  uint32_t* offsets, int32_t n, hipStream_t s) {
+    run_or_record(s, [=] {
+        const uint32_t nv = std::min<uint32_t>(*valid, (uint32_t)n);
+        for (uint32_t j = 0; j < nv; ++j) offsets[order[j]] = scanned[j];
     });
     return hipSuccess;
 }
@@ -275,7 +293,17 @@ hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_
 
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     run_or_record(s, [=] {
-        const uint32_t K = a.batch_counts[a.order ? a.batches : a.nviews * a.batches];
+        uint32_t K;
+        if (a.records) {
+            const uint32_t nvis = std::min<uint32_t>(*a.record_count, (uint32_t)(a.nviews * a.count));
+            K = 0;
+            for (uint32_t q = 0; q < nvis; ++q) {
+                if (a.record_offsets[q] > 3u * nvis) std::abort();   // offsets lie inside the list
+                K += a.record_counts[q];
+            }
+        } else {
+            K = a.batch_counts[a.nviews * a.batches];
+        }
         const uint32_t n = std::min<uint32_t>(K, (uint32_t)a.capacity);
         touch(a.keys, (size_t)n * 8);
         touch(a.values, (size_t)n * 4);
