@@ -86,6 +86,11 @@ class FrameView(C.Structure):
                 ("tiles_x", C.c_int32), ("tiles_y", C.c_int32), ("splat_count", C.c_int32)]
 
 
+class KernelTimes(C.Structure):
+    _fields_ = [("tile_sort_pass_ms", C.c_double), ("tile_sort_passes", C.c_int32), ("pad", C.c_int32),
+                ("tile_list_size", C.c_int64)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("max_views_per_batch", C.c_int32),
                 ("use_graphs", C.c_int32), ("hbm_budget_bytes", C.c_int64)]
@@ -158,6 +163,7 @@ SIGNATURES = [
     ("gr_renderer_render", C.c_int, [_P, C.POINTER(CameraData), C.c_int32, C.c_int32, _P, C.c_int32,
                                      C.POINTER(RenderStats)]),
     ("gr_renderer_frame_view", C.c_int, [_P, C.POINTER(FrameView)]),
+    ("gr_renderer_kernel_times", C.c_int, [_P, C.POINTER(KernelTimes)]),
     ("gr_server_create", C.c_int, [_PP, C.POINTER(ServerConfig)]),
     ("gr_server_destroy", C.c_int, [_P]),
     ("gr_server_add_scene", C.c_int, [_P, C.POINTER(ModelHost), C.POINTER(C.c_int32)]),
@@ -415,6 +421,13 @@ class Renderer:
             return img, st.as_dict()
         check(lib().gr_renderer_render(self.h, C.byref(cam), width, height, C.c_void_p(int(out)), 1, C.byref(st)))
         return None, st.as_dict()
+
+    def kernel_times(self) -> dict:
+        """Tile-bit sort pass timing of the last frame (HIP events on the renderer's stream)."""
+        t = KernelTimes()
+        check(lib().gr_renderer_kernel_times(self.h, C.byref(t)))
+        return {"tile_sort_pass_ms": t.tile_sort_pass_ms, "tile_sort_passes": t.tile_sort_passes,
+                "tile_list_size": t.tile_list_size}
 
     def frame_view(self) -> FrameView:
         v = FrameView()

@@ -112,7 +112,10 @@ struct BuildArgs {
     const float4* records;       // record walk (NULL: the per-view index-order walk over clip_xy/ellipse)
     const uint32_t* record_count;// number of valid records (device)
     uint32_t* record_counts;     // count pass: exact entries per record
+    uint64_t* record_masks;      // count pass: passing candidates 0..63 per record
     const uint32_t* record_offsets; // emit pass: output offset per record
+    uint32_t* tile_hist;         // record emit: 256-bin histograms of the tile's 8-bit digits (NULL: none)
+    int32_t tile_passes;         // digits counted into tile_hist (<= 3)
     int32_t count;
     int32_t capacity;
     int32_t batches;             // ceil(count / 64) per view; record walk: ceil(nviews * count / 64)
@@ -147,6 +150,8 @@ struct RasterArgs {
 // consecutive Gaussians.
 constexpr int kBuildBatch = 64;
 constexpr int kBuildThreads = 256;
+// Workgroups of the record emit pass (its waves take batches grid-stride).
+constexpr int kEmitBlocks = 2048;
 
 // launchers (all asynchronous on `stream`)
 hipError_t launch_sh(const ShArgs& a, int degree, int precision, hipStream_t stream);
@@ -156,6 +161,11 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t stream);
 int scan_scratch_words(int32_t n);
 hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_t* scratch,
                              hipStream_t stream);
+// The same scan, but the exclusive prefix of element j goes to offsets[order[j]] (for j < *valid)
+// instead of back to data[j].
+hipError_t launch_build_scan_scatter(uint32_t* data, int32_t n, uint32_t* total, uint32_t* scratch,
+                                     const int32_t* order, const uint32_t* valid, uint32_t* offsets,
+                                     hipStream_t stream);
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t stream);
 hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t stream);
 // tmp[j] = counts[order[j]] for j < *valid, 0 for valid <= j < n
@@ -178,6 +188,19 @@ size_t radix_sort_temp_bytes(int32_t count, int32_t begin_bit, int32_t end_bit);
 hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                             const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                             size_t temp_bytes, hipStream_t stream);
+// Two-step form for a producer that accumulates the digit histograms itself:
+// radix_sort_prepare zeroes the histograms (passes x 256 uint32 at *hist, pass
+// p counting digit (key >> (begin_bit + 8p)) & 255), the partition tickets and
+// the first look-back region of `temp` sized for `count`; after the producer
+// has added exactly the pairs the sort will see, radix_sort_pairs_prepared
+// scans them and runs the passes.  A device-side count above `count` then
+// means the producer did not write the list: the prepared sort leaves the
+// buffers untouched.
+hipError_t radix_sort_prepare(int32_t count, void* temp, size_t temp_bytes, uint32_t** hist, hipStream_t stream);
+hipError_t radix_sort_pairs_prepared(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                                     const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                                     size_t temp_bytes, hipStream_t stream, hipEvent_t passes_begin = nullptr,
+                                     hipEvent_t passes_end = nullptr);
 // The same for 32-bit keys (depth words); same temp layout and size.
 hipError_t radix_sort_pairs_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                                 const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,

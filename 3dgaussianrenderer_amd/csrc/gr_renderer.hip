@@ -2,25 +2,27 @@
 // state of gauss_render::Renderer (core/renderer.h:23-49) and the frame of
 // core/renderer.cpp:94-139 / app/main.cpp:326-433.
 //
-// Frame (one HIP stream, no host sync except the 4-byte K readback that sizes
-// the tile list, as the reference's buildTileList does at
-// core/gaussian_render.cu:581):
+// Frame (one HIP stream; the only host sync is at the end, after the image):
 //   SH (degree > 0) -> project (+ per-64 visibility ballot)
 //   -> compaction of the visible Gaussians into build records and (depth
-//      word, record) pairs -> build count: exact entries per record
+//      word, record) pairs -> build count: exact entries and the mask of
+//      passing candidates per record
 //   -> 32-bit onesweep sort of the pairs by depth word -> the counts taken
-//      in depth order are scanned and scattered back as per-record output
-//      offsets; read K -> grow the tile list (grow-only, 1.25x headroom)
-//   -> emit: each record's entries at its offset
+//      in depth order are scanned (K) and scattered back as per-record
+//      output offsets
+//   -> emit: each record's entries at its offset (+ tile digit histograms)
 //   -> onesweep sort of the tile bits only (bits [32, 32 + tile bits))
-//   -> ranges -> raster.
+//   -> ranges -> raster -> image and K readback.
 // Entries sit in (depth word, Gaussian index) order before the last sort, so
 // the stable tile-bit sort yields exactly the order of the reference's
 // full-key sort of (tile << 32 | depth word) keys emitted in index order,
 // with 2 passes over the K entries instead of 6 (the 4 depth passes run over
 // the visible Gaussians only).
 // The reference truncates a full tile list and doubles the capacity for the
-// next frame (app/main.cpp:330-338,405); here a frame is never truncated.
+// next frame (app/main.cpp:330-338,405); here a frame is never truncated:
+// when K exceeds the list, the emit writes only what fits, the sort, ranges
+// and raster see K > capacity and do nothing, and the frame is rendered
+// again with a list grown to 1.25 K.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -87,6 +89,8 @@ struct gr_renderer {
     int device = 0;
     gr_context* ctx = nullptr;
     hipEvent_t ev[9] = {};
+    hipEvent_t pass_ev[2] = {};     // around the tile-bit sort's onesweep passes
+    int32_t last_passes = 0;
     int32_t n = 0, degree = 0, sh_count = 3, precision = GR_PRECISION_FP32;
     DevBuf<float4> position, scale_rot, color, conic, ellipse;
     DevBuf<uint2> packed;        // precision form: scale + opacity, 16-bit
@@ -102,6 +106,7 @@ struct gr_renderer {
     DevBuf<uint32_t> batch_counts;  // record counts in depth order -> offsets, total, scan scratch
     DevBuf<uint32_t> rec_counts;    // entries per build record
     DevBuf<uint32_t> rec_offsets;   // output offset per build record
+    DevBuf<uint64_t> rec_masks;     // passing candidates 0..63 per build record
     DevBuf<uint32_t> dkeys[2];      // depth words of the visible Gaussians
     DevBuf<int32_t> dorder[2];      // their build-record indices
     DevBuf<float4> records;         // build records of the visible Gaussians (2 x float4 each)
@@ -125,6 +130,7 @@ gr_status gr_renderer_create(gr_renderer** out, int32_t device, void* stream) {
         return st;
     }
     for (auto& e : r->ev) R_HIP(hipEventCreate(&e));
+    for (auto& e : r->pass_ev) R_HIP(hipEventCreate(&e));
     *out = r;
     return GR_OK;
 }
@@ -140,9 +146,10 @@ gr_status gr_renderer_destroy(gr_renderer* r) {
     r->values[0].release(); r->values[1].release(); r->out.release();
     r->vis_bits.release(); r->vis_counts.release(); r->batch_counts.release();
     r->dkeys[0].release(); r->dkeys[1].release(); r->dorder[0].release(); r->dorder[1].release();
-    r->records.release(); r->rec_counts.release(); r->rec_offsets.release();
+    r->records.release(); r->rec_counts.release(); r->rec_offsets.release(); r->rec_masks.release();
     if (r->sort_temp) (void)hipFree(r->sort_temp);
     for (auto& e : r->ev) if (e) (void)hipEventDestroy(e);
+    for (auto& e : r->pass_ev) if (e) (void)hipEventDestroy(e);
     gr_context_destroy(r->ctx);
     delete r;
     return GR_OK;
@@ -247,12 +254,18 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     R_CHECK(r->batch_counts.ensure(nn + 2 + scan_scratch_words((int32_t)nn)));
     R_CHECK(r->rec_counts.ensure(nn));
     R_CHECK(r->rec_offsets.ensure(nn));
+    R_CHECK(r->rec_masks.ensure(nn));
     for (int k = 0; k < 2; ++k) {
+        R_CHECK(r->keys[k].ensure(1));
+        R_CHECK(r->values[k].ensure(1));
         R_CHECK(r->dkeys[k].ensure(nn));
         R_CHECK(r->dorder[k].ensure(nn));
     }
     R_CHECK(r->records.ensure(2 * nn));
-    R_CHECK(ensure_temp(r->sort_temp, r->sort_temp_bytes, radix_sort_temp_bytes((int32_t)nn, 0, 32)));
+    // one scratch serves the depth sort and the tile-bit sort; sized before anything is enqueued
+    const int32_t capacity = (int32_t)std::min<size_t>(std::min(r->keys[0].cap, r->values[0].cap), (1u << 30) - 1);
+    R_CHECK(ensure_temp(r->sort_temp, r->sort_temp_bytes,
+                        std::max(radix_sort_temp_bytes((int32_t)nn, 0, 32), radix_sort_temp_bytes(capacity, 32, 64))));
 
     ViewDesc vd{};
     vd.cam = camera_args(*cam);
@@ -313,20 +326,23 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         a.nviews = 1;
         R_HIP(launch_compact_depth(a, stream));
     }
-    // exact entry count of every record (index order)
+    // exact entry count and candidate mask of every record (index order)
     BuildArgs b{};
     b.records = r->records.p;
     b.record_count = vis_total;
     b.record_counts = r->rec_counts.p;
+    b.record_masks = r->rec_masks.p;
     b.record_offsets = r->rec_offsets.p;
+    b.keys = r->keys[0].p;
+    b.values = r->values[0].p;
     b.count = n;
-    b.capacity = (int32_t)std::min<size_t>(r->keys[0].cap, (size_t)0x7fffffff);
+    b.capacity = capacity;
     b.batches = batches;
     b.nviews = 1;
     b.v0 = vd;
     R_HIP(launch_build_count(b, stream));
     R_HIP(hipEventRecord(r->ev[3], stream));
-    // depth order -> record offsets (count scan in depth order)
+    // depth order of the records
     int32_t dsel = 0;
     {
         uint32_t* dk[2] = {r->dkeys[0].p, r->dkeys[1].p};
@@ -334,45 +350,33 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         R_HIP(radix_sort_pairs_u32(dk, dv, &dsel, n, vis_total, 0, 32, r->sort_temp, r->sort_temp_bytes, stream));
     }
     R_HIP(hipEventRecord(r->ev[4], stream));
+    // output offsets: counts in depth order, scanned (total K), scattered back to the records
     R_HIP(launch_gather_counts(r->rec_counts.p, r->dorder[dsel].p, vis_total, r->batch_counts.p, n, stream));
-    R_HIP(launch_build_scan(r->batch_counts.p, n, k_total, k_total + 1, stream));
-    R_HIP(launch_scatter_offsets(r->batch_counts.p, r->dorder[dsel].p, vis_total, r->rec_offsets.p, n, stream));
-    R_HIP(hipMemcpyAsync(ctx->pinned, k_total, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    R_HIP(hipStreamSynchronize(stream));
-    const int64_t need = (int64_t)ctx->pinned[0];
-    if (need >= (int64_t)(1 << 30))
-        return rfail(GR_ERR_CAPACITY, "tile list of " + std::to_string(need) + " entries exceeds 2^30");
-    if (need > b.capacity || !r->keys[0].p) {
-        const size_t want = (size_t)std::max<int64_t>(need, 1);
-        for (int k = 0; k < 2; ++k) {
-            R_CHECK(r->keys[k].ensure(want, 1.25));
-            R_CHECK(r->values[k].ensure(want, 1.25));
-        }
-        b.capacity = (int32_t)std::min<size_t>(r->keys[0].cap, (size_t)0x7fffffff);
-    }
-    b.keys = r->keys[0].p;
-    b.values = r->values[0].p;
-    const int32_t size = (int32_t)need;
-    if (size > 0) R_HIP(launch_build_emit(b, stream));
+    R_HIP(launch_build_scan_scatter(r->batch_counts.p, n, k_total, k_total + 1, r->dorder[dsel].p, vis_total,
+                                    r->rec_offsets.p, stream));
+    const int tile_bits = tile_bits_for(tiles);
+    R_HIP(radix_sort_prepare(capacity, r->sort_temp, r->sort_temp_bytes, &b.tile_hist, stream));
+    b.tile_passes = (tile_bits + 7) / 8;
+    R_HIP(launch_build_emit(b, stream));
     R_HIP(hipEventRecord(r->ev[5], stream));
     // tile bits
     int32_t sel = 0;
-    if (size > 1) {
-        R_CHECK(ensure_temp(r->sort_temp, r->sort_temp_bytes, radix_sort_temp_bytes(size, 32, 32 + tile_bits_for(tiles))));
+    {
         uint64_t* kb[2] = {r->keys[0].p, r->keys[1].p};
         int32_t* vb[2] = {r->values[0].p, r->values[1].p};
-        R_HIP(radix_sort_pairs(kb, vb, &sel, size, nullptr, 32, 32 + tile_bits_for(tiles), r->sort_temp,
-                               r->sort_temp_bytes, stream));
+        R_HIP(radix_sort_pairs_prepared(kb, vb, &sel, capacity, k_total, 32, 32 + tile_bits, r->sort_temp,
+                                        r->sort_temp_bytes, stream, r->pass_ev[0], r->pass_ev[1]));
+        r->last_passes = capacity > 1 ? (tile_bits + 7) / 8 : 0;
     }
     R_HIP(hipEventRecord(r->ev[6], stream));
     r->sel = sel;
     R_HIP(launch_fill_i32(r->tile_range.p, -1, 2 * (int64_t)tiles, stream));
-    if (size > 0) {
+    {
         RangeArgs ra{};
         ra.keys = r->keys[sel].p;
         ra.tile_range = r->tile_range.p;
-        ra.size_dev = nullptr;
-        ra.size = size;
+        ra.size_dev = k_total;
+        ra.size = capacity;
         ra.tiles = tiles;
         R_HIP(launch_ranges(ra, stream));
     }
@@ -393,24 +397,49 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     R_HIP(hipEventRecord(r->ev[7], stream));
     if (!out_is_device)
         R_HIP(hipMemcpyAsync(out_rgba8, outp, (size_t)width * height * 4, hipMemcpyDeviceToHost, stream));
+    R_HIP(hipMemcpyAsync(ctx->pinned, k_total, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     R_HIP(hipEventRecord(r->ev[8], stream));
     R_HIP(hipEventSynchronize(r->ev[8]));
+    const int64_t need = (int64_t)ctx->pinned[0];
+    if (need > capacity) {
+        // the list was too small: nothing was sorted or drawn; grow it and render the frame again
+        if (need >= (int64_t)(1 << 30))
+            return rfail(GR_ERR_CAPACITY, "tile list of " + std::to_string(need) + " entries exceeds 2^30");
+        for (int k = 0; k < 2; ++k) {
+            R_CHECK(r->keys[k].ensure((size_t)need, 1.25));
+            R_CHECK(r->values[k].ensure((size_t)need, 1.25));
+        }
+        return gr_renderer_render(r, cam, width, height, out_rgba8, out_is_device, stats);
+    }
     r->last_k = need;
     r->last_w = width;
     r->last_h = height;
     if (stats) {
-        // events: 0 SH 1 project 2 compaction + count 3 depth sort 4 offsets, K readback, emit
+        // events: 0 SH 1 project 2 compaction + count 3 depth sort 4 offsets + emit
         // 5 tile-bit sort 6 ranges + raster 7
         float t[8];
         for (int k = 0; k < 7; ++k) R_HIP(hipEventElapsedTime(&t[k], r->ev[k], r->ev[k + 1]));
         R_HIP(hipEventElapsedTime(&t[7], r->ev[0], r->ev[7]));
         stats->sh_eval_ms = t[0];
         stats->project_ms = t[1];
-        stats->build_tile_ms = t[2] + t[4];   // compaction, count, offsets, K readback, emit
+        stats->build_tile_ms = t[2] + t[4];   // compaction, count, offsets, emit
         stats->sort_ms = t[3] + t[5];         // depth sort + tile-bit sort
         stats->raster_ms = t[6];              // ranges + raster
         stats->total_ms = t[7];
         stats->tile_list_size = need;
+    }
+    return GR_OK;
+}
+
+gr_status gr_renderer_kernel_times(gr_renderer* r, gr_kernel_times* out) {
+    if (!r || !out) return rfail(GR_ERR_INVALID_ARGUMENT, "gr_renderer_kernel_times: NULL argument");
+    *out = gr_kernel_times{};
+    out->tile_sort_passes = r->last_passes;
+    out->tile_list_size = r->last_k;
+    if (r->last_passes > 0) {
+        float ms = 0.0f;
+        R_HIP(hipEventElapsedTime(&ms, r->pass_ev[0], r->pass_ev[1]));
+        out->tile_sort_pass_ms = ms;
     }
     return GR_OK;
 }

@@ -8,12 +8,12 @@
 // mix of resolutions) runs on one slot in two phases:
 //   A  camera block H2D -> SH (all views) -> projection (all views, with the
 //      visibility ballot) -> compaction of every view's visible Gaussians
-//      into build records -> build count per record -> one 32-bit sort of
-//      all views' records by depth word -> record offsets from the counts in
-//      depth order -> 4-byte K readback
-//   B  emit (each record at its offset) -> onesweep sort of the tile bits
-//      (device-side K) -> ranges -> raster (all views) -> images D2H into
-//      pinned staging
+//      into build records -> build count per record (entries and mask of
+//      passing candidates) -> one 32-bit sort of all views' records by depth
+//      word -> output offsets from the counts in depth order -> K readback
+//   B  emit (entries at their depth-order offsets, tile-digit histograms)
+//      -> onesweep sort of the tile bits (device-side K) -> ranges -> raster
+//      (all views) -> images D2H into pinned staging
 // Tile ids are global over the batch's views, so one sort serves all views,
 // and within a tile entries keep (depth word, index) order.
 // Between A and B the host reads K and grows the slot's tile-list buffers
@@ -146,6 +146,7 @@ struct Workspace {
     Dev dsort_temp;                   // depth sort scratch
     Dev counts;                       // record counts in depth order -> offsets [V*N] + total + scratch
     Dev rec_counts, rec_offsets;      // [V*N] entries and output offset per record
+    Dev rec_masks;                    // [V*N] passing candidates 0..63 per record
     Dev keys0, keys1, vals0, vals1;   // tile lists
     Dev sort_temp;
     Dev ranges;                       // 2 * tiles
@@ -155,7 +156,7 @@ struct Workspace {
     uint64_t generation = 0;          // bumps when any device pointer changes
     std::vector<Dev*> devs() {
         return {&color, &xy, &z, &ell, &con, &vis_bits, &vis_counts, &dkeys0, &dkeys1, &dorder0, &dorder1,
-                &records, &rec_counts, &rec_offsets, &dsort_temp, &counts, &keys0, &keys1, &vals0, &vals1, &sort_temp, &ranges, &images, &views};
+                &records, &rec_counts, &rec_offsets, &rec_masks, &dsort_temp, &counts, &keys0, &keys1, &vals0, &vals1, &sort_temp, &ranges, &images, &views};
     }
 };
 
@@ -278,6 +279,7 @@ std::vector<PhaseANeed> phase_a_needs(Slot& sl, const Scene& sc, size_t nviews, 
             {&w.counts, 4 * (V * N + 2 + scan_scratch_words((int32_t)(V * N)))},
             {&w.rec_counts, V * N * 4},
             {&w.rec_offsets, V * N * 4},
+            {&w.rec_masks, V * N * 8},
             {&w.images, img_bytes},
             {&w.ranges, 8 * (int64_t)tiles_total},
             {&w.views, V * (int64_t)sizeof(ViewDesc)}};
@@ -345,6 +347,7 @@ BuildArgs record_build_args(Slot& sl, const Scene& sc) {
     b.record_count = w.vis_counts.as<uint32_t>() + nbv;
     b.record_counts = w.rec_counts.as<uint32_t>();
     b.record_offsets = w.rec_offsets.as<uint32_t>();
+    b.record_masks = w.rec_masks.as<uint64_t>();
     b.count = N;
     b.capacity = (int32_t)sl.capacity;
     b.batches = (int32_t)(((int64_t)V * N + kBuildBatch - 1) / kBuildBatch);
@@ -423,9 +426,10 @@ gr_status enqueue_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
                                sl.stream));
     uint32_t* counts = w.counts.as<uint32_t>();
     const int vn = V * N;
+    // output offsets: counts in depth order, scanned (K), scattered back to the records
     S_HIP(launch_gather_counts(w.rec_counts.as<uint32_t>(), dv[dsel], vis + nbv, counts, vn, sl.stream));
-    S_HIP(launch_build_scan(counts, vn, counts + vn, counts + vn + 2, sl.stream));
-    S_HIP(launch_scatter_offsets(counts, dv[dsel], vis + nbv, w.rec_offsets.as<uint32_t>(), vn, sl.stream));
+    S_HIP(launch_build_scan_scatter(counts, vn, counts + vn, counts + vn + 2, dv[dsel], vis + nbv,
+                                    w.rec_offsets.as<uint32_t>(), sl.stream));
     S_HIP(hipMemcpyAsync(w.k_host.p, counts + vn, 4, hipMemcpyDeviceToHost, sl.stream));
     return GR_OK;
 }
@@ -437,14 +441,16 @@ gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
     const ViewDesc* vhost = w.views_host.as<ViewDesc>();
     ViewDesc* vdev = w.views.as<ViewDesc>();
     uint32_t* counts = w.counts.as<uint32_t>();
-    BuildArgs b = record_build_args(sl, sc);
     const int nb = V * N;   // the device-side K sits after the V*N scanned counts
+    BuildArgs b = record_build_args(sl, sc);
+    S_HIP(radix_sort_prepare((int32_t)sl.capacity, w.sort_temp.p, w.sort_temp.bytes, &b.tile_hist, sl.stream));
+    b.tile_passes = (sl.end_bit - 32 + 7) / 8;
     S_HIP(launch_build_emit(b, sl.stream));
     uint64_t* kb[2] = {w.keys0.as<uint64_t>(), w.keys1.as<uint64_t>()};
     int32_t* vb[2] = {w.vals0.as<int32_t>(), w.vals1.as<int32_t>()};
     int32_t sel = 0;
-    S_HIP(radix_sort_pairs(kb, vb, &sel, (int32_t)sl.capacity, counts + nb, 32, sl.end_bit, w.sort_temp.p,
-                           w.sort_temp.bytes, sl.stream));
+    S_HIP(radix_sort_pairs_prepared(kb, vb, &sel, (int32_t)sl.capacity, counts + nb, 32, sl.end_bit, w.sort_temp.p,
+                                    w.sort_temp.bytes, sl.stream));
     S_HIP(launch_fill_i32(w.ranges.as<int32_t>(), -1, 2 * (int64_t)sl.tiles_total, sl.stream));
     RangeArgs r{};
     r.keys = kb[sel];
@@ -557,6 +563,22 @@ bool other_slots_busy(gr_server* s, const Slot& me) {
     return false;
 }
 
+// Another slot has work on the GPU (it frees or stops claiming memory by itself).
+bool other_slots_running(gr_server* s, const Slot& me) {
+    for (const Slot& o : s->slots)
+        if (&o != &me && (o.state == SlotState::PhaseA || o.state == SlotState::PhaseB)) return true;
+    return false;
+}
+
+// Another slot, earlier in the slot list, waits for memory.
+bool earlier_slot_waits(gr_server* s, const Slot& me) {
+    for (const Slot& o : s->slots) {
+        if (&o == &me) return false;
+        if (o.state == SlotState::WaitMemory) return true;
+    }
+    return false;
+}
+
 void release_idle_workspaces(gr_server* s, const Slot& me) {
     for (Slot& o : s->slots) {
         if (&o == &me || o.state != SlotState::Idle) continue;
@@ -567,6 +589,9 @@ void release_idle_workspaces(gr_server* s, const Slot& me) {
 
 // Starts phase A of a new batch on an idle slot; returns true if it launched.
 bool start_batch(gr_server* s, Slot& sl) {
+    // a slot waiting for memory goes first: new batches would take the memory it waits for
+    for (const Slot& o : s->slots)
+        if (o.state == SlotState::WaitMemory) return false;
     if (!form_batch(s, sl)) return false;
     Scene* sc = scene_ptr(s, sl.scene);
     if (!sc) {
@@ -653,8 +678,6 @@ bool start_batch(gr_server* s, Slot& sl) {
         return false;
     }
     sl.state = SlotState::PhaseA;
-    ++s->batches;
-    s->views += (int64_t)sl.batch.size();
     return true;
 }
 
@@ -684,10 +707,19 @@ bool start_phase_b(gr_server* s, Slot& sl) {
     size_t need = std::max<size_t>(sl.k, 1);
     int64_t grow = growth_for(need);
     if (grow > 0 && s->over_budget(grow)) {
-        if (other_slots_busy(s, sl)) {
+        if (other_slots_running(s, sl)) {
             ++s->budget_waits;
             sl.state = SlotState::WaitMemory;
             return false;
+        }
+        if (earlier_slot_waits(s, sl)) {
+            // nothing runs and waiting slots would wait on each other: the earliest waiting slot
+            // goes on, the others hand their batches and memory back
+            release_workspace(s, sl);
+            drop_graphs(sl);
+            s->batch_limit[sl.scene] = (int)sl.batch.size();
+            requeue_front(s, sl);
+            return true;
         }
         release_idle_workspaces(s, sl);
         if (s->over_budget(grow) && sl.batch.size() > 1) {
@@ -755,6 +787,8 @@ void finish_batch(gr_server* s, Slot& sl) {
         if (p.req.out_rgba8) std::memcpy(p.req.out_rgba8, sl.ws.img_host.as<uint8_t>() + off, bytes);
         off += (int64_t)bytes;
     }
+    ++s->batches;
+    s->views += (int64_t)sl.batch.size();
     complete(s, sl, GR_OK);
     sl.batch.clear();
     sl.state = SlotState::Idle;

@@ -174,7 +174,7 @@ __device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
 template <typename KeyT>
 __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     const KeyT* __restrict__ kin, const int32_t* __restrict__ vin, KeyT* __restrict__ kout,
-    int32_t* __restrict__ vout, int32_t n, const uint32_t* count_dev, int shift, int bits,
+    int32_t* __restrict__ vout, int32_t n, const uint32_t* count_dev, int strict, int shift, int bits,
     const uint32_t* __restrict__ digit_off, uint32_t* status, uint32_t* status_next, uint32_t* ticket) {
     __shared__ uint32_t s_part;
     __shared__ uint32_t s_wcnt[kSortWaves][kRadix];
@@ -189,7 +189,12 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     for (int i = t; i < kSortWaves * kRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0u;
     __syncthreads();
     const int part = (int)s_part;
-    if (count_dev) n = (int32_t)min((uint32_t)n, *count_dev);
+    if (count_dev) {
+        // strict (histograms made by the producer): a count above the capacity means the list
+        // was not written -- sort nothing
+        if (strict && *count_dev > (uint32_t)n) return;
+        n = (int32_t)min((uint32_t)n, *count_dev);
+    }
     if ((int64_t)part * kSortTile >= n) return;   // beyond the device-side count: whole workgroup
     if (status_next && t < kRadix) status_next[(size_t)part * kRadix + t] = 0u;
 
@@ -301,45 +306,81 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     }
 }
 
+static int sort_passes(int32_t begin_bit, int32_t end_bit) { return (end_bit - begin_bit + 7) / 8; }
+
+// Zeroes the histograms, the partition tickets and the first look-back region.
+static hipError_t sort_prepare(const SortLayout& l, char* tb, hipStream_t s) {
+    uint32_t* hist = reinterpret_cast<uint32_t*>(tb + l.hist_off);
+    uint32_t* tickets = reinterpret_cast<uint32_t*>(tb + l.ticket_off);
+    uint32_t* status0 = reinterpret_cast<uint32_t*>(tb + l.status_off[0]);
+    const int64_t status_n = (int64_t)l.partitions * kRadix;
+    int64_t init_blocks = (status_n + 255) / 256;
+    if (init_blocks < (kMaxPasses * kRadix + 255) / 256) init_blocks = (kMaxPasses * kRadix + 255) / 256;
+    if (init_blocks > 4096) init_blocks = 4096;
+    hipLaunchKernelGGL(sort_init_kernel, dim3((unsigned)init_blocks), dim3(256), 0, s, hist, tickets, status0,
+                       status_n);
+    return hipGetLastError();
+}
+
 template <typename KeyT>
-static hipError_t sort_pairs_t(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
-                               const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
-                               size_t temp_bytes, hipStream_t s) {
+static hipError_t sort_check(int32_t count, int32_t begin_bit, int32_t end_bit, const void* temp, size_t temp_bytes) {
     constexpr int kKeyBits = 8 * (int)sizeof(KeyT);
-    if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
     if ((uint32_t)count > kCountMask || begin_bit < 0 || end_bit > kKeyBits) return hipErrorInvalidValue;
-    const int passes = (end_bit - begin_bit + 7) / 8;
-    if (passes > kMaxPasses) return hipErrorInvalidValue;
+    if (sort_passes(begin_bit, end_bit) > kMaxPasses) return hipErrorInvalidValue;
+    if (temp_bytes < sort_layout(count < 1 ? 1 : count).total || temp == nullptr) return hipErrorInvalidValue;
+    return hipSuccess;
+}
+
+// Histogram scan + the onesweep passes (histograms already accumulated).
+template <typename KeyT>
+static hipError_t sort_passes_t(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                                const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, char* tb, int strict,
+                                hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) {
     const SortLayout l = sort_layout(count);
-    if (temp_bytes < l.total || temp == nullptr) return hipErrorInvalidValue;
-    char* tb = static_cast<char*>(temp);
+    const int passes = sort_passes(begin_bit, end_bit);
     uint32_t* hist = reinterpret_cast<uint32_t*>(tb + l.hist_off);
     uint32_t* tickets = reinterpret_cast<uint32_t*>(tb + l.ticket_off);
     uint32_t* status[2] = {reinterpret_cast<uint32_t*>(tb + l.status_off[0]),
                            reinterpret_cast<uint32_t*>(tb + l.status_off[1])};
-    const int64_t status_n = (int64_t)l.partitions * kRadix;
-
-    int64_t init_blocks = (status_n + 255) / 256;
-    if (init_blocks < (kMaxPasses * kRadix + 255) / 256) init_blocks = (kMaxPasses * kRadix + 255) / 256;
-    if (init_blocks > 4096) init_blocks = 4096;
-    hipLaunchKernelGGL(sort_init_kernel, dim3((unsigned)init_blocks), dim3(256), 0, s, hist, tickets,
-                       status[0], status_n);
-    int hist_blocks = (count + kHistChunk - 1) / kHistChunk;
-    if (hist_blocks > 1024) hist_blocks = 1024;
-    hipLaunchKernelGGL(sort_hist_kernel<KeyT>, dim3(hist_blocks), dim3(kHistThreads), 0, s, keys[*sel], count,
-                       count_dev, begin_bit, end_bit, passes, hist);
     hipLaunchKernelGGL(sort_hist_scan_kernel, dim3(passes), dim3(256), 0, s, hist);
+    if (ev_begin) {
+        const hipError_t e = hipEventRecord(ev_begin, s);
+        if (e != hipSuccess) return e;
+    }
     for (int p = 0; p < passes; ++p) {
         const int shift = begin_bit + 8 * p;
         const int bits = min(8, end_bit - shift);
         const int cur = *sel;
         hipLaunchKernelGGL(onesweep_kernel<KeyT>, dim3(l.partitions), dim3(kSortThreads), 0, s, keys[cur],
-                           values[cur], keys[cur ^ 1], values[cur ^ 1], count, count_dev, shift, bits,
+                           values[cur], keys[cur ^ 1], values[cur ^ 1], count, count_dev, strict, shift, bits,
                            hist + p * kRadix, status[p & 1], (p + 1 < passes) ? status[(p + 1) & 1] : nullptr,
                            tickets + p);
         *sel = cur ^ 1;
     }
+    if (ev_end) {
+        const hipError_t e = hipEventRecord(ev_end, s);
+        if (e != hipSuccess) return e;
+    }
     return hipGetLastError();
+}
+
+template <typename KeyT>
+static hipError_t sort_pairs_t(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                               const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                               size_t temp_bytes, hipStream_t s) {
+    if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
+    hipError_t e = sort_check<KeyT>(count, begin_bit, end_bit, temp, temp_bytes);
+    if (e != hipSuccess) return e;
+    const SortLayout l = sort_layout(count);
+    char* tb = static_cast<char*>(temp);
+    e = sort_prepare(l, tb, s);
+    if (e != hipSuccess) return e;
+    int hist_blocks = (count + kHistChunk - 1) / kHistChunk;
+    if (hist_blocks > 1024) hist_blocks = 1024;
+    hipLaunchKernelGGL(sort_hist_kernel<KeyT>, dim3(hist_blocks), dim3(kHistThreads), 0, s, keys[*sel], count,
+                       count_dev, begin_bit, end_bit, sort_passes(begin_bit, end_bit),
+                       reinterpret_cast<uint32_t*>(tb + l.hist_off));
+    return sort_passes_t<KeyT>(keys, values, sel, count, count_dev, begin_bit, end_bit, tb, 0, s);
 }
 
 hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
@@ -352,6 +393,25 @@ hipError_t radix_sort_pairs_u32(uint32_t* keys[2], int32_t* values[2], int32_t* 
                                 const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                                 size_t temp_bytes, hipStream_t s) {
     return sort_pairs_t<uint32_t>(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
+}
+
+hipError_t radix_sort_prepare(int32_t count, void* temp, size_t temp_bytes, uint32_t** hist, hipStream_t s) {
+    if (count < 1) count = 1;
+    const SortLayout l = sort_layout(count);
+    if (temp_bytes < l.total || temp == nullptr || hist == nullptr) return hipErrorInvalidValue;
+    *hist = reinterpret_cast<uint32_t*>(static_cast<char*>(temp) + l.hist_off);
+    return sort_prepare(l, static_cast<char*>(temp), s);
+}
+
+hipError_t radix_sort_pairs_prepared(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                                     const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                                     size_t temp_bytes, hipStream_t s, hipEvent_t passes_begin,
+                                     hipEvent_t passes_end) {
+    if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
+    hipError_t e = sort_check<uint64_t>(count, begin_bit, end_bit, temp, temp_bytes);
+    if (e != hipSuccess) return e;
+    return sort_passes_t<uint64_t>(keys, values, sel, count, count_dev, begin_bit, end_bit, static_cast<char*>(temp),
+                                   1, s, passes_begin, passes_end);
 }
 
 }  // namespace gr

@@ -12,6 +12,8 @@
 //                  quadrants), entries staged through LDS 256 at a time,
 //                  front-to-back blending, tile-wide stop vote every 32
 //                  entries (the reference's rule).
+#include <algorithm>
+
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -315,10 +317,10 @@ hipError_t launch_project(const ProjectArgs& a, hipStream_t s) {
 // Key: getKey :306-310.  The reference lets atomics decide entry order
 // (:485, :498); here entries follow Gaussian index, then row-major candidate
 // order, so the stable sort that follows is deterministic.  The record walk
-// (RECORDS) runs over the compacted visible Gaussians of all views (a wave's
-// 64 records may belong to different views); its count pass writes each
-// record's entry count and its emit pass writes each record's entries at an
-// offset assigned in depth order.
+// runs over the compacted visible Gaussians of all views (a wave's 64 records
+// may belong to different views); its count pass writes each record's entry
+// count and candidate mask and its emit pass writes each record's entries at
+// an offset assigned in depth order.
 // ---------------------------------------------------------------------------
 // IEEE quotient a / b from rb = RN(1/b): q0 = RN(a*rb), e = a - b*q0 (exact
 // through the FMA), q = RN(q0 + e*rb) is the correctly rounded quotient when
@@ -395,173 +397,351 @@ __device__ __forceinline__ void divmod_small(int t, int n, int& q, int& r) {
     else if (r >= n) { ++q; r -= n; }
 }
 
-template <bool EMIT, bool RECORDS>
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t value, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+// Wave-wide inclusive scan.
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int n = __shfl_up(v, d, 64);
+        if (lane >= d) v += n;
+    }
+    return v;
+}
+
+// Record index o (0..63) of candidate/entry c: the last o with excl[o] <= c
+// (excl non-decreasing, excl[0] = 0).
+__device__ __forceinline__ int owner_of(const int* excl, int c) {
+    int o = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1)
+        if (excl[o + step] <= c) o += step;
+    return o;
+}
+
+// Position of the k-th set bit of m (k < popcount(m)), by halving.
+__device__ __forceinline__ int select_bit(uint64_t m, int k) {
+    int pos = 0;
+    uint32_t x = (uint32_t)m;
+    int c = __popc(x);
+    if (k >= c) {
+        k -= c;
+        pos = 32;
+        x = (uint32_t)(m >> 32);
+    }
+    c = __popc(x & 0xffffu);
+    if (k >= c) { k -= c; pos += 16; x >>= 16; }
+    c = __popc(x & 0xffu);
+    if (k >= c) { k -= c; pos += 8; x >>= 8; }
+    c = __popc(x & 0xfu);
+    if (k >= c) { k -= c; pos += 4; x >>= 4; }
+    c = __popc(x & 0x3u);
+    if (k >= c) { k -= c; pos += 2; x >>= 2; }
+    return pos + (k >= (int)(x & 1u) ? 1 : 0);
+}
+
+__device__ __forceinline__ uint64_t low_bits(int n) { return n >= 64 ? ~0ull : ((1ull << n) - 1ull); }
+
+// Index-order walk (the stage-level gr_build_tile_list): one wave per
+// 64-Gaussian batch of one view.  The wave lays its Gaussians' candidate tiles
+// (rect cells, row-major) end to end and tests 64 candidates per step, one per
+// lane; the count pass writes the batch's entry count, the emit pass writes
+// the entries at the batch's scanned offset in (Gaussian, row, column) order.
+template <bool EMIT>
 __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
     constexpr int W = kBuildThreads / 64;
     __shared__ int s_excl[W][64];
     __shared__ int4 s_rect[W][64];          // x0, y0, nx, depth word
     __shared__ EllipseRec s_ell[W][64];
-    __shared__ int4 s_tile[RECORDS ? W : 1][64];   // RECORDS: tile_base, tiles_x, tile clip x/y bits
-    __shared__ int s_gidx[RECORDS ? W : 1][64];    // RECORDS: Gaussian index within the scene
-    __shared__ uint32_t s_start[RECORDS ? W : 1][64];  // RECORDS: entries of the wave before each owner
-    __shared__ uint32_t s_off[(RECORDS && EMIT) ? W : 1][64];  // RECORDS emit: output offset of each owner
-
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int batch = blockIdx.x * W + w;
-    const int first = batch * kBuildBatch;
-    int limit = a.count;
-    if constexpr (RECORDS) {
-        limit = (int)min(*a.record_count, (uint32_t)a.nviews * (uint32_t)a.count);
-    }
-    if (first >= limit) return;   // wave-uniform; no block barrier below
-    const int pos = first + lane;
-    int view, gi = -1;
-    float2 c = make_float2(0.0f, 0.0f);
-    float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    uint32_t dw = 0u;
-    if constexpr (RECORDS) {
-        view = 0;
-        if (pos < limit) {
-            const float4 r0 = a.records[2 * (size_t)pos];
-            const float4 r1 = a.records[2 * (size_t)pos + 1];
-            const int pg = __float_as_int(r1.w);
-            view = pg / a.count;
-            gi = pg - view * a.count;
-            c = make_float2(r0.x, r0.y);
-            e = make_float4(r0.z, r0.w, r1.x, r1.y);
-            dw = __float_as_uint(r1.z);
-        }
-    } else {
-        view = blockIdx.y;
-        if (pos < a.count) {
-            gi = pos;
-            const size_t vo = (size_t)view * a.count;
-            c = a.clip_xy[vo + gi];
-            e = a.ellipse[vo + gi];
-            if (EMIT) dw = depth_word(a.clip_z[vo + gi]);
-        }
-    }
+    if (batch >= a.batches) return;   // wave-uniform; no block barriers below
+    const int view = blockIdx.y;
     const ViewDesc& vd = view_of(view, a.v0, a.views);
     const FrameGeom& geom = vd.geom;
-
+    const int first = batch * kBuildBatch;
     int cnt = 0;
-    if (gi >= 0) {
+    if (first + lane < a.count) {
+        const size_t vo = (size_t)view * a.count + first + lane;
+        const float2 c = a.clip_xy[vo];
+        const float4 e = a.ellipse[vo];
         const Footprint f = footprint(c.x, c.y, e.x, e.y, e.z, e.w, geom);
         // the reference keeps max(0, w*h); a rect with w <= 0 or h <= 0 only
         // arises from NaN corners and never passes the exact test
         cnt = (f.nx > 0 && f.ny > 0) ? f.nx * f.ny : 0;
         if (cnt > 0) {
-            s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, (int)dw);
+            s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, EMIT ? (int)depth_word(a.clip_z[vo]) : 0);
             s_ell[w][lane] = EllipseRec{c.x, c.y, e.x, e.y, e.z, e.w, 1.0f / e.z, 1.0f / e.w};
-            if constexpr (RECORDS) {
-                s_tile[w][lane] = make_int4(vd.tile_base, geom.tiles_x, __float_as_int(geom.tile_clip_x),
-                                            __float_as_int(geom.tile_clip_y));
-                s_gidx[w][lane] = gi;
-            }
         }
     }
-    if constexpr (RECORDS) {
-        s_start[w][lane] = 0u;
-        if constexpr (EMIT) s_off[w][lane] = pos < limit ? a.record_offsets[pos] : 0u;
-    }
-    int incl = cnt;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int n = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += n;
-    }
+    const int incl = wave_incl_scan(cnt, lane);
     const int total = __shfl(incl, 63, 64);
     s_excl[w][lane] = incl - cnt;
     wave_sync();
-
-    uint32_t running = 0;
     const size_t bslot = (size_t)view * a.batches + batch;
-    const uint32_t out_base = (EMIT && !RECORDS) ? a.batch_counts[bslot] : 0u;
+    const uint32_t out_base = EMIT ? a.batch_counts[bslot] : 0u;
+    uint32_t running = 0;
     for (int base = 0; base < total; base += 64) {
         const int cc = base + lane;
         bool pass = false;
-        int o = 0, gx = 0, gy = 0;
-        int4 rc = make_int4(0, 0, 1, 0);
+        int o = 0;
         uint32_t tile = 0;
         if (cc < total) {
-#pragma unroll
-            for (int step = 32; step >= 1; step >>= 1)
-                if (s_excl[w][o + step] <= cc) o += step;
-            rc = s_rect[w][o];
+            o = owner_of(s_excl[w], cc);
+            const int4 rc = s_rect[w][o];
             int q, r;
             divmod_small(cc - s_excl[w][o], rc.z, q, r);
-            gx = rc.x + r;
-            gy = rc.y + q;
-            if constexpr (RECORDS) {
-                const int4 tv = s_tile[w][o];
-                FrameGeom og;
-                og.tile_clip_x = __int_as_float(tv.z);
-                og.tile_clip_y = __int_as_float(tv.w);
-                pass = tile_overlap(gx, gy, s_ell[w][o], og);
-                tile = (uint32_t)(tv.x + gy * tv.y + gx);
-            } else {
-                pass = tile_overlap(gx, gy, s_ell[w][o], geom);
-                tile = (uint32_t)(vd.tile_base + gy * geom.tiles_x + gx);
-            }
+            pass = tile_overlap(rc.x + r, rc.y + q, s_ell[w][o], geom);
+            tile = (uint32_t)(vd.tile_base + (rc.y + q) * geom.tiles_x + rc.x + r);
         }
         const uint64_t m = __ballot(pass);
-        const uint32_t before = running + (uint32_t)__popcll(m & lanemask_lt(lane));
-        if constexpr (RECORDS) {
-            // the lane of an owner's first candidate records the wave's entry count there
-            if (cc < total && cc == s_excl[w][o]) s_start[w][o] = before;
-            wave_sync();
-            if (EMIT && pass) {
-                const uint32_t p = s_off[w][o] + (before - s_start[w][o]);
-                if (p < (uint32_t)a.capacity) {
-                    a.keys[p] = ((uint64_t)tile << 32) | (uint32_t)rc.w;
-                    a.values[p] = s_gidx[w][o];
-                }
-            }
-        } else if (EMIT && pass) {
-            const uint32_t p = out_base + before;
+        if (EMIT && pass) {
+            const uint32_t p = out_base + running + (uint32_t)__popcll(m & lanemask_lt(lane));
             if (p < (uint32_t)a.capacity) {
-                a.keys[p] = ((uint64_t)tile << 32) | (uint32_t)rc.w;
+                a.keys[p] = ((uint64_t)tile << 32) | (uint32_t)s_rect[w][o].w;
                 a.values[p] = first + o;
             }
         }
         running += (uint32_t)__popcll(m);
     }
-    if constexpr (RECORDS) {
-        if (!EMIT && pos < limit) {
-            // owners are consecutive and every record has at least one candidate, so an owner's
-            // entries are the difference of consecutive start counts
-            wave_sync();
-            const int last = min(63, limit - 1 - first);
-            const uint32_t next = lane < last ? s_start[w][lane + 1] : running;
-            a.record_counts[pos] = next - s_start[w][lane];
+    if (!EMIT && lane == 0) a.batch_counts[bslot] = running;
+}
+
+// Record walk, count pass: one wave per 64 consecutive records in index
+// order (which spreads small and large footprints evenly over the waves; a
+// wave's records may belong to different views).  Per record it writes the
+// exact entry count and a 64-bit mask whose bit l says whether candidate l
+// (row-major in the footprint) passed the exact test -- the whole answer for
+// footprints of at most 64 tiles, so the emit pass needs no tests for them.
+__global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a) {
+    constexpr int W = kBuildThreads / 64;
+    __shared__ int s_excl[W][64];
+    __shared__ int4 s_rect[W][64];       // x0, y0, nx, candidates
+    __shared__ EllipseRec s_ell[W][64];
+    __shared__ float2 s_clip[W][64];     // tile size of the record's view, clip units
+    __shared__ uint32_t s_cnt[W][64];
+    __shared__ uint64_t s_mask[W][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int limit = (int)min(*a.record_count, (uint32_t)a.nviews * (uint32_t)a.count);
+    const int first = (blockIdx.x * W + w) * kBuildBatch;
+    if (first >= limit) return;   // wave-uniform; no block barriers below
+    const int pos = first + lane;
+    int cnt = 0;
+    if (pos < limit) {
+        const float4 r0 = a.records[2 * (size_t)pos];
+        const float4 r1 = a.records[2 * (size_t)pos + 1];
+        const int view = __float_as_int(r1.w) / a.count;
+        const FrameGeom& g = view_of(view, a.v0, a.views).geom;
+        const Footprint f = footprint(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, g);
+        cnt = (f.nx > 0 && f.ny > 0) ? f.nx * f.ny : 0;
+        s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, cnt);
+        s_ell[w][lane] = EllipseRec{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, 1.0f / r1.x, 1.0f / r1.y};
+        s_clip[w][lane] = make_float2(g.tile_clip_x, g.tile_clip_y);
+    }
+    s_cnt[w][lane] = 0u;
+    s_mask[w][lane] = 0ull;
+    const int incl = wave_incl_scan(cnt, lane);
+    const int total = __shfl(incl, 63, 64);
+    s_excl[w][lane] = incl - cnt;
+    wave_sync();
+    for (int base = 0; base < total; base += 64) {
+        const int cc = base + lane;
+        bool pass = false;
+        int o = 0, l = 0;
+        if (cc < total) {
+            o = owner_of(s_excl[w], cc);
+            l = cc - s_excl[w][o];
+            const int4 rc = s_rect[w][o];
+            int q, r;
+            divmod_small(l, rc.z, q, r);
+            FrameGeom og;
+            og.tile_clip_x = s_clip[w][o].x;
+            og.tile_clip_y = s_clip[w][o].y;
+            pass = tile_overlap(rc.x + r, rc.y + q, s_ell[w][o], og);
         }
-    } else {
-        if (!EMIT && lane == 0) a.batch_counts[bslot] = running;
+        const uint64_t m = __ballot(pass);
+        // the lane holding a record's first candidate of this step adds the record's part
+        if (cc < total && (l == 0 || lane == 0)) {
+            const int len = min(s_excl[w][o] + s_rect[w][o].w, base + 64) - cc;
+            const uint64_t bits = (m >> lane) & low_bits(len);
+            s_cnt[w][o] += (uint32_t)__popcll(bits);
+            if (l < 64) s_mask[w][o] |= bits << l;
+        }
+        wave_sync();
+    }
+    if (pos < limit) {
+        a.record_counts[pos] = s_cnt[w][lane];
+        a.record_masks[pos] = s_mask[w][lane];
+    }
+}
+
+// Record walk, emit pass: the same 64-record batches, each wave taking
+// batches grid-stride (one tile-digit histogram flush per workgroup).  Record
+// o's entries go to record_offsets[o] + 0, 1, ... in candidate order.  For
+// footprints of at most 64 tiles the wave walks the entries themselves, 64
+// per step: entry k of record o is candidate select_bit(mask_o, k), so
+// consecutive lanes write consecutive positions of a record.  Larger
+// footprints are walked candidate by candidate with the exact test.  Nothing
+// at or past `capacity` is written (the list is then rendered again with room).
+__global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a) {
+    constexpr int W = kBuildThreads / 64;
+    __shared__ int s_excl[W][64];
+    __shared__ int4 s_rect[W][64];       // x0, y0, nx, tiles_x
+    __shared__ uint64_t s_mask[W][64];
+    __shared__ uint32_t s_off[W][64];
+    __shared__ uint32_t s_dw[W][64];
+    __shared__ int s_gi[W][64];
+    __shared__ int s_tb[W][64];          // tile base of the record's view
+    __shared__ EllipseRec s_ell[W][64];  // large footprints only
+    __shared__ float2 s_clip[W][64];
+    __shared__ uint32_t s_done[W][64];
+    __shared__ int s_cand[W][64];        // candidates of a large footprint
+    __shared__ uint32_t s_hist[3][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool hist = a.tile_hist != nullptr;
+    if (hist)
+        for (int i = threadIdx.x; i < 3 * 256; i += kBuildThreads) (&s_hist[0][0])[i] = 0u;
+    __syncthreads();
+    const int limit = (int)min(*a.record_count, (uint32_t)a.nviews * (uint32_t)a.count);
+    const uint32_t cap = (uint32_t)a.capacity;
+    auto put = [&](uint32_t p, uint32_t tile, int o) {
+        if (p < cap) {
+            a.keys[p] = ((uint64_t)tile << 32) | s_dw[w][o];
+            a.values[p] = s_gi[w][o];
+            if (hist)
+                for (int k = 0; k < a.tile_passes; ++k) atomicAdd(&s_hist[k][(tile >> (8 * k)) & 255u], 1u);
+        }
+    };
+    for (int first = (blockIdx.x * W + w) * kBuildBatch; first < limit; first += gridDim.x * W * kBuildBatch) {
+        const int pos = first + lane;
+        int ecnt = 0, big = 0;
+        if (pos < limit) {
+            const float4 r0 = a.records[2 * (size_t)pos];
+            const float4 r1 = a.records[2 * (size_t)pos + 1];
+            const int pg = __float_as_int(r1.w);
+            const int view = pg / a.count;
+            const ViewDesc& vd = view_of(view, a.v0, a.views);
+            const Footprint f = footprint(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, vd.geom);
+            const int cand = (f.nx > 0 && f.ny > 0) ? f.nx * f.ny : 0;
+            s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, vd.geom.tiles_x);
+            s_tb[w][lane] = vd.tile_base;
+            s_mask[w][lane] = a.record_masks[pos];
+            s_off[w][lane] = a.record_offsets[pos];
+            s_dw[w][lane] = __float_as_uint(r1.z);
+            s_gi[w][lane] = pg - view * a.count;
+            if (cand <= 64) {
+                ecnt = (int)a.record_counts[pos];
+            } else {
+                big = cand;
+                s_cand[w][lane] = cand;
+                s_ell[w][lane] = EllipseRec{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, 1.0f / r1.x, 1.0f / r1.y};
+                s_clip[w][lane] = make_float2(vd.geom.tile_clip_x, vd.geom.tile_clip_y);
+            }
+        }
+        // entries of the small footprints
+        int incl = wave_incl_scan(ecnt, lane);
+        int total = __shfl(incl, 63, 64);
+        s_excl[w][lane] = incl - ecnt;
+        wave_sync();
+        for (int base = 0; base < total; base += 64) {
+            const int e = base + lane;
+            if (e < total) {
+                const int o = owner_of(s_excl[w], e);
+                const int k = e - s_excl[w][o];
+                const int l = select_bit(s_mask[w][o], k);
+                const int4 rc = s_rect[w][o];
+                int q, r;
+                divmod_small(l, rc.z, q, r);
+                put(s_off[w][o] + (uint32_t)k, (uint32_t)(s_tb[w][o] + (rc.y + q) * rc.w + rc.x + r), o);
+            }
+        }
+        // candidates of the large footprints
+        wave_sync();
+        incl = wave_incl_scan(big, lane);
+        total = __shfl(incl, 63, 64);
+        s_excl[w][lane] = incl - big;
+        s_done[w][lane] = 0u;
+        wave_sync();
+        for (int base = 0; base < total; base += 64) {
+            const int cc = base + lane;
+            bool pass = false;
+            int o = 0, l = 0;
+            uint32_t tile = 0u;
+            if (cc < total) {
+                o = owner_of(s_excl[w], cc);
+                l = cc - s_excl[w][o];
+                const int4 rc = s_rect[w][o];
+                int q, r;
+                divmod_small(l, rc.z, q, r);
+                FrameGeom og;
+                og.tile_clip_x = s_clip[w][o].x;
+                og.tile_clip_y = s_clip[w][o].y;
+                pass = tile_overlap(rc.x + r, rc.y + q, s_ell[w][o], og);
+                tile = (uint32_t)(s_tb[w][o] + (rc.y + q) * rc.w + rc.x + r);
+            }
+            const uint64_t m = __ballot(pass);
+            if (pass) {
+                // lanes of this record in this step start at lane max(0, excl - base)
+                const int lo = max(0, s_excl[w][o] - base);
+                const uint64_t before = m & lanemask_lt(lane) & ~lanemask_lt(lo);
+                put(s_off[w][o] + s_done[w][o] + (uint32_t)__popcll(before), tile, o);
+            }
+            wave_sync();
+            if (cc < total && (l == 0 || lane == 0)) {
+                const int len = min(s_excl[w][o] + s_cand[w][o], base + 64) - cc;
+                s_done[w][o] += (uint32_t)__popcll((m >> lane) & low_bits(len));
+            }
+            wave_sync();
+        }
+        wave_sync();
+    }
+    if (hist) {
+        __syncthreads();
+        for (int k = 0; k < a.tile_passes; ++k) {
+            const uint32_t v = s_hist[k][threadIdx.x];
+            if (v) atomicAdd(&a.tile_hist[k * 256 + threadIdx.x], v);
+        }
     }
 }
 
 static dim3 build_grid(const BuildArgs& a) {
     constexpr int W = kBuildThreads / 64;
-    return dim3((a.batches + W - 1) / W, a.records ? 1 : a.nviews);
+    if (a.records) return dim3((unsigned)((a.batches + W - 1) / W));
+    return dim3((a.batches + W - 1) / W, a.nviews);
 }
 
 hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
-    if (a.records)
-        hipLaunchKernelGGL((build_kernel<false, true>), build_grid(a), dim3(kBuildThreads), 0, s, a);
-    else
-        hipLaunchKernelGGL((build_kernel<false, false>), build_grid(a), dim3(kBuildThreads), 0, s, a);
+    if (a.records) {
+        if (!a.record_counts || !a.record_masks) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(record_count_kernel, build_grid(a), dim3(kBuildThreads), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(build_kernel<false>, build_grid(a), dim3(kBuildThreads), 0, s, a);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
-    if (a.records)
-        hipLaunchKernelGGL((build_kernel<true, true>), build_grid(a), dim3(kBuildThreads), 0, s, a);
-    else
-        hipLaunchKernelGGL((build_kernel<true, false>), build_grid(a), dim3(kBuildThreads), 0, s, a);
+    if (a.records) {
+        if (!a.record_counts || !a.record_masks || !a.record_offsets || a.tile_passes > 3) return hipErrorInvalidValue;
+        constexpr int W = kBuildThreads / 64;
+        const unsigned blocks = (unsigned)std::max(1, std::min((a.batches + W - 1) / W, kEmitBlocks));
+        hipLaunchKernelGGL(record_emit_kernel, dim3(blocks), dim3(kBuildThreads), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(build_kernel<true>, build_grid(a), dim3(kBuildThreads), 0, s, a);
+    }
     return hipGetLastError();
+}
+
+static unsigned grid_for(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
 }
 
 // Compaction of the visible Gaussians (non-empty tile rect) of every view into
@@ -602,11 +782,6 @@ __global__ void scatter_offsets_kernel(const uint32_t* scanned, const int32_t* o
         offsets[order[j]] = scanned[j];
 }
 
-static unsigned grid_for(int64_t n) {
-    int64_t b = (n + 255) / 256;
-    return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
-}
-
 hipError_t launch_gather_counts(const uint32_t* counts, const int32_t* order, const uint32_t* valid, uint32_t* tmp,
                                 int32_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
@@ -627,11 +802,6 @@ hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-__global__ void fill_u32_kernel(uint32_t* p, uint32_t value, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = value;
-}
 
 // Exclusive scan of the per-batch counts, in place, in three launches
 // (reduce per 2048-element block, scan of the block sums, local scan + block
@@ -689,7 +859,10 @@ __global__ __launch_bounds__(1024) void scan_sums_kernel(uint32_t* sums, int32_t
     if (threadIdx.x == 0) *total = (uint32_t)(s_carry > 0xffffffffull ? 0xffffffffull : s_carry);
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data, int32_t n, const uint32_t* sums) {
+template <bool SCATTER>
+__global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data, int32_t n, const uint32_t* sums,
+                                                                 const int32_t* order, const uint32_t* valid,
+                                                                 uint32_t* offsets) {
     __shared__ uint32_t s_w[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t my = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
@@ -710,9 +883,14 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data,
     __syncthreads();
     uint32_t run = sums[blockIdx.x] + incl - sum;
     for (int k = 0; k < w; ++k) run += s_w[k];
+    const int64_t nv = SCATTER ? (int64_t)min(*valid, (uint32_t)n) : 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        if (my + k < n) data[my + k] = run;
+        if constexpr (SCATTER) {
+            if (my + k < nv) offsets[order[my + k]] = run;
+        } else {
+            if (my + k < n) data[my + k] = run;
+        }
         run += v[k];
     }
 }
@@ -727,7 +905,22 @@ hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_
     const int nb = (n + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch);
     hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, scratch, nb, total);
-    hipLaunchKernelGGL(scan_down_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch);
+    hipLaunchKernelGGL(scan_down_kernel<false>, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch, nullptr,
+                       nullptr, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_scan_scatter(uint32_t* data, int32_t n, uint32_t* total, uint32_t* scratch,
+                                     const int32_t* order, const uint32_t* valid, uint32_t* offsets, hipStream_t s) {
+    if (n <= 0) {
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, total, 0u, (int64_t)1);
+        return hipGetLastError();
+    }
+    const int nb = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, scratch, nb, total);
+    hipLaunchKernelGGL(scan_down_kernel<true>, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch, order, valid,
+                       offsets);
     return hipGetLastError();
 }
 
@@ -751,19 +944,22 @@ hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t s) 
 
 __global__ __launch_bounds__(256) void ranges_kernel(RangeArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int size = a.size_dev ? (int)min((uint32_t)a.size, *a.size_dev) : a.size;
+    if (a.size_dev && *a.size_dev > (uint32_t)a.size) return;   // the list did not fit: nothing to range
+    const int size = a.size_dev ? (int)*a.size_dev : a.size;
     if (size <= 0) return;
+    // tile ids at or past `tiles` never occur in a well-formed list; they are ignored, not written
+    const uint32_t nt = (uint32_t)a.tiles;
     if (i == 0) {
         const uint32_t first = (uint32_t)(a.keys[0] >> 32);
-        a.tile_range[(size_t)first * 2] = 0;
+        if (first < nt) a.tile_range[(size_t)first * 2] = 0;
         const uint32_t last = (uint32_t)(a.keys[size - 1] >> 32);
-        a.tile_range[(size_t)last * 2 + 1] = size;
+        if (last < nt) a.tile_range[(size_t)last * 2 + 1] = size;
     } else if (i < size) {
         const uint32_t prev = (uint32_t)(a.keys[i - 1] >> 32);
         const uint32_t cur = (uint32_t)(a.keys[i] >> 32);
         if (cur != prev) {
-            a.tile_range[(size_t)prev * 2 + 1] = i;
-            a.tile_range[(size_t)cur * 2] = i;
+            if (prev < nt) a.tile_range[(size_t)prev * 2 + 1] = i;
+            if (cur < nt) a.tile_range[(size_t)cur * 2] = i;
         }
     }
 }

@@ -322,6 +322,19 @@ typedef struct gr_frame_view {
 } gr_frame_view;
 GR_API gr_status gr_renderer_frame_view(gr_renderer* r, gr_frame_view* out);
 
+/* Per-kernel timing of the last frame, from HIP events on the renderer's
+ * stream (no counterpart in the reference; for roofline reporting).  The
+ * tile-bit sort's onesweep passes run back to back between two events:
+ * tile_sort_pass_ms is their total, tile_sort_passes their number and
+ * tile_list_size the pairs each pass moves. */
+typedef struct gr_kernel_times {
+    double tile_sort_pass_ms;
+    int32_t tile_sort_passes;
+    int32_t pad;
+    int64_t tile_list_size;
+} gr_kernel_times;
+GR_API gr_status gr_renderer_kernel_times(gr_renderer* r, gr_kernel_times* out);
+
 /* ---- serving: continuous batching over resident scenes -----------------
  * No counterpart in the reference (one scene, one frame at a time,
  * app/main.cpp:326-472).  A server keeps scenes resident in HBM and renders
@@ -373,8 +386,8 @@ typedef struct gr_server_stats {
     int64_t resident_bytes;       /* scenes */
     int64_t workspace_bytes;      /* all slots */
     int64_t peak_bytes;
-    int64_t batches;
-    int64_t views;
+    int64_t batches;              /* batches completed */
+    int64_t views;                /* views rendered in them */
     int64_t graph_launches;
     int64_t budget_waits;         /* times a batch waited for memory */
 } gr_server_stats;

@@ -245,7 +245,10 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
         if (a.records) {
             const uint32_t nvis = std::min<uint32_t>(*a.record_count, (uint32_t)(a.nviews * a.count));
             rd(a.records, (size_t)nvis * 32);
-            for (uint32_t q = 0; q < nvis; ++q) a.record_counts[q] = 3u;
+            for (uint32_t q = 0; q < nvis; ++q) {
+                a.record_counts[q] = 3u;
+                a.record_masks[q] = 0xbull;   // 4 candidates, 3 of them pass
+            }
             return;
         }
         const size_t n = (size_t)a.nviews * a.count;
@@ -262,6 +265,21 @@ hipError_t launch_gather_counts(const uint32_t* counts, const int32_t* order, co
     run_or_record(s, [=] {
         const uint32_t nv = std::min<uint32_t>(*valid, (uint32_t)n);
         for (int32_t j = 0; j < n; ++j) tmp[j] = (uint32_t)j < nv ? counts[order[j]] : 0u;
+    });
+    return hipSuccess;
+}
+
+hipError_t launch_build_scan_scatter(uint32_t* data, int32_t n, uint32_t* total, uint32_t* scratch,
+                                     const int32_t* order, const uint32_t* valid, uint32_t* offsets, hipStream_t s) {
+    run_or_record(s, [=] {
+        const uint32_t nv = std::min<uint32_t>(*valid, (uint32_t)std::max(n, 0));
+        uint32_t acc = 0;
+        for (int i = 0; i < n; ++i) {
+            if ((uint32_t)i < nv) offsets[order[i]] = acc;
+            acc += data[i];
+        }
+        *total = acc;
+        touch(scratch, (size_t)scan_scratch_words(n) * 4);
     });
     return hipSuccess;
 }
@@ -299,8 +317,11 @@ hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
             K = 0;
             for (uint32_t q = 0; q < nvis; ++q) {
                 if (a.record_offsets[q] > 3u * nvis) std::abort();   // offsets lie inside the list
+                if (__builtin_popcountll(a.record_masks[q]) != (int)a.record_counts[q]) std::abort();
                 K += a.record_counts[q];
             }
+            if (a.tile_hist)
+                for (int k = 0; k < a.tile_passes; ++k) a.tile_hist[k * 256] += std::min<uint32_t>(K, (uint32_t)a.capacity);
         } else {
             K = a.batch_counts[a.nviews * a.batches];
         }
@@ -346,6 +367,15 @@ template <typename KeyT>
 hipError_t mock_sort(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
                      int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes, hipStream_t s);
 
+hipError_t radix_sort_prepare(int32_t count, void* temp, size_t temp_bytes, uint32_t** hist, hipStream_t s) {
+    if (count < 1) count = 1;
+    if (temp_bytes < (size_t)count * 4 + 4096 || !temp) return hipErrorInvalidValue;
+    *hist = static_cast<uint32_t*>(temp);
+    const size_t zero = std::min<size_t>(temp_bytes, 8 * 256 * 4);
+    run_or_record(s, [=] { std::memset(temp, 0, zero); });
+    return hipSuccess;
+}
+
 size_t radix_sort_temp_bytes(int32_t count, int32_t, int32_t end_bit) {
     return (size_t)count * 4 + 4096 + (size_t)end_bit;
 }
@@ -353,6 +383,12 @@ size_t radix_sort_temp_bytes(int32_t count, int32_t, int32_t end_bit) {
 hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                             const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                             size_t temp_bytes, hipStream_t s) {
+    return mock_sort(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
+}
+
+hipError_t radix_sort_pairs_prepared(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                                     const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                                     size_t temp_bytes, hipStream_t s, hipEvent_t, hipEvent_t) {
     return mock_sort(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
 }
 
