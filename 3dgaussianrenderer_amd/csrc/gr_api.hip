@@ -340,7 +340,7 @@ gr_status gr_evaluate_tile_range(gr_context* ctx, gr_timer* timer, int32_t size)
     GR_CHECK(timer_start(ctx, timer));
     GR_HIP(launch_fill_i32(ctx->args.tile_range, -1, 2 * (int64_t)tiles, ctx->stream));
     if (size > 0) {
-        RangeArgs a;
+        RangeArgs a{};
         a.keys = ctx->tl.keys;
         a.tile_range = ctx->args.tile_range;
         a.size_dev = nullptr;

@@ -107,7 +107,8 @@ struct BuildArgs {
     const float* clip_z;
     const float4* ellipse;
     uint32_t* batch_counts;      // [nviews][batches]: counts (count pass) / exclusive offsets (emit pass)
-    uint64_t* keys;
+    uint64_t* keys;              // index-order walk: (tile << 32 | depth word)
+    uint32_t* tile_keys;         // record walk: tile id (the entries arrive in depth order)
     int32_t* values;             // Gaussian index within the scene
     const float4* records;       // record walk (NULL: the per-view index-order walk over clip_xy/ellipse)
     const uint32_t* record_count;// number of valid records (device)
@@ -125,7 +126,8 @@ struct BuildArgs {
 };
 
 struct RangeArgs {
-    const uint64_t* keys;
+    const uint64_t* keys;        // (tile << 32 | depth word) keys, or
+    const uint32_t* tile_keys;   // tile ids (when keys is NULL)
     int32_t* tile_range;
     const uint32_t* size_dev;    // device-side entry count (NULL: use `size`)
     int32_t size;                // entries, or the upper bound when size_dev is set
@@ -151,7 +153,10 @@ struct RasterArgs {
 constexpr int kBuildBatch = 64;
 constexpr int kBuildThreads = 256;
 // Workgroups of the record emit pass (its waves take batches grid-stride).
-constexpr int kEmitBlocks = 2048;
+#ifndef GR_EMIT_BLOCKS
+#define GR_EMIT_BLOCKS 2048
+#endif
+constexpr int kEmitBlocks = GR_EMIT_BLOCKS;
 
 // launchers (all asynchronous on `stream`)
 hipError_t launch_sh(const ShArgs& a, int degree, int precision, hipStream_t stream);
@@ -171,9 +176,9 @@ hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t stream);
 // tmp[j] = counts[order[j]] for j < *valid, 0 for valid <= j < n
 hipError_t launch_gather_counts(const uint32_t* counts, const int32_t* order, const uint32_t* valid, uint32_t* tmp,
                                 int32_t n, hipStream_t stream);
-// offsets[order[j]] = scanned[j] for j < *valid
-hipError_t launch_scatter_offsets(const uint32_t* scanned, const int32_t* order, const uint32_t* valid,
-                                  uint32_t* offsets, int32_t n, hipStream_t stream);
+// keys[i] = tile_keys[i] << 32 | depth word of clip_z[values[i]] (i < n)
+hipError_t launch_expand_keys(const uint32_t* tile_keys, const int32_t* values, const float* clip_z, uint64_t* keys,
+                              int64_t n, hipStream_t stream);
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t stream);
 hipError_t launch_ranges(const RangeArgs& a, hipStream_t stream);
 hipError_t launch_raster(const RasterArgs& a, hipStream_t stream);
@@ -201,6 +206,11 @@ hipError_t radix_sort_pairs_prepared(uint64_t* keys[2], int32_t* values[2], int3
                                      const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                                      size_t temp_bytes, hipStream_t stream, hipEvent_t passes_begin = nullptr,
                                      hipEvent_t passes_end = nullptr);
+// The prepared form for 32-bit keys (tile ids of the renderer's tile list).
+hipError_t radix_sort_pairs_prepared_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                                         const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                                         size_t temp_bytes, hipStream_t stream, hipEvent_t passes_begin = nullptr,
+                                         hipEvent_t passes_end = nullptr);
 // The same for 32-bit keys (depth words); same temp layout and size.
 hipError_t radix_sort_pairs_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                                 const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,

@@ -10,7 +10,8 @@
 //   -> 32-bit onesweep sort of the pairs by depth word -> the counts taken
 //      in depth order are scanned (K) and scattered back as per-record
 //      output offsets
-//   -> emit: each record's entries at its offset (+ tile digit histograms)
+//   -> emit (records in index order): each record's entries at its offset,
+//      + tile digit histograms
 //   -> onesweep sort of the tile bits only (bits [32, 32 + tile bits))
 //   -> ranges -> raster -> image and K readback.
 // Entries sit in (depth word, Gaussian index) order before the last sort, so
@@ -98,12 +99,13 @@ struct gr_renderer {
     DevBuf<float2> clip_xy;
     DevBuf<float> clip_z, sh;
     DevBuf<int32_t> tile_range;
-    DevBuf<uint64_t> keys[2];
+    DevBuf<uint32_t> keys[2];       // tile ids of the tile list (double buffer of the tile sort)
+    DevBuf<uint64_t> full_keys;     // the list's (tile << 32 | depth word) keys, made on inspection
     DevBuf<int32_t> values[2];
     DevBuf<uchar4> out;
     DevBuf<uint64_t> vis_bits;      // per 64 Gaussians: non-empty tile rect ballot
     DevBuf<uint32_t> vis_counts;    // popcounts -> exclusive offsets, total, scan scratch
-    DevBuf<uint32_t> batch_counts;  // record counts in depth order -> offsets, total, scan scratch
+    DevBuf<uint32_t> batch_counts;  // record counts in depth order -> offsets, total K, scan scratch
     DevBuf<uint32_t> rec_counts;    // entries per build record
     DevBuf<uint32_t> rec_offsets;   // output offset per build record
     DevBuf<uint64_t> rec_masks;     // passing candidates 0..63 per build record
@@ -142,7 +144,7 @@ gr_status gr_renderer_destroy(gr_renderer* r) {
     r->position.release(); r->scale_rot.release(); r->color.release(); r->conic.release();
     r->ellipse.release(); r->clip_xy.release(); r->clip_z.release(); r->sh.release();
     r->packed.release(); r->sh16.release();
-    r->tile_range.release(); r->keys[0].release(); r->keys[1].release();
+    r->tile_range.release(); r->keys[0].release(); r->keys[1].release(); r->full_keys.release();
     r->values[0].release(); r->values[1].release(); r->out.release();
     r->vis_bits.release(); r->vis_counts.release(); r->batch_counts.release();
     r->dkeys[0].release(); r->dkeys[1].release(); r->dorder[0].release(); r->dorder[1].release();
@@ -265,7 +267,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     // one scratch serves the depth sort and the tile-bit sort; sized before anything is enqueued
     const int32_t capacity = (int32_t)std::min<size_t>(std::min(r->keys[0].cap, r->values[0].cap), (1u << 30) - 1);
     R_CHECK(ensure_temp(r->sort_temp, r->sort_temp_bytes,
-                        std::max(radix_sort_temp_bytes((int32_t)nn, 0, 32), radix_sort_temp_bytes(capacity, 32, 64))));
+                        std::max(radix_sort_temp_bytes((int32_t)nn, 0, 32), radix_sort_temp_bytes(capacity, 0, 32))));
 
     ViewDesc vd{};
     vd.cam = camera_args(*cam);
@@ -332,8 +334,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     b.record_count = vis_total;
     b.record_counts = r->rec_counts.p;
     b.record_masks = r->rec_masks.p;
-    b.record_offsets = r->rec_offsets.p;
-    b.keys = r->keys[0].p;
+    b.tile_keys = r->keys[0].p;
     b.values = r->values[0].p;
     b.count = n;
     b.capacity = capacity;
@@ -354,6 +355,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     R_HIP(launch_gather_counts(r->rec_counts.p, r->dorder[dsel].p, vis_total, r->batch_counts.p, n, stream));
     R_HIP(launch_build_scan_scatter(r->batch_counts.p, n, k_total, k_total + 1, r->dorder[dsel].p, vis_total,
                                     r->rec_offsets.p, stream));
+    b.record_offsets = r->rec_offsets.p;
     const int tile_bits = tile_bits_for(tiles);
     R_HIP(radix_sort_prepare(capacity, r->sort_temp, r->sort_temp_bytes, &b.tile_hist, stream));
     b.tile_passes = (tile_bits + 7) / 8;
@@ -362,10 +364,10 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     // tile bits
     int32_t sel = 0;
     {
-        uint64_t* kb[2] = {r->keys[0].p, r->keys[1].p};
+        uint32_t* kb[2] = {r->keys[0].p, r->keys[1].p};
         int32_t* vb[2] = {r->values[0].p, r->values[1].p};
-        R_HIP(radix_sort_pairs_prepared(kb, vb, &sel, capacity, k_total, 32, 32 + tile_bits, r->sort_temp,
-                                        r->sort_temp_bytes, stream, r->pass_ev[0], r->pass_ev[1]));
+        R_HIP(radix_sort_pairs_prepared_u32(kb, vb, &sel, capacity, k_total, 0, tile_bits, r->sort_temp,
+                                            r->sort_temp_bytes, stream, r->pass_ev[0], r->pass_ev[1]));
         r->last_passes = capacity > 1 ? (tile_bits + 7) / 8 : 0;
     }
     R_HIP(hipEventRecord(r->ev[6], stream));
@@ -373,7 +375,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     R_HIP(launch_fill_i32(r->tile_range.p, -1, 2 * (int64_t)tiles, stream));
     {
         RangeArgs ra{};
-        ra.keys = r->keys[sel].p;
+        ra.tile_keys = r->keys[sel].p;
         ra.tile_range = r->tile_range.p;
         ra.size_dev = k_total;
         ra.size = capacity;
@@ -452,7 +454,16 @@ gr_status gr_renderer_frame_view(gr_renderer* r, gr_frame_view* v) {
     v->position_clip_z = r->clip_z.p;
     v->screen_ellipse = reinterpret_cast<const float*>(r->ellipse.p);
     v->conic = reinterpret_cast<const float*>(r->conic.p);
-    v->sorted_keys = r->keys[r->sel].p;
+    // the list is held as tile ids in (tile, depth word, index) order; its reference-form
+    // keys are made here, once per frame inspected
+    const int64_t k = r->last_k;
+    if (k > 0) {
+        R_CHECK(r->full_keys.ensure((size_t)k));
+        R_HIP(launch_expand_keys(r->keys[r->sel].p, r->values[r->sel].p, r->clip_z.p, r->full_keys.p, k,
+                                 r->ctx->stream));
+        R_HIP(hipStreamSynchronize(r->ctx->stream));
+    }
+    v->sorted_keys = k > 0 ? r->full_keys.p : nullptr;
     v->sorted_values = r->values[r->sel].p;
     v->tile_range = r->tile_range.p;
     v->tile_list_size = r->last_k;

@@ -144,10 +144,10 @@ struct Workspace {
     Dev dkeys0, dkeys1, dorder0, dorder1;   // [<= V*N] depth words and record indices of visible Gaussians
     Dev records;                      // [<= V*N][2] float4 build records
     Dev dsort_temp;                   // depth sort scratch
-    Dev counts;                       // record counts in depth order -> offsets [V*N] + total + scratch
+    Dev counts;                       // record counts in depth order -> offsets [V*N] + K + scratch
     Dev rec_counts, rec_offsets;      // [V*N] entries and output offset per record
     Dev rec_masks;                    // [V*N] passing candidates 0..63 per record
-    Dev keys0, keys1, vals0, vals1;   // tile lists
+    Dev keys0, keys1, vals0, vals1;   // tile lists: tile ids (uint32) and Gaussian indices
     Dev sort_temp;
     Dev ranges;                       // 2 * tiles
     Dev images;                       // sum of W*H*4
@@ -177,7 +177,7 @@ struct Slot {
     std::vector<Pending> batch;
     int32_t scene = -1;
     int32_t tiles_total = 0;
-    int32_t end_bit = 32;
+    int32_t tile_bits = 1;            // bits of the batch's global tile ids (the tile sort's key)
     int64_t img_bytes = 0;
     uint32_t k = 0;
     uint32_t capacity = 0;
@@ -299,9 +299,9 @@ int64_t fresh_workspace_bytes(Slot& sl, const Scene& sc, size_t entries) {
     int64_t b = 0;
     for (const PhaseANeed& n : phase_a_needs(sl, sc, sl.batch.size(), sl.img_bytes, sl.tiles_total))
         b += fresh_bytes(n.bytes);
-    const int64_t cap = std::min<int64_t>(fresh_bytes((int64_t)entries * 8) / 8, fresh_bytes((int64_t)entries * 4) / 4);
-    b += 2 * fresh_bytes((int64_t)entries * 8) + 2 * fresh_bytes((int64_t)entries * 4);
-    b += fresh_bytes((int64_t)radix_sort_temp_bytes((int32_t)std::min<int64_t>(cap, (1 << 30) - 1), 0, sl.end_bit));
+    const int64_t cap = fresh_bytes((int64_t)entries * 4) / 4;
+    b += 4 * fresh_bytes((int64_t)entries * 4);
+    b += fresh_bytes((int64_t)radix_sort_temp_bytes((int32_t)std::min<int64_t>(cap, (1 << 30) - 1), 0, sl.tile_bits));
     return b;
 }
 
@@ -341,7 +341,7 @@ BuildArgs record_build_args(Slot& sl, const Scene& sc) {
     const int N = sc.n;
     const int nbv = V * ((N + kBuildBatch - 1) / kBuildBatch);
     BuildArgs b{};
-    b.keys = w.keys0.as<uint64_t>();
+    b.tile_keys = w.keys0.as<uint32_t>();
     b.values = w.vals0.as<int32_t>();
     b.records = w.records.as<float4>();
     b.record_count = w.vis_counts.as<uint32_t>() + nbv;
@@ -444,16 +444,16 @@ gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
     const int nb = V * N;   // the device-side K sits after the V*N scanned counts
     BuildArgs b = record_build_args(sl, sc);
     S_HIP(radix_sort_prepare((int32_t)sl.capacity, w.sort_temp.p, w.sort_temp.bytes, &b.tile_hist, sl.stream));
-    b.tile_passes = (sl.end_bit - 32 + 7) / 8;
+    b.tile_passes = (sl.tile_bits + 7) / 8;
     S_HIP(launch_build_emit(b, sl.stream));
-    uint64_t* kb[2] = {w.keys0.as<uint64_t>(), w.keys1.as<uint64_t>()};
+    uint32_t* kb[2] = {w.keys0.as<uint32_t>(), w.keys1.as<uint32_t>()};
     int32_t* vb[2] = {w.vals0.as<int32_t>(), w.vals1.as<int32_t>()};
     int32_t sel = 0;
-    S_HIP(radix_sort_pairs_prepared(kb, vb, &sel, (int32_t)sl.capacity, counts + nb, 32, sl.end_bit, w.sort_temp.p,
-                                    w.sort_temp.bytes, sl.stream));
+    S_HIP(radix_sort_pairs_prepared_u32(kb, vb, &sel, (int32_t)sl.capacity, counts + nb, 0, sl.tile_bits,
+                                        w.sort_temp.p, w.sort_temp.bytes, sl.stream));
     S_HIP(launch_fill_i32(w.ranges.as<int32_t>(), -1, 2 * (int64_t)sl.tiles_total, sl.stream));
     RangeArgs r{};
-    r.keys = kb[sel];
+    r.tile_keys = kb[sel];
     r.tile_range = w.ranges.as<int32_t>();
     r.size_dev = counts + nb;
     r.size = (int32_t)sl.capacity;
@@ -617,7 +617,7 @@ bool start_batch(gr_server* s, Slot& sl) {
     }
     sl.tiles_total = tiles;
     sl.img_bytes = img;
-    sl.end_bit = 32 + tile_bits_for(tiles);
+    sl.tile_bits = tile_bits_for(tiles);
     const int64_t growth = phase_a_growth(sl, *sc, sl.batch.size(), img, tiles);
     if (s->over_budget(growth)) {
         if (other_slots_busy(s, sl)) {
@@ -648,7 +648,7 @@ bool start_batch(gr_server* s, Slot& sl) {
             }
             sl.tiles_total = tiles;
             sl.img_bytes = img;
-            sl.end_bit = 32 + tile_bits_for(tiles);
+            sl.tile_bits = tile_bits_for(tiles);
         }
         if (s->over_budget(phase_a_growth(sl, *sc, sl.batch.size(), img, tiles))) {
             gr_detail::set_last_error("request does not fit the HBM budget");
@@ -697,11 +697,11 @@ bool start_phase_b(gr_server* s, Slot& sl) {
     // bytes each tile-list buffer holds after ensure(), and the net growth for `need` entries
     auto after = [](const Dev& d, size_t want) { return d.p && d.bytes >= want ? d.bytes : std::max<size_t>(want + want / 4, 256); };
     auto growth_for = [&](size_t need) {
-        const size_t cap_after = std::min(std::min(after(w.keys0, need * 8), after(w.keys1, need * 8)) / 8,
+        const size_t cap_after = std::min(std::min(after(w.keys0, need * 4), after(w.keys1, need * 4)) / 4,
                                           std::min(after(w.vals0, need * 4), after(w.vals1, need * 4)) / 4);
         const size_t sort_after =
-            radix_sort_temp_bytes((int32_t)std::min<size_t>(cap_after, (1u << 30) - 1), 0, sl.end_bit);
-        return w.keys0.growth(need * 8) + w.keys1.growth(need * 8) + w.vals0.growth(need * 4) +
+            radix_sort_temp_bytes((int32_t)std::min<size_t>(cap_after, (1u << 30) - 1), 0, sl.tile_bits);
+        return w.keys0.growth(need * 4) + w.keys1.growth(need * 4) + w.vals0.growth(need * 4) +
                w.vals1.growth(need * 4) + w.sort_temp.growth(sort_after);
     };
     size_t need = std::max<size_t>(sl.k, 1);
@@ -749,8 +749,8 @@ bool start_phase_b(gr_server* s, Slot& sl) {
         }
     }
     bool grew = false;
-    if (w.keys0.ensure(need * 8, s->workspace, &grew) != GR_OK ||
-        w.keys1.ensure(need * 8, s->workspace, &grew) != GR_OK ||
+    if (w.keys0.ensure(need * 4, s->workspace, &grew) != GR_OK ||
+        w.keys1.ensure(need * 4, s->workspace, &grew) != GR_OK ||
         w.vals0.ensure(need * 4, s->workspace, &grew) != GR_OK ||
         w.vals1.ensure(need * 4, s->workspace, &grew) != GR_OK) {
         complete(s, sl, GR_ERR_OUT_OF_MEMORY);
@@ -758,10 +758,10 @@ bool start_phase_b(gr_server* s, Slot& sl) {
         sl.state = SlotState::Idle;
         return true;
     }
-    sl.capacity = (uint32_t)std::min(std::min(w.keys0.bytes, w.keys1.bytes) / 8,
+    sl.capacity = (uint32_t)std::min(std::min(w.keys0.bytes, w.keys1.bytes) / 4,
                                      std::min(w.vals0.bytes, w.vals1.bytes) / 4);
     sl.capacity = std::min<uint32_t>(sl.capacity, (1u << 30) - 1);
-    const size_t sort_bytes = radix_sort_temp_bytes((int32_t)sl.capacity, 0, sl.end_bit);
+    const size_t sort_bytes = radix_sort_temp_bytes((int32_t)sl.capacity, 0, sl.tile_bits);
     if (w.sort_temp.ensure(sort_bytes, s->workspace, &grew) != GR_OK) {
         complete(s, sl, GR_ERR_OUT_OF_MEMORY);
         sl.batch.clear();

@@ -31,7 +31,7 @@ namespace gr {
 #define GR_SORT_THREADS 512
 #endif
 #ifndef GR_SORT_ITEMS
-#define GR_SORT_ITEMS 8
+#define GR_SORT_ITEMS 12
 #endif
 #ifndef GR_SORT_LOOKBACK
 #define GR_SORT_LOOKBACK 16
@@ -403,15 +403,31 @@ hipError_t radix_sort_prepare(int32_t count, void* temp, size_t temp_bytes, uint
     return sort_prepare(l, static_cast<char*>(temp), s);
 }
 
+template <typename KeyT>
+static hipError_t sort_prepared_t(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                                  const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                                  size_t temp_bytes, hipStream_t s, hipEvent_t passes_begin, hipEvent_t passes_end) {
+    if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
+    hipError_t e = sort_check<KeyT>(count, begin_bit, end_bit, temp, temp_bytes);
+    if (e != hipSuccess) return e;
+    return sort_passes_t<KeyT>(keys, values, sel, count, count_dev, begin_bit, end_bit, static_cast<char*>(temp), 1,
+                               s, passes_begin, passes_end);
+}
+
 hipError_t radix_sort_pairs_prepared(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                                      const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                                      size_t temp_bytes, hipStream_t s, hipEvent_t passes_begin,
                                      hipEvent_t passes_end) {
-    if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
-    hipError_t e = sort_check<uint64_t>(count, begin_bit, end_bit, temp, temp_bytes);
-    if (e != hipSuccess) return e;
-    return sort_passes_t<uint64_t>(keys, values, sel, count, count_dev, begin_bit, end_bit, static_cast<char*>(temp),
-                                   1, s, passes_begin, passes_end);
+    return sort_prepared_t<uint64_t>(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s,
+                                     passes_begin, passes_end);
+}
+
+hipError_t radix_sort_pairs_prepared_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                                         const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                                         size_t temp_bytes, hipStream_t s, hipEvent_t passes_begin,
+                                         hipEvent_t passes_end) {
+    return sort_prepared_t<uint32_t>(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s,
+                                     passes_begin, passes_end);
 }
 
 }  // namespace gr

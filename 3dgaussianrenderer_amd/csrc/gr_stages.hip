@@ -580,21 +580,20 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
     }
 }
 
-// Record walk, emit pass: the same 64-record batches, each wave taking
-// batches grid-stride (one tile-digit histogram flush per workgroup).  Record
-// o's entries go to record_offsets[o] + 0, 1, ... in candidate order.  For
-// footprints of at most 64 tiles the wave walks the entries themselves, 64
-// per step: entry k of record o is candidate select_bit(mask_o, k), so
-// consecutive lanes write consecutive positions of a record.  Larger
-// footprints are walked candidate by candidate with the exact test.  Nothing
-// at or past `capacity` is written (the list is then rendered again with room).
+// Record walk, emit pass: the same 64-record batches (index order), each
+// wave taking batches grid-stride (one tile-digit histogram flush per
+// workgroup).  Record o's entries go to record_offsets[o] + 0, 1, ... in
+// candidate order, as (tile id, Gaussian index) pairs: the offsets follow
+// depth order, so the depth word of the key is carried by the position and
+// the stable tile sort that follows needs the tile id alone.  (Walking the
+// records in depth order instead would make the stores contiguous but turn
+// the record loads into gathers, which costs more.)
 __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a) {
     constexpr int W = kBuildThreads / 64;
     __shared__ int s_excl[W][64];
     __shared__ int4 s_rect[W][64];       // x0, y0, nx, tiles_x
     __shared__ uint64_t s_mask[W][64];
     __shared__ uint32_t s_off[W][64];
-    __shared__ uint32_t s_dw[W][64];
     __shared__ int s_gi[W][64];
     __shared__ int s_tb[W][64];          // tile base of the record's view
     __shared__ EllipseRec s_ell[W][64];  // large footprints only
@@ -611,7 +610,7 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
     const uint32_t cap = (uint32_t)a.capacity;
     auto put = [&](uint32_t p, uint32_t tile, int o) {
         if (p < cap) {
-            a.keys[p] = ((uint64_t)tile << 32) | s_dw[w][o];
+            a.tile_keys[p] = tile;
             a.values[p] = s_gi[w][o];
             if (hist)
                 for (int k = 0; k < a.tile_passes; ++k) atomicAdd(&s_hist[k][(tile >> (8 * k)) & 255u], 1u);
@@ -628,11 +627,11 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
             const ViewDesc& vd = view_of(view, a.v0, a.views);
             const Footprint f = footprint(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, vd.geom);
             const int cand = (f.nx > 0 && f.ny > 0) ? f.nx * f.ny : 0;
+            const uint32_t off = a.record_offsets[pos];
             s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, vd.geom.tiles_x);
             s_tb[w][lane] = vd.tile_base;
             s_mask[w][lane] = a.record_masks[pos];
-            s_off[w][lane] = a.record_offsets[pos];
-            s_dw[w][lane] = __float_as_uint(r1.z);
+            s_off[w][lane] = off;
             s_gi[w][lane] = pg - view * a.count;
             if (cand <= 64) {
                 ecnt = (int)a.record_counts[pos];
@@ -729,7 +728,8 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
     if (a.records) {
-        if (!a.record_counts || !a.record_masks || !a.record_offsets || a.tile_passes > 3) return hipErrorInvalidValue;
+        if (!a.record_counts || !a.record_masks || !a.record_offsets || !a.tile_keys || a.tile_passes > 3)
+            return hipErrorInvalidValue;
         constexpr int W = kBuildThreads / 64;
         const unsigned blocks = (unsigned)std::max(1, std::min((a.batches + W - 1) / W, kEmitBlocks));
         hipLaunchKernelGGL(record_emit_kernel, dim3(blocks), dim3(kBuildThreads), 0, s, a);
@@ -775,24 +775,10 @@ __global__ void gather_counts_kernel(const uint32_t* counts, const int32_t* orde
         tmp[j] = (uint32_t)j < nv ? counts[order[j]] : 0u;
 }
 
-__global__ void scatter_offsets_kernel(const uint32_t* scanned, const int32_t* order, const uint32_t* valid,
-                                       uint32_t* offsets, int32_t n) {
-    const uint32_t nv = min(*valid, (uint32_t)n);
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nv; j += (int64_t)gridDim.x * blockDim.x)
-        offsets[order[j]] = scanned[j];
-}
-
 hipError_t launch_gather_counts(const uint32_t* counts, const int32_t* order, const uint32_t* valid, uint32_t* tmp,
                                 int32_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(gather_counts_kernel, dim3(grid_for(n)), dim3(256), 0, s, counts, order, valid, tmp, n);
-    return hipGetLastError();
-}
-
-hipError_t launch_scatter_offsets(const uint32_t* scanned, const int32_t* order, const uint32_t* valid,
-                                  uint32_t* offsets, int32_t n, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(scatter_offsets_kernel, dim3(grid_for(n)), dim3(256), 0, s, scanned, order, valid, offsets, n);
     return hipGetLastError();
 }
 
@@ -859,6 +845,7 @@ __global__ __launch_bounds__(1024) void scan_sums_kernel(uint32_t* sums, int32_t
     if (threadIdx.x == 0) *total = (uint32_t)(s_carry > 0xffffffffull ? 0xffffffffull : s_carry);
 }
 
+// SCATTER: the exclusive prefix of element j goes to offsets[order[j]] (j < *valid) instead of data[j]
 template <bool SCATTER>
 __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data, int32_t n, const uint32_t* sums,
                                                                  const int32_t* order, const uint32_t* valid,
@@ -924,6 +911,21 @@ hipError_t launch_build_scan_scatter(uint32_t* data, int32_t n, uint32_t* total,
     return hipGetLastError();
 }
 
+// The reference's (tile << 32 | depth word) keys of a tile list held as tile
+// ids: the depth word of entry i is that of Gaussian values[i] in view 0.
+__global__ void expand_keys_kernel(const uint32_t* tile_keys, const int32_t* values, const float* clip_z,
+                                   uint64_t* keys, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        keys[i] = ((uint64_t)tile_keys[i] << 32) | depth_word(clip_z[values[i]]);
+}
+
+hipError_t launch_expand_keys(const uint32_t* tile_keys, const int32_t* values, const float* clip_z, uint64_t* keys,
+                              int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(expand_keys_kernel, dim3(grid_for(n)), dim3(256), 0, s, tile_keys, values, clip_z, keys, n);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Tile ranges: core/gaussian_render.cu:634-660; the -1 pre-fill is
 // app/main.cpp:365.
@@ -942,6 +944,10 @@ hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t s) 
     return hipGetLastError();
 }
 
+__device__ __forceinline__ uint32_t range_tile(const RangeArgs& a, int i) {
+    return a.keys ? (uint32_t)(a.keys[i] >> 32) : a.tile_keys[i];
+}
+
 __global__ __launch_bounds__(256) void ranges_kernel(RangeArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (a.size_dev && *a.size_dev > (uint32_t)a.size) return;   // the list did not fit: nothing to range
@@ -950,13 +956,13 @@ __global__ __launch_bounds__(256) void ranges_kernel(RangeArgs a) {
     // tile ids at or past `tiles` never occur in a well-formed list; they are ignored, not written
     const uint32_t nt = (uint32_t)a.tiles;
     if (i == 0) {
-        const uint32_t first = (uint32_t)(a.keys[0] >> 32);
+        const uint32_t first = range_tile(a, 0);
         if (first < nt) a.tile_range[(size_t)first * 2] = 0;
-        const uint32_t last = (uint32_t)(a.keys[size - 1] >> 32);
+        const uint32_t last = range_tile(a, size - 1);
         if (last < nt) a.tile_range[(size_t)last * 2 + 1] = size;
     } else if (i < size) {
-        const uint32_t prev = (uint32_t)(a.keys[i - 1] >> 32);
-        const uint32_t cur = (uint32_t)(a.keys[i] >> 32);
+        const uint32_t prev = range_tile(a, i - 1);
+        const uint32_t cur = range_tile(a, i);
         if (cur != prev) {
             if (prev < nt) a.tile_range[(size_t)prev * 2 + 1] = i;
             if (cur < nt) a.tile_range[(size_t)cur * 2] = i;
@@ -966,6 +972,7 @@ __global__ __launch_bounds__(256) void ranges_kernel(RangeArgs a) {
 
 hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
     if (a.size <= 0) return hipSuccess;
+    if (!a.keys && !a.tile_keys) return hipErrorInvalidValue;
     hipLaunchKernelGGL(ranges_kernel, dim3((a.size + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }

@@ -10,6 +10,16 @@ HBM and the image left in HBM.
 Output: one JSON line (the driver's contract) with the per-stage breakdown, a
 roofline object for the dominant kernel and the CPU oracle timed on a bounded
 sample of the same workload.
+
+Roofline: the dominant kernel is one pass of the tile-bit onesweep sort (the
+largest single-kernel time of the frame).  The renderer's tile list holds
+(tile id, Gaussian index) pairs of 4 + 4 bytes (the entries are in depth
+order before the tile sort, so the depth word needs no storage), and a pass
+reads and writes each pair once: 16 B per tile-list entry, so
+achieved = 16 * K / (average pass duration), the duration measured with HIP
+events around the passes on the renderer's stream in every timed frame.
+`traffic` is the measured HBM bytes per pass launch, taken from the committed
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated).
 """
 from __future__ import annotations
 
@@ -45,10 +55,35 @@ def parse():
     ap.add_argument("--serving", default="c4", help="serving workload: c4, c5 or none")
     ap.add_argument("--serving-requests", type=int, default=1200)
     ap.add_argument("--serving-rate", type=float, default=400.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU oracle sample length (frames cycle poses)")
+    ap.add_argument("--traffic-profile", default=str(ROOT / "profiles" / "tile_sort_pass_6M_1080p.md"),
+                    help="rocprofv3 summary holding the measured HBM bytes per tile-sort pass")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 launches), to scale "
                          "FETCH_SIZE/WRITE_SIZE counter readings")
     return ap.parse_args()
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def profiled_traffic(path: str):
+    """HBM bytes per tile-sort pass launch from the summary line
+    `traffic_bytes_per_launch tile-sort pass = <bytes>` (None if absent)."""
+    try:
+        for line in Path(path).read_text().splitlines():
+            if line.strip().startswith("traffic_bytes_per_launch tile-sort pass"):
+                return int(float(line.split("=")[1].strip()))
+    except OSError:
+        return None
+    return None
 
 
 def copy_calibration(mib: int):
@@ -124,7 +159,7 @@ def main():
     upload_s = time.perf_counter() - t0
     out = torch.empty((H, W, 4), dtype=torch.uint8, device="cuda")
     stages = {k: [] for k in ("sh_eval_ms", "project_ms", "build_tile_ms", "sort_ms", "raster_ms", "total_ms")}
-    ks = []
+    ks, pass_ms = [], []
     for i in range(args.warmup):
         r.render(cams[i % len(cams)], W, H, out=out.data_ptr())
     torch.cuda.synchronize()
@@ -134,23 +169,34 @@ def main():
         for k in stages:
             stages[k].append(st[k])
         ks.append(st["tile_list_size"])
+        kt = r.kernel_times()
+        pass_ms.append(kt["tile_sort_pass_ms"])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     fps = args.steps / elapsed
     mean = {k: float(np.mean(v)) for k, v in stages.items()}
     K = float(np.mean(ks))
-    passes = (32 + int(np.ceil(np.log2(((W + 15) // 16) * ((H + 15) // 16)))) + 7) // 8
-    # algorithmic bytes per stage (FP32, degree 3)
+    passes = kt["tile_sort_passes"]
+    # algorithmic bytes per stage (FP32, degree 3): SH reads position + 48 coefficients and
+    # writes colour; projection reads position + scale/rotation and writes xy, z, ellipse, conic;
+    # build reads xy/z/ellipse and writes K (tile, index) entries; sort = depth words of the
+    # visible Gaussians (not counted) + the tile-bit passes over K entries
     alg = {
         "sh_eval_ms": 224.0 * N,
         "project_ms": 72.0 * N,
-        "build_tile_ms": 28.0 * N + 12.0 * K,
-        "sort_ms": K * (8.0 + 24.0 * passes),
+        "build_tile_ms": 28.0 * N + 8.0 * K,
+        "sort_ms": K * 16.0 * passes,
     }
     gbs = {k: alg[k] / (mean[k] * 1e-3) / 1e9 for k in alg}
-    hbm_stage = max(alg, key=lambda k: mean[k])
-    roof = {"bound": "hbm", "kernel": hbm_stage.replace("_ms", ""), "achieved": round(gbs[hbm_stage], 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs[hbm_stage] / HBM_PEAK_GBS, 4), "traffic": None}
+    # dominant kernel: one tile-bit onesweep pass; per-entry bytes x the entries of one launch
+    # tile_sort_pass_ms spans all passes of a frame (back to back between two events)
+    pass_avg_ms = float(np.mean(pass_ms)) / passes if passes > 0 else float("nan")
+    achieved = 16.0 * K / (pass_avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": "onesweep_kernel<unsigned int> (tile-bit sort pass)",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.traffic_profile),
+            "algorithmic_bytes_per_launch": int(16.0 * K), "avg_launch_us": round(pass_avg_ms * 1e3, 2),
+            "launches_per_frame": passes}
     serving = None
     if args.serving != "none":
         r.close()
@@ -163,15 +209,22 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         orc.set_threads(threads)
         shal = orc.realign_sh(scene["sh"], N, 48)
+        # bounded sample: whole frames cycling the same poses until --cpu-seconds have passed
+        frames, cpu_s, stage_sum = 0, 0.0, None
+        while frames < 32 and (frames < 2 or cpu_s < args.cpu_seconds):
+            cam = cams[frames % len(cams)]
+            t0 = time.perf_counter()
+            res = orc.render(scene["position"], scene["scale_rot"], shal, None, N, 3, cam.packed(), W, H,
+                             want_list=False)
+            cpu_s += time.perf_counter() - t0
+            stage_sum = res["stage_ms"] if stage_sum is None else stage_sum + res["stage_ms"]
+            frames += 1
+        cpu = {"value": round(frames / cpu_s, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": f"{frames} frames of the same workload (poses 0..{frames - 1}), FP32 oracle, "
+                         f"{threads} threads, {cpu_s:.1f} s",
+               "stage_ms_mean": [round(x / frames, 2) for x in stage_sum.tolist()],
+               "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
         cam = cams[0]
-        t0 = time.perf_counter()
-        res = orc.render(scene["position"], scene["scale_rot"], shal, None, N, 3, cam.packed(), W, H,
-                         want_list=False)
-        cpu_s = time.perf_counter() - t0
-        cpu = {"value": round(1.0 / cpu_s, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": f"1 frame of the same workload (pose 0), FP32 oracle, {threads} threads",
-               "stage_ms": [round(x, 2) for x in res["stage_ms"].tolist()],
-               "cpu_model": platform.processor() or platform.machine()}
         # accuracy gates (the oracle as checker): the benchmarked frame vs the FP64 render,
         # and served FP16-storage views vs the FP64 render of their FP32 scenes
         img, _ = r.render(cam, W, H)

@@ -305,7 +305,12 @@ GR_API gr_status gr_renderer_upload_model(gr_renderer* r, const gr_model_host* m
 GR_API gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* camera, int32_t width,
                                     int32_t height, uint8_t* out_rgba8, int32_t out_is_device,
                                     gr_render_stats* stats);
-/* Device pointers of the last frame's intermediate buffers (for tests). */
+/* Device pointers of the last frame's intermediate buffers (for tests).
+ * sorted_keys/sorted_values is the frame's sorted tile list in the form of
+ * the reference's sortTileList output (core/gaussian_render.cu:591-632):
+ * (tile << 32 | depth word) keys.  The renderer keeps the list as tile ids
+ * (the depth order is fixed before the tile sort), so this call makes the
+ * full keys on the renderer's stream and waits for them. */
 typedef struct gr_frame_view {
     const float* color;
     const float* position_clip_xy;

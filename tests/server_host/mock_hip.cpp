@@ -284,15 +284,6 @@ hipError_t launch_build_scan_scatter(uint32_t* data, int32_t n, uint32_t* total,
     return hipSuccess;
 }
 
-hipError_t launch_scatter_offsets(const uint32_t* scanned, const int32_t* order, const uint32_t* valid,
-                                  uint32_t* offsets, int32_t n, hipStream_t s) {
-    run_or_record(s, [=] {
-        const uint32_t nv = std::min<uint32_t>(*valid, (uint32_t)n);
-        for (uint32_t j = 0; j < nv; ++j) offsets[order[j]] = scanned[j];
-    });
-    return hipSuccess;
-}
-
 int scan_scratch_words(int32_t n) { return (n + 1023) / 1024 + 64; }
 
 hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_t* scratch, hipStream_t s) {
@@ -326,7 +317,8 @@ hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
             K = a.batch_counts[a.nviews * a.batches];
         }
         const uint32_t n = std::min<uint32_t>(K, (uint32_t)a.capacity);
-        touch(a.keys, (size_t)n * 8);
+        if (a.records) touch(a.tile_keys, (size_t)n * 4);
+        else touch(a.keys, (size_t)n * 8);
         touch(a.values, (size_t)n * 4);
     });
     return hipSuccess;
@@ -342,7 +334,8 @@ hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t s) 
 hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
     run_or_record(s, [=] {
         const uint32_t n = a.size_dev ? std::min<uint32_t>(*a.size_dev, (uint32_t)a.size) : (uint32_t)a.size;
-        rd(a.keys, (size_t)n * 8);
+        if (a.keys) rd(a.keys, (size_t)n * 8);
+        else rd(a.tile_keys, (size_t)n * 4);
         touch(a.tile_range, (size_t)a.tiles * 8);
     });
     return hipSuccess;
@@ -389,6 +382,12 @@ hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel,
 hipError_t radix_sort_pairs_prepared(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                                      const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                                      size_t temp_bytes, hipStream_t s, hipEvent_t, hipEvent_t) {
+    return mock_sort(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
+}
+
+hipError_t radix_sort_pairs_prepared_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
+                                         const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
+                                         size_t temp_bytes, hipStream_t s, hipEvent_t, hipEvent_t) {
     return mock_sort(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
 }
 
