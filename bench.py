@@ -174,6 +174,12 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     fps = args.steps / elapsed
+    # the same frames with the image copied to host memory (PCIe-inclusive; not the metric)
+    host_n = min(args.steps, 16)
+    t0 = time.perf_counter()
+    for i in range(host_n):
+        r.render(cams[i % len(cams)], W, H)
+    host_fps = host_n / (time.perf_counter() - t0)
     mean = {k: float(np.mean(v)) for k, v in stages.items()}
     K = float(np.mean(ks))
     passes = kt["tile_sort_passes"]
@@ -265,6 +271,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in mean.items()},
         "stage_gbs": {k.replace("_ms", ""): round(v, 1) for k, v in gbs.items()},
         "tile_list_entries": int(K),
+        "host_output_frames_per_s": round(host_fps, 2),
         "roofline": roof,
         "cpu_baseline": cpu,
         "serving": serving,
