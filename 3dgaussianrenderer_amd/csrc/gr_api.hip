@@ -342,6 +342,7 @@ gr_status gr_evaluate_tile_range(gr_context* ctx, gr_timer* timer, int32_t size)
     if (size > 0) {
         RangeArgs a{};
         a.keys = ctx->tl.keys;
+        a.tile_shift = 32;
         a.tile_range = ctx->args.tile_range;
         a.size_dev = nullptr;
         a.size = size;

@@ -108,7 +108,7 @@ struct BuildArgs {
     const float4* ellipse;
     uint32_t* batch_counts;      // [nviews][batches]: counts (count pass) / exclusive offsets (emit pass)
     uint64_t* keys;              // index-order walk: (tile << 32 | depth word)
-    uint32_t* tile_keys;         // record walk: tile id (the entries arrive in depth order)
+    uint64_t* entries;           // record walk: Gaussian index << 32 | tile id (placed in depth order)
     int32_t* values;             // Gaussian index within the scene
     const float4* records;       // record walk (NULL: the per-view index-order walk over clip_xy/ellipse)
     const uint32_t* record_count;// number of valid records (device)
@@ -126,8 +126,8 @@ struct BuildArgs {
 };
 
 struct RangeArgs {
-    const uint64_t* keys;        // (tile << 32 | depth word) keys, or
-    const uint32_t* tile_keys;   // tile ids (when keys is NULL)
+    const uint64_t* keys;        // tile id in bits [tile_shift, tile_shift + 32)
+    int32_t tile_shift;          // 32: (tile << 32 | depth word) keys; 0: (index << 32 | tile) entries
     int32_t* tile_range;
     const uint32_t* size_dev;    // device-side entry count (NULL: use `size`)
     int32_t size;                // entries, or the upper bound when size_dev is set
@@ -136,7 +136,8 @@ struct RangeArgs {
 
 struct RasterArgs {
     const int32_t* tile_range;
-    const int32_t* values;
+    const int32_t* values;       // Gaussian index per sorted entry, or (when NULL)
+    const uint64_t* entries;     // index << 32 | tile entries
     const float2* clip_xy;       // [nviews][count]
     const float4* conic;
     const float4* color;
@@ -176,8 +177,8 @@ hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t stream);
 // tmp[j] = counts[order[j]] for j < *valid, 0 for valid <= j < n
 hipError_t launch_gather_counts(const uint32_t* counts, const int32_t* order, const uint32_t* valid, uint32_t* tmp,
                                 int32_t n, hipStream_t stream);
-// keys[i] = tile_keys[i] << 32 | depth word of clip_z[values[i]] (i < n)
-hipError_t launch_expand_keys(const uint32_t* tile_keys, const int32_t* values, const float* clip_z, uint64_t* keys,
+// entries (index << 32 | tile) -> keys[i] = tile << 32 | depth word of clip_z[index], values[i] = index
+hipError_t launch_expand_keys(const uint64_t* entries, const float* clip_z, uint64_t* keys, int32_t* values,
                               int64_t n, hipStream_t stream);
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t stream);
 hipError_t launch_ranges(const RangeArgs& a, hipStream_t stream);
@@ -206,7 +207,13 @@ hipError_t radix_sort_pairs_prepared(uint64_t* keys[2], int32_t* values[2], int3
                                      const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                                      size_t temp_bytes, hipStream_t stream, hipEvent_t passes_begin = nullptr,
                                      hipEvent_t passes_end = nullptr);
-// The prepared form for 32-bit keys (tile ids of the renderer's tile list).
+// The prepared form for 64-bit keys without values: the renderer's tile-list
+// entries (Gaussian index << 32 | tile id), sorted on the tile bits.
+hipError_t radix_sort_keys_prepared(uint64_t* keys[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
+                                    int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes,
+                                    hipStream_t stream, hipEvent_t passes_begin = nullptr,
+                                    hipEvent_t passes_end = nullptr);
+// The prepared form for 32-bit keys.
 hipError_t radix_sort_pairs_prepared_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                                          const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                                          size_t temp_bytes, hipStream_t stream, hipEvent_t passes_begin = nullptr,

@@ -99,9 +99,9 @@ struct gr_renderer {
     DevBuf<float2> clip_xy;
     DevBuf<float> clip_z, sh;
     DevBuf<int32_t> tile_range;
-    DevBuf<uint32_t> keys[2];       // tile ids of the tile list (double buffer of the tile sort)
-    DevBuf<uint64_t> full_keys;     // the list's (tile << 32 | depth word) keys, made on inspection
-    DevBuf<int32_t> values[2];
+    DevBuf<uint64_t> entries[2];    // tile list: index << 32 | tile id (double buffer of the tile sort)
+    DevBuf<uint64_t> full_keys;     // the list in the reference's form, made on inspection
+    DevBuf<int32_t> full_values;
     DevBuf<uchar4> out;
     DevBuf<uint64_t> vis_bits;      // per 64 Gaussians: non-empty tile rect ballot
     DevBuf<uint32_t> vis_counts;    // popcounts -> exclusive offsets, total, scan scratch
@@ -144,8 +144,8 @@ gr_status gr_renderer_destroy(gr_renderer* r) {
     r->position.release(); r->scale_rot.release(); r->color.release(); r->conic.release();
     r->ellipse.release(); r->clip_xy.release(); r->clip_z.release(); r->sh.release();
     r->packed.release(); r->sh16.release();
-    r->tile_range.release(); r->keys[0].release(); r->keys[1].release(); r->full_keys.release();
-    r->values[0].release(); r->values[1].release(); r->out.release();
+    r->tile_range.release(); r->entries[0].release(); r->entries[1].release(); r->full_keys.release();
+    r->full_values.release(); r->out.release();
     r->vis_bits.release(); r->vis_counts.release(); r->batch_counts.release();
     r->dkeys[0].release(); r->dkeys[1].release(); r->dorder[0].release(); r->dorder[1].release();
     r->records.release(); r->rec_counts.release(); r->rec_offsets.release(); r->rec_masks.release();
@@ -258,14 +258,13 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     R_CHECK(r->rec_offsets.ensure(nn));
     R_CHECK(r->rec_masks.ensure(nn));
     for (int k = 0; k < 2; ++k) {
-        R_CHECK(r->keys[k].ensure(1));
-        R_CHECK(r->values[k].ensure(1));
+        R_CHECK(r->entries[k].ensure(1));
         R_CHECK(r->dkeys[k].ensure(nn));
         R_CHECK(r->dorder[k].ensure(nn));
     }
     R_CHECK(r->records.ensure(2 * nn));
     // one scratch serves the depth sort and the tile-bit sort; sized before anything is enqueued
-    const int32_t capacity = (int32_t)std::min<size_t>(std::min(r->keys[0].cap, r->values[0].cap), (1u << 30) - 1);
+    const int32_t capacity = (int32_t)std::min<size_t>(std::min(r->entries[0].cap, r->entries[1].cap), (1u << 30) - 1);
     R_CHECK(ensure_temp(r->sort_temp, r->sort_temp_bytes,
                         std::max(radix_sort_temp_bytes((int32_t)nn, 0, 32), radix_sort_temp_bytes(capacity, 0, 32))));
 
@@ -334,8 +333,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     b.record_count = vis_total;
     b.record_counts = r->rec_counts.p;
     b.record_masks = r->rec_masks.p;
-    b.tile_keys = r->keys[0].p;
-    b.values = r->values[0].p;
+    b.entries = r->entries[0].p;
     b.count = n;
     b.capacity = capacity;
     b.batches = batches;
@@ -364,10 +362,9 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     // tile bits
     int32_t sel = 0;
     {
-        uint32_t* kb[2] = {r->keys[0].p, r->keys[1].p};
-        int32_t* vb[2] = {r->values[0].p, r->values[1].p};
-        R_HIP(radix_sort_pairs_prepared_u32(kb, vb, &sel, capacity, k_total, 0, tile_bits, r->sort_temp,
-                                            r->sort_temp_bytes, stream, r->pass_ev[0], r->pass_ev[1]));
+        uint64_t* kb[2] = {r->entries[0].p, r->entries[1].p};
+        R_HIP(radix_sort_keys_prepared(kb, &sel, capacity, k_total, 0, tile_bits, r->sort_temp, r->sort_temp_bytes,
+                                       stream, r->pass_ev[0], r->pass_ev[1]));
         r->last_passes = capacity > 1 ? (tile_bits + 7) / 8 : 0;
     }
     R_HIP(hipEventRecord(r->ev[6], stream));
@@ -375,7 +372,8 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     R_HIP(launch_fill_i32(r->tile_range.p, -1, 2 * (int64_t)tiles, stream));
     {
         RangeArgs ra{};
-        ra.tile_keys = r->keys[sel].p;
+        ra.keys = r->entries[sel].p;
+        ra.tile_shift = 0;
         ra.tile_range = r->tile_range.p;
         ra.size_dev = k_total;
         ra.size = capacity;
@@ -385,7 +383,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     {
         RasterArgs a{};
         a.tile_range = r->tile_range.p;
-        a.values = r->values[sel].p;
+        a.entries = r->entries[sel].p;
         a.clip_xy = r->clip_xy.p;
         a.conic = r->conic.p;
         a.color = r->color.p;
@@ -407,10 +405,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         // the list was too small: nothing was sorted or drawn; grow it and render the frame again
         if (need >= (int64_t)(1 << 30))
             return rfail(GR_ERR_CAPACITY, "tile list of " + std::to_string(need) + " entries exceeds 2^30");
-        for (int k = 0; k < 2; ++k) {
-            R_CHECK(r->keys[k].ensure((size_t)need, 1.25));
-            R_CHECK(r->values[k].ensure((size_t)need, 1.25));
-        }
+        for (int k = 0; k < 2; ++k) R_CHECK(r->entries[k].ensure((size_t)need, 1.25));
         return gr_renderer_render(r, cam, width, height, out_rgba8, out_is_device, stats);
     }
     r->last_k = need;
@@ -454,17 +449,18 @@ gr_status gr_renderer_frame_view(gr_renderer* r, gr_frame_view* v) {
     v->position_clip_z = r->clip_z.p;
     v->screen_ellipse = reinterpret_cast<const float*>(r->ellipse.p);
     v->conic = reinterpret_cast<const float*>(r->conic.p);
-    // the list is held as tile ids in (tile, depth word, index) order; its reference-form
-    // keys are made here, once per frame inspected
+    // the list is held as (index << 32 | tile) entries in (tile, depth word, index) order; its
+    // reference-form keys and values are made here, once per frame inspected
     const int64_t k = r->last_k;
     if (k > 0) {
         R_CHECK(r->full_keys.ensure((size_t)k));
-        R_HIP(launch_expand_keys(r->keys[r->sel].p, r->values[r->sel].p, r->clip_z.p, r->full_keys.p, k,
+        R_CHECK(r->full_values.ensure((size_t)k));
+        R_HIP(launch_expand_keys(r->entries[r->sel].p, r->clip_z.p, r->full_keys.p, r->full_values.p, k,
                                  r->ctx->stream));
         R_HIP(hipStreamSynchronize(r->ctx->stream));
     }
     v->sorted_keys = k > 0 ? r->full_keys.p : nullptr;
-    v->sorted_values = r->values[r->sel].p;
+    v->sorted_values = k > 0 ? r->full_values.p : nullptr;
     v->tile_range = r->tile_range.p;
     v->tile_list_size = r->last_k;
     v->tiles_x = g.tiles_x;

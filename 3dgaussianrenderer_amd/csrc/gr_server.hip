@@ -147,7 +147,7 @@ struct Workspace {
     Dev counts;                       // record counts in depth order -> offsets [V*N] + K + scratch
     Dev rec_counts, rec_offsets;      // [V*N] entries and output offset per record
     Dev rec_masks;                    // [V*N] passing candidates 0..63 per record
-    Dev keys0, keys1, vals0, vals1;   // tile lists: tile ids (uint32) and Gaussian indices
+    Dev list0, list1;                 // tile lists: Gaussian index << 32 | global tile id
     Dev sort_temp;
     Dev ranges;                       // 2 * tiles
     Dev images;                       // sum of W*H*4
@@ -156,7 +156,7 @@ struct Workspace {
     uint64_t generation = 0;          // bumps when any device pointer changes
     std::vector<Dev*> devs() {
         return {&color, &xy, &z, &ell, &con, &vis_bits, &vis_counts, &dkeys0, &dkeys1, &dorder0, &dorder1,
-                &records, &rec_counts, &rec_offsets, &rec_masks, &dsort_temp, &counts, &keys0, &keys1, &vals0, &vals1, &sort_temp, &ranges, &images, &views};
+                &records, &rec_counts, &rec_offsets, &rec_masks, &dsort_temp, &counts, &list0, &list1, &sort_temp, &ranges, &images, &views};
     }
 };
 
@@ -299,8 +299,8 @@ int64_t fresh_workspace_bytes(Slot& sl, const Scene& sc, size_t entries) {
     int64_t b = 0;
     for (const PhaseANeed& n : phase_a_needs(sl, sc, sl.batch.size(), sl.img_bytes, sl.tiles_total))
         b += fresh_bytes(n.bytes);
-    const int64_t cap = fresh_bytes((int64_t)entries * 4) / 4;
-    b += 4 * fresh_bytes((int64_t)entries * 4);
+    const int64_t cap = fresh_bytes((int64_t)entries * 8) / 8;
+    b += 2 * fresh_bytes((int64_t)entries * 8);
     b += fresh_bytes((int64_t)radix_sort_temp_bytes((int32_t)std::min<int64_t>(cap, (1 << 30) - 1), 0, sl.tile_bits));
     return b;
 }
@@ -341,8 +341,7 @@ BuildArgs record_build_args(Slot& sl, const Scene& sc) {
     const int N = sc.n;
     const int nbv = V * ((N + kBuildBatch - 1) / kBuildBatch);
     BuildArgs b{};
-    b.tile_keys = w.keys0.as<uint32_t>();
-    b.values = w.vals0.as<int32_t>();
+    b.entries = w.list0.as<uint64_t>();
     b.records = w.records.as<float4>();
     b.record_count = w.vis_counts.as<uint32_t>() + nbv;
     b.record_counts = w.rec_counts.as<uint32_t>();
@@ -446,14 +445,14 @@ gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
     S_HIP(radix_sort_prepare((int32_t)sl.capacity, w.sort_temp.p, w.sort_temp.bytes, &b.tile_hist, sl.stream));
     b.tile_passes = (sl.tile_bits + 7) / 8;
     S_HIP(launch_build_emit(b, sl.stream));
-    uint32_t* kb[2] = {w.keys0.as<uint32_t>(), w.keys1.as<uint32_t>()};
-    int32_t* vb[2] = {w.vals0.as<int32_t>(), w.vals1.as<int32_t>()};
+    uint64_t* kb[2] = {w.list0.as<uint64_t>(), w.list1.as<uint64_t>()};
     int32_t sel = 0;
-    S_HIP(radix_sort_pairs_prepared_u32(kb, vb, &sel, (int32_t)sl.capacity, counts + nb, 0, sl.tile_bits,
-                                        w.sort_temp.p, w.sort_temp.bytes, sl.stream));
+    S_HIP(radix_sort_keys_prepared(kb, &sel, (int32_t)sl.capacity, counts + nb, 0, sl.tile_bits, w.sort_temp.p,
+                                   w.sort_temp.bytes, sl.stream));
     S_HIP(launch_fill_i32(w.ranges.as<int32_t>(), -1, 2 * (int64_t)sl.tiles_total, sl.stream));
     RangeArgs r{};
-    r.tile_keys = kb[sel];
+    r.keys = kb[sel];
+    r.tile_shift = 0;
     r.tile_range = w.ranges.as<int32_t>();
     r.size_dev = counts + nb;
     r.size = (int32_t)sl.capacity;
@@ -461,7 +460,7 @@ gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
     S_HIP(launch_ranges(r, sl.stream));
     RasterArgs ra{};
     ra.tile_range = w.ranges.as<int32_t>();
-    ra.values = vb[sel];
+    ra.entries = kb[sel];
     ra.clip_xy = w.xy.as<float2>();
     ra.conic = w.con.as<float4>();
     ra.color = w.color.as<float4>();
@@ -697,12 +696,10 @@ bool start_phase_b(gr_server* s, Slot& sl) {
     // bytes each tile-list buffer holds after ensure(), and the net growth for `need` entries
     auto after = [](const Dev& d, size_t want) { return d.p && d.bytes >= want ? d.bytes : std::max<size_t>(want + want / 4, 256); };
     auto growth_for = [&](size_t need) {
-        const size_t cap_after = std::min(std::min(after(w.keys0, need * 4), after(w.keys1, need * 4)) / 4,
-                                          std::min(after(w.vals0, need * 4), after(w.vals1, need * 4)) / 4);
+        const size_t cap_after = std::min(after(w.list0, need * 8), after(w.list1, need * 8)) / 8;
         const size_t sort_after =
             radix_sort_temp_bytes((int32_t)std::min<size_t>(cap_after, (1u << 30) - 1), 0, sl.tile_bits);
-        return w.keys0.growth(need * 4) + w.keys1.growth(need * 4) + w.vals0.growth(need * 4) +
-               w.vals1.growth(need * 4) + w.sort_temp.growth(sort_after);
+        return w.list0.growth(need * 8) + w.list1.growth(need * 8) + w.sort_temp.growth(sort_after);
     };
     size_t need = std::max<size_t>(sl.k, 1);
     int64_t grow = growth_for(need);
@@ -749,17 +746,14 @@ bool start_phase_b(gr_server* s, Slot& sl) {
         }
     }
     bool grew = false;
-    if (w.keys0.ensure(need * 4, s->workspace, &grew) != GR_OK ||
-        w.keys1.ensure(need * 4, s->workspace, &grew) != GR_OK ||
-        w.vals0.ensure(need * 4, s->workspace, &grew) != GR_OK ||
-        w.vals1.ensure(need * 4, s->workspace, &grew) != GR_OK) {
+    if (w.list0.ensure(need * 8, s->workspace, &grew) != GR_OK ||
+        w.list1.ensure(need * 8, s->workspace, &grew) != GR_OK) {
         complete(s, sl, GR_ERR_OUT_OF_MEMORY);
         sl.batch.clear();
         sl.state = SlotState::Idle;
         return true;
     }
-    sl.capacity = (uint32_t)std::min(std::min(w.keys0.bytes, w.keys1.bytes) / 4,
-                                     std::min(w.vals0.bytes, w.vals1.bytes) / 4);
+    sl.capacity = (uint32_t)(std::min(w.list0.bytes, w.list1.bytes) / 8);
     sl.capacity = std::min<uint32_t>(sl.capacity, (1u << 30) - 1);
     const size_t sort_bytes = radix_sort_temp_bytes((int32_t)sl.capacity, 0, sl.tile_bits);
     if (w.sort_temp.ensure(sort_bytes, s->workspace, &grew) != GR_OK) {

@@ -171,7 +171,9 @@ __device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <typename KeyT>
+// PAIRS = false: keys only (vin/vout unused), e.g. tile-list entries that carry
+// their payload in the key bits above the sorted ones.
+template <typename KeyT, bool PAIRS>
 __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     const KeyT* __restrict__ kin, const int32_t* __restrict__ vin, KeyT* __restrict__ kout,
     int32_t* __restrict__ vout, int32_t n, const uint32_t* count_dev, int strict, int shift, int bits,
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     __shared__ uint32_t s_gbase[kRadix];
     __shared__ uint32_t s_wsum[kRadix / 64];
     __shared__ KeyT s_keys[kSortTile];
-    __shared__ int32_t s_vals[kSortTile];
+    __shared__ int32_t s_vals[PAIRS ? kSortTile : 1];
 
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t == 0) s_part = atomicAdd(ticket, 1u);
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     for (int i = 0; i < kSortItems; ++i) {
         const int64_t idx = wbase + i * 64;
         key[i] = idx < n ? kin[idx] : (KeyT)0;
-        val[i] = idx < n ? vin[idx] : 0;
+        if constexpr (PAIRS) val[i] = idx < n ? vin[idx] : 0;
     }
     // per-wave stable ranking: peers = lanes with the same digit (8 ballots)
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -292,7 +294,7 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
             const uint32_t d = (uint32_t)(key[i] >> shift) & dmask;
             const uint32_t lp = s_dstart[d] + s_wcnt[w][d] + rank[i];
             s_keys[lp] = key[i];
-            s_vals[lp] = val[i];
+            if constexpr (PAIRS) s_vals[lp] = val[i];
         }
     }
     __syncthreads();
@@ -302,7 +304,7 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
         const uint32_t d = (uint32_t)(k >> shift) & dmask;
         const uint32_t gpos = s_gbase[d] + (uint32_t)j;
         kout[gpos] = k;
-        vout[gpos] = s_vals[j];
+        if constexpr (PAIRS) vout[gpos] = s_vals[j];
     }
 }
 
@@ -351,10 +353,16 @@ static hipError_t sort_passes_t(KeyT* keys[2], int32_t* values[2], int32_t* sel,
         const int shift = begin_bit + 8 * p;
         const int bits = min(8, end_bit - shift);
         const int cur = *sel;
-        hipLaunchKernelGGL(onesweep_kernel<KeyT>, dim3(l.partitions), dim3(kSortThreads), 0, s, keys[cur],
-                           values[cur], keys[cur ^ 1], values[cur ^ 1], count, count_dev, strict, shift, bits,
-                           hist + p * kRadix, status[p & 1], (p + 1 < passes) ? status[(p + 1) & 1] : nullptr,
-                           tickets + p);
+        if (values[0])
+            hipLaunchKernelGGL((onesweep_kernel<KeyT, true>), dim3(l.partitions), dim3(kSortThreads), 0, s,
+                               keys[cur], values[cur], keys[cur ^ 1], values[cur ^ 1], count, count_dev, strict,
+                               shift, bits, hist + p * kRadix, status[p & 1],
+                               (p + 1 < passes) ? status[(p + 1) & 1] : nullptr, tickets + p);
+        else
+            hipLaunchKernelGGL((onesweep_kernel<KeyT, false>), dim3(l.partitions), dim3(kSortThreads), 0, s,
+                               keys[cur], nullptr, keys[cur ^ 1], nullptr, count, count_dev, strict, shift, bits,
+                               hist + p * kRadix, status[p & 1], (p + 1 < passes) ? status[(p + 1) & 1] : nullptr,
+                               tickets + p);
         *sel = cur ^ 1;
     }
     if (ev_end) {
@@ -419,6 +427,14 @@ hipError_t radix_sort_pairs_prepared(uint64_t* keys[2], int32_t* values[2], int3
                                      size_t temp_bytes, hipStream_t s, hipEvent_t passes_begin,
                                      hipEvent_t passes_end) {
     return sort_prepared_t<uint64_t>(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s,
+                                     passes_begin, passes_end);
+}
+
+hipError_t radix_sort_keys_prepared(uint64_t* keys[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
+                                    int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes, hipStream_t s,
+                                    hipEvent_t passes_begin, hipEvent_t passes_end) {
+    int32_t* none[2] = {nullptr, nullptr};
+    return sort_prepared_t<uint64_t>(keys, none, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s,
                                      passes_begin, passes_end);
 }
 

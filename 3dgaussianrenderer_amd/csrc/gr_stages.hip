@@ -583,9 +583,9 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
 // Record walk, emit pass: the same 64-record batches (index order), each
 // wave taking batches grid-stride (one tile-digit histogram flush per
 // workgroup).  Record o's entries go to record_offsets[o] + 0, 1, ... in
-// candidate order, as (tile id, Gaussian index) pairs: the offsets follow
-// depth order, so the depth word of the key is carried by the position and
-// the stable tile sort that follows needs the tile id alone.  (Walking the
+// candidate order, as (Gaussian index << 32 | tile id) entries: the offsets
+// follow depth order, so the depth word of the key is carried by the position
+// and the stable tile sort that follows needs the tile bits alone.  (Walking the
 // records in depth order instead would make the stores contiguous but turn
 // the record loads into gathers, which costs more.)
 __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a) {
@@ -610,8 +610,7 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
     const uint32_t cap = (uint32_t)a.capacity;
     auto put = [&](uint32_t p, uint32_t tile, int o) {
         if (p < cap) {
-            a.tile_keys[p] = tile;
-            a.values[p] = s_gi[w][o];
+            a.entries[p] = ((uint64_t)(uint32_t)s_gi[w][o] << 32) | tile;
             if (hist)
                 for (int k = 0; k < a.tile_passes; ++k) atomicAdd(&s_hist[k][(tile >> (8 * k)) & 255u], 1u);
         }
@@ -728,7 +727,7 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
     if (a.records) {
-        if (!a.record_counts || !a.record_masks || !a.record_offsets || !a.tile_keys || a.tile_passes > 3)
+        if (!a.record_counts || !a.record_masks || !a.record_offsets || !a.entries || a.tile_passes > 3)
             return hipErrorInvalidValue;
         constexpr int W = kBuildThreads / 64;
         const unsigned blocks = (unsigned)std::max(1, std::min((a.batches + W - 1) / W, kEmitBlocks));
@@ -911,18 +910,23 @@ hipError_t launch_build_scan_scatter(uint32_t* data, int32_t n, uint32_t* total,
     return hipGetLastError();
 }
 
-// The reference's (tile << 32 | depth word) keys of a tile list held as tile
-// ids: the depth word of entry i is that of Gaussian values[i] in view 0.
-__global__ void expand_keys_kernel(const uint32_t* tile_keys, const int32_t* values, const float* clip_z,
-                                   uint64_t* keys, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        keys[i] = ((uint64_t)tile_keys[i] << 32) | depth_word(clip_z[values[i]]);
+// The reference's (tile << 32 | depth word) keys and index values of a tile
+// list held as (index << 32 | tile) entries: the depth word of an entry is
+// that of its Gaussian in view 0.
+__global__ void expand_keys_kernel(const uint64_t* entries, const float* clip_z, uint64_t* keys, int32_t* values,
+                                   int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t e = entries[i];
+        const int32_t g = (int32_t)(e >> 32);
+        keys[i] = ((e & 0xffffffffull) << 32) | depth_word(clip_z[g]);
+        values[i] = g;
+    }
 }
 
-hipError_t launch_expand_keys(const uint32_t* tile_keys, const int32_t* values, const float* clip_z, uint64_t* keys,
+hipError_t launch_expand_keys(const uint64_t* entries, const float* clip_z, uint64_t* keys, int32_t* values,
                               int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(expand_keys_kernel, dim3(grid_for(n)), dim3(256), 0, s, tile_keys, values, clip_z, keys, n);
+    hipLaunchKernelGGL(expand_keys_kernel, dim3(grid_for(n)), dim3(256), 0, s, entries, clip_z, keys, values, n);
     return hipGetLastError();
 }
 
@@ -945,7 +949,7 @@ hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t s) 
 }
 
 __device__ __forceinline__ uint32_t range_tile(const RangeArgs& a, int i) {
-    return a.keys ? (uint32_t)(a.keys[i] >> 32) : a.tile_keys[i];
+    return (uint32_t)(a.keys[i] >> a.tile_shift);
 }
 
 __global__ __launch_bounds__(256) void ranges_kernel(RangeArgs a) {
@@ -972,7 +976,7 @@ __global__ __launch_bounds__(256) void ranges_kernel(RangeArgs a) {
 
 hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
     if (a.size <= 0) return hipSuccess;
-    if (!a.keys && !a.tile_keys) return hipErrorInvalidValue;
+    if (!a.keys || (a.tile_shift != 0 && a.tile_shift != 32)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(ranges_kernel, dim3((a.size + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
@@ -1027,7 +1031,8 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
             const int nr = min(kRasterThreads, n - r0);
             __syncthreads();
             if ((int)threadIdx.x < nr) {
-                const size_t gi = vo + (size_t)a.values[start + r0 + threadIdx.x];
+                const int k = start + r0 + threadIdx.x;
+                const size_t gi = vo + (size_t)(a.values ? a.values[k] : (int32_t)(a.entries[k] >> 32));
                 const float2 c = a.clip_xy[gi];
                 const float4 q = a.conic[gi];
                 const float4 col = a.color[gi];

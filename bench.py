@@ -13,9 +13,10 @@ sample of the same workload.
 
 Roofline: the dominant kernel is one pass of the tile-bit onesweep sort (the
 largest single-kernel time of the frame).  The renderer's tile list holds
-(tile id, Gaussian index) pairs of 4 + 4 bytes (the entries are in depth
-order before the tile sort, so the depth word needs no storage), and a pass
-reads and writes each pair once: 16 B per tile-list entry, so
+8-byte entries (Gaussian index << 32 | tile id; the entries are in depth
+order before the tile sort, so the depth word needs no storage), sorted on
+the tile bits, and a pass reads and writes each entry once: 16 B per
+tile-list entry, so
 achieved = 16 * K / (average pass duration), the duration measured with HIP
 events around the passes on the renderer's stream in every timed frame.
 `traffic` is the measured HBM bytes per pass launch, taken from the committed
@@ -185,7 +186,7 @@ def main():
     passes = kt["tile_sort_passes"]
     # algorithmic bytes per stage (FP32, degree 3): SH reads position + 48 coefficients and
     # writes colour; projection reads position + scale/rotation and writes xy, z, ellipse, conic;
-    # build reads xy/z/ellipse and writes K (tile, index) entries; sort = depth words of the
+    # build reads xy/z/ellipse and writes K 8-byte entries; sort = depth words of the
     # visible Gaussians (not counted) + the tile-bit passes over K entries
     alg = {
         "sh_eval_ms": 224.0 * N,
@@ -198,7 +199,7 @@ def main():
     # tile_sort_pass_ms spans all passes of a frame (back to back between two events)
     pass_avg_ms = float(np.mean(pass_ms)) / passes if passes > 0 else float("nan")
     achieved = 16.0 * K / (pass_avg_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": "onesweep_kernel<unsigned int> (tile-bit sort pass)",
+    roof = {"bound": "hbm", "kernel": "onesweep_kernel<unsigned long, false> (tile-bit sort pass)",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.traffic_profile),
             "algorithmic_bytes_per_launch": int(16.0 * K), "avg_launch_us": round(pass_avg_ms * 1e3, 2),

@@ -317,9 +317,12 @@ hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
             K = a.batch_counts[a.nviews * a.batches];
         }
         const uint32_t n = std::min<uint32_t>(K, (uint32_t)a.capacity);
-        if (a.records) touch(a.tile_keys, (size_t)n * 4);
-        else touch(a.keys, (size_t)n * 8);
-        touch(a.values, (size_t)n * 4);
+        if (a.records) {
+            touch(a.entries, (size_t)n * 8);
+        } else {
+            touch(a.keys, (size_t)n * 8);
+            touch(a.values, (size_t)n * 4);
+        }
     });
     return hipSuccess;
 }
@@ -334,8 +337,8 @@ hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t s) 
 hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
     run_or_record(s, [=] {
         const uint32_t n = a.size_dev ? std::min<uint32_t>(*a.size_dev, (uint32_t)a.size) : (uint32_t)a.size;
-        if (a.keys) rd(a.keys, (size_t)n * 8);
-        else rd(a.tile_keys, (size_t)n * 4);
+        if (a.tile_shift != 0 && a.tile_shift != 32) std::abort();
+        rd(a.keys, (size_t)n * 8);
         touch(a.tile_range, (size_t)a.tiles * 8);
     });
     return hipSuccess;
@@ -383,6 +386,23 @@ hipError_t radix_sort_pairs_prepared(uint64_t* keys[2], int32_t* values[2], int3
                                      const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                                      size_t temp_bytes, hipStream_t s, hipEvent_t, hipEvent_t) {
     return mock_sort(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
+}
+
+hipError_t radix_sort_keys_prepared(uint64_t* keys[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
+                                    int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes, hipStream_t s,
+                                    hipEvent_t, hipEvent_t) {
+    if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
+    if (temp_bytes < radix_sort_temp_bytes(count, begin_bit, end_bit)) return hipErrorInvalidValue;
+    const int passes = (end_bit - begin_bit + 7) / 8;
+    uint64_t* k0 = keys[*sel];
+    uint64_t* k1 = keys[*sel ^ 1];
+    run_or_record(s, [=] {
+        const uint32_t n = count_dev ? std::min<uint32_t>(*count_dev, (uint32_t)count) : (uint32_t)count;
+        rd(k0, (size_t)n * 8);
+        touch(k1, (size_t)n * 8);
+    });
+    if (passes & 1) *sel ^= 1;
+    return hipSuccess;
 }
 
 hipError_t radix_sort_pairs_prepared_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,

@@ -1,0 +1,113 @@
+"""Full-scale frames: 6,000,000 clustered Gaussians (SH degree 3) at 1920x1080 and
+3840x2160, FP32 storage and the FP16/BF16 precision form.
+
+For every frame: the renderer's sorted tile list (reference key form) and image
+against the FP32 oracle, bit-exact (on the decoded inputs for the precision
+form; the bar is max |diff| <= 1/255, bit-exact is stronger); the image
+against the FP64 render of the FP32 scene, PSNR >= 50 dB (40 dB for BF16
+storage), with max |diff| reported; and the tile list against the FP64
+render's list -- pairs present in only one of them and, over the pairs both
+hold, positions whose order differs (FP32 and FP64 place footprint edges and
+equal-looking depths differently, so these are reported, not required to be
+zero).  Each case prints one `[c3]` line with these figures.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import frame_list, gr
+
+pytestmark = pytest.mark.gpu
+
+N = 6_000_000
+SIZES = {"1080p": (1920, 1080), "4k": (3840, 2160)}
+_cache = {}
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def c3(orc):
+    say("c3: generating 6M clustered Gaussians")
+    scene = gr.generate_scene(2, N, 3, 3)
+    shal = orc.realign_sh(scene["sh"], N, 48)
+    return scene, shal
+
+
+def camera(scene, W, H):
+    c = scene["centroid"]
+    return gr.camera_make(gr.pose_eye(3, 0, c, 8.0), c, float(np.deg2rad(50.0)), W, H, 0.01, 100.0)
+
+
+def fp64(orc, scene, shal, size):
+    """FP64 render of the FP32 scene (image and tile list), once per size."""
+    if size not in _cache:
+        W, H = SIZES[size]
+        _cache[size] = orc.render64(scene["position"], scene["scale_rot"], shal, scene["color"], N, 3,
+                                    camera(scene, W, H).packed(), W, H, want_list=True)
+    return _cache[size]
+
+
+def list_diff(keys_a, vals_a, keys_b, vals_b):
+    """(pairs only in a, pairs only in b, order mismatches over the common pairs)."""
+    ca = ((keys_a >> np.uint64(32)).astype(np.int64) << 32) | vals_a.astype(np.int64)
+    cb = ((keys_b >> np.uint64(32)).astype(np.int64) << 32) | vals_b.astype(np.int64)
+    common = np.intersect1d(ca, cb, assume_unique=True)
+    sa = ca[np.isin(ca, common, assume_unique=True)]
+    sb = cb[np.isin(cb, common, assume_unique=True)]
+    return int(len(ca) - len(common)), int(len(cb) - len(common)), int(np.count_nonzero(sa != sb))
+
+
+@pytest.mark.parametrize("size", list(SIZES))
+def test_c3_fp32(torch_cuda, orc, c3, size):
+    scene, shal = c3
+    W, H = SIZES[size]
+    say(f"c3 fp32 {size}")
+    cam = camera(scene, W, H)
+    r = gr.Renderer()
+    r.upload(scene)
+    img, st = r.render(cam, W, H)
+    keys, vals, _ = frame_list(r)
+    r.close()
+    ref = orc.render(scene["position"], scene["scale_rot"], shal, None, N, 3, cam.packed(), W, H)
+    assert st["tile_list_size"] == ref["k"]
+    assert np.array_equal(keys, ref["keys"]) and np.array_equal(vals, ref["values"])
+    assert np.array_equal(img, ref["image"])
+    r64 = fp64(orc, scene, shal, size)
+    p = orc.psnr(img, r64["image"])
+    dmax = int(np.abs(img[..., :3].astype(int) - r64["image"][..., :3].astype(int)).max())
+    only32, only64, order = list_diff(keys, vals, r64["keys"], r64["values"])
+    say(f"[c3] fp32 {size}: K {ref['k']} (FP64 {r64['k']}), tile list == FP32 oracle, image == FP32 oracle, "
+        f"vs FP64: max|d| {dmax} LSB, PSNR {p:.2f} dB, pairs only FP32 {only32}, only FP64 {only64}, "
+        f"order differences {order}")
+    assert p >= 50.0
+
+
+@pytest.mark.parametrize("precision", [1, 2], ids=["fp16", "bf16"])
+@pytest.mark.parametrize("size", list(SIZES))
+def test_c3_precision(torch_cuda, orc, c3, size, precision):
+    scene, shal = c3
+    W, H = SIZES[size]
+    name = "fp16" if precision == 1 else "bf16"
+    say(f"c3 {name} {size}")
+    cam = camera(scene, W, H)
+    r = gr.Renderer()
+    r.upload(scene, precision)
+    img, st = r.render(cam, W, H)
+    keys, vals, _ = frame_list(r)
+    r.close()
+    pos_p, packed, sh16 = gr.pack_precision(scene, precision)
+    pos, sr, shd = orc.decode_precision(pos_p, packed, sh16, N, 3, precision)
+    ref = orc.render(pos, sr, shd, None, N, 3, cam.packed(), W, H)
+    assert st["tile_list_size"] == ref["k"]
+    assert np.array_equal(keys, ref["keys"]) and np.array_equal(vals, ref["values"])
+    assert np.array_equal(img, ref["image"])
+    r64 = fp64(orc, scene, shal, size)
+    p = orc.psnr(img, r64["image"])
+    dmax = int(np.abs(img[..., :3].astype(int) - r64["image"][..., :3].astype(int)).max())
+    only_p, only64, order = list_diff(keys, vals, r64["keys"], r64["values"])
+    say(f"[c3] {name} {size}: K {ref['k']} (FP64 of the FP32 scene {r64['k']}), tile list == oracle on the "
+        f"decoded inputs, vs FP64: max|d| {dmax} LSB, PSNR {p:.2f} dB, pairs only {name} {only_p}, "
+        f"only FP64 {only64}, order differences {order}")
+    assert p >= (50.0 if precision == 1 else 40.0)
