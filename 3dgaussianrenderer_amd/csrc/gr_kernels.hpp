@@ -194,30 +194,21 @@ size_t radix_sort_temp_bytes(int32_t count, int32_t begin_bit, int32_t end_bit);
 hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                             const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                             size_t temp_bytes, hipStream_t stream);
-// Two-step form for a producer that accumulates the digit histograms itself:
-// radix_sort_prepare zeroes the histograms (passes x 256 uint32 at *hist, pass
-// p counting digit (key >> (begin_bit + 8p)) & 255), the partition tickets and
-// the first look-back region of `temp` sized for `count`; after the producer
-// has added exactly the pairs the sort will see, radix_sort_pairs_prepared
-// scans them and runs the passes.  A device-side count above `count` then
-// means the producer did not write the list: the prepared sort leaves the
-// buffers untouched.
+// Two-step form of the keys-only 64-bit sort for a producer that accumulates
+// the digit histograms itself: radix_sort_prepare zeroes the histograms
+// (passes x 256 uint32 at *hist, pass p counting digit
+// (key >> (begin_bit + 8p)) & 255), the partition tickets and the first
+// look-back region of `temp` sized for `count`; after the producer has added
+// exactly the keys the sort will see, radix_sort_keys_prepared scans them and
+// runs the passes.  A device-side count above `count` then means the producer
+// did not write the list: the prepared sort leaves the buffers untouched.
+// Used for the renderer's tile-list entries (Gaussian index << 32 | tile id),
+// sorted on the tile bits.
 hipError_t radix_sort_prepare(int32_t count, void* temp, size_t temp_bytes, uint32_t** hist, hipStream_t stream);
-hipError_t radix_sort_pairs_prepared(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
-                                     const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
-                                     size_t temp_bytes, hipStream_t stream, hipEvent_t passes_begin = nullptr,
-                                     hipEvent_t passes_end = nullptr);
-// The prepared form for 64-bit keys without values: the renderer's tile-list
-// entries (Gaussian index << 32 | tile id), sorted on the tile bits.
 hipError_t radix_sort_keys_prepared(uint64_t* keys[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
                                     int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes,
                                     hipStream_t stream, hipEvent_t passes_begin = nullptr,
                                     hipEvent_t passes_end = nullptr);
-// The prepared form for 32-bit keys.
-hipError_t radix_sort_pairs_prepared_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
-                                         const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
-                                         size_t temp_bytes, hipStream_t stream, hipEvent_t passes_begin = nullptr,
-                                         hipEvent_t passes_end = nullptr);
 // The same for 32-bit keys (depth words); same temp layout and size.
 hipError_t radix_sort_pairs_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                                 const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,

@@ -522,6 +522,8 @@ gr_status run_phase(gr_server* s, Slot& sl, const Scene& sc, bool phase_b) {
 }
 
 // Takes up to max_views pending requests of the scene with the oldest request.
+constexpr int32_t kBatchTiles = 1 << 16;   // tiles per batch beyond which views go to the next batch
+
 bool form_batch(gr_server* s, Slot& sl) {
     int32_t best = -1;
     double oldest = 0.0;
@@ -541,7 +543,14 @@ bool form_batch(gr_server* s, Slot& sl) {
         s->batch_limit.erase(lim);
     }
     sl.batch.clear();
+    // the batch's global tile ids are sorted on their bits: while they fit 16 bits the sort
+    // takes two 8-bit passes, a 17th bit would add a third pass over every entry
+    int64_t tiles = 0;
     while (!q.empty() && (int)sl.batch.size() < limit) {
+        const FrameGeom g = make_geom(q.front().req.width, q.front().req.height);
+        const int64_t t = (int64_t)g.tiles_x * g.tiles_y;
+        if (!sl.batch.empty() && tiles + t > (int64_t)kBatchTiles) break;
+        tiles += t;
         sl.batch.push_back(q.front());
         q.pop_front();
     }

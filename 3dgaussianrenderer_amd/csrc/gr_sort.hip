@@ -27,19 +27,38 @@
 
 namespace gr {
 
+// Partition shapes (lanes x pairs per lane) per sort: the 64-bit sorts (tile
+// lists) and the 32-bit depth sort of the visible Gaussians; measured on the
+// 6M-Gaussian 1080p frame (512 x 12 and 1024 x 8 fastest respectively; smaller
+// partitions were slower for both).  Build-time overrides exist for sweeps.
 #ifndef GR_SORT_THREADS
 #define GR_SORT_THREADS 512
 #endif
 #ifndef GR_SORT_ITEMS
 #define GR_SORT_ITEMS 12
 #endif
+#ifndef GR_DSORT_THREADS
+#define GR_DSORT_THREADS 1024
+#endif
+#ifndef GR_DSORT_ITEMS
+#define GR_DSORT_ITEMS 8
+#endif
 #ifndef GR_SORT_LOOKBACK
 #define GR_SORT_LOOKBACK 16
 #endif
-constexpr int kSortThreads = GR_SORT_THREADS;
-constexpr int kSortWaves = kSortThreads / 64;
-constexpr int kSortItems = GR_SORT_ITEMS;
-constexpr int kSortTile = kSortThreads * kSortItems;   // pairs per partition
+template <int THREADS, int ITEMS>
+struct SortShape {
+    static constexpr int kThreads = THREADS;
+    static constexpr int kWaves = THREADS / 64;
+    static constexpr int kItems = ITEMS;
+    static constexpr int kTile = THREADS * ITEMS;   // pairs per partition
+    static_assert(THREADS % 64 == 0 && THREADS >= 256, "one lane per digit in the look-back");
+};
+template <typename KeyT> struct ShapeOf { using type = SortShape<GR_SORT_THREADS, GR_SORT_ITEMS>; };
+template <> struct ShapeOf<uint32_t> { using type = SortShape<GR_DSORT_THREADS, GR_DSORT_ITEMS>; };
+constexpr int kMinTile = GR_SORT_THREADS * GR_SORT_ITEMS < GR_DSORT_THREADS * GR_DSORT_ITEMS
+                             ? GR_SORT_THREADS * GR_SORT_ITEMS
+                             : GR_DSORT_THREADS * GR_DSORT_ITEMS;
 constexpr int kRadix = 256;
 constexpr int kMaxPasses = 8;
 constexpr int kLookback = GR_SORT_LOOKBACK;            // predecessors read per look-back step
@@ -47,21 +66,25 @@ constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagIncl = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1u;
 
+// Scratch layout: histograms, tickets, two look-back regions sized for the
+// smallest partition shape (so one scratch serves every sort of `count` pairs);
+// `partitions` is the count of the sort at hand.
 struct SortLayout {
     size_t hist_off, ticket_off, status_off[2], total;
     int partitions;
 };
 
-static SortLayout sort_layout(int32_t count) {
+static SortLayout sort_layout(int32_t count, int tile) {
     SortLayout l;
-    l.partitions = (count + kSortTile - 1) / kSortTile;
+    l.partitions = (count + tile - 1) / tile;
+    const int64_t max_parts = ((int64_t)count + kMinTile - 1) / kMinTile;
     size_t off = 0;
     l.hist_off = off;
     off += sizeof(uint32_t) * kMaxPasses * kRadix;
     l.ticket_off = off;
     off += 256;
     off = (off + 255) & ~(size_t)255;
-    const size_t region = (size_t)l.partitions * kRadix * sizeof(uint32_t);
+    const size_t region = (size_t)max_parts * kRadix * sizeof(uint32_t);
     l.status_off[0] = off;
     off += (region + 255) & ~(size_t)255;
     l.status_off[1] = off;
@@ -70,8 +93,12 @@ static SortLayout sort_layout(int32_t count) {
     return l;
 }
 
+template <typename KeyT> static SortLayout layout_for(int32_t count) {
+    return sort_layout(count < 1 ? 1 : count, ShapeOf<KeyT>::type::kTile);
+}
+
 size_t radix_sort_temp_bytes(int32_t count, int32_t, int32_t) {
-    return sort_layout(count < 1 ? 1 : count).total;
+    return sort_layout(count < 1 ? 1 : count, kMinTile).total;
 }
 
 __global__ void sort_init_kernel(uint32_t* hist, uint32_t* tickets, uint32_t* status0, int64_t status_n) {
@@ -173,11 +200,15 @@ __device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
 
 // PAIRS = false: keys only (vin/vout unused), e.g. tile-list entries that carry
 // their payload in the key bits above the sorted ones.
-template <typename KeyT, bool PAIRS>
-__global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
+template <typename KeyT, bool PAIRS, class S>
+__global__ __launch_bounds__(S::kThreads) void onesweep_kernel(
     const KeyT* __restrict__ kin, const int32_t* __restrict__ vin, KeyT* __restrict__ kout,
     int32_t* __restrict__ vout, int32_t n, const uint32_t* count_dev, int strict, int shift, int bits,
     const uint32_t* __restrict__ digit_off, uint32_t* status, uint32_t* status_next, uint32_t* ticket) {
+    constexpr int kSortThreads = S::kThreads;
+    constexpr int kSortWaves = S::kWaves;
+    constexpr int kSortItems = S::kItems;
+    constexpr int kSortTile = S::kTile;
     __shared__ uint32_t s_part;
     __shared__ uint32_t s_wcnt[kSortWaves][kRadix];
     __shared__ uint32_t s_dstart[kRadix];
@@ -329,7 +360,7 @@ static hipError_t sort_check(int32_t count, int32_t begin_bit, int32_t end_bit, 
     constexpr int kKeyBits = 8 * (int)sizeof(KeyT);
     if ((uint32_t)count > kCountMask || begin_bit < 0 || end_bit > kKeyBits) return hipErrorInvalidValue;
     if (sort_passes(begin_bit, end_bit) > kMaxPasses) return hipErrorInvalidValue;
-    if (temp_bytes < sort_layout(count < 1 ? 1 : count).total || temp == nullptr) return hipErrorInvalidValue;
+    if (temp_bytes < layout_for<KeyT>(count).total || temp == nullptr) return hipErrorInvalidValue;
     return hipSuccess;
 }
 
@@ -338,7 +369,8 @@ template <typename KeyT>
 static hipError_t sort_passes_t(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                                 const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, char* tb, int strict,
                                 hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) {
-    const SortLayout l = sort_layout(count);
+    using S = typename ShapeOf<KeyT>::type;
+    const SortLayout l = layout_for<KeyT>(count);
     const int passes = sort_passes(begin_bit, end_bit);
     uint32_t* hist = reinterpret_cast<uint32_t*>(tb + l.hist_off);
     uint32_t* tickets = reinterpret_cast<uint32_t*>(tb + l.ticket_off);
@@ -354,12 +386,12 @@ static hipError_t sort_passes_t(KeyT* keys[2], int32_t* values[2], int32_t* sel,
         const int bits = min(8, end_bit - shift);
         const int cur = *sel;
         if (values[0])
-            hipLaunchKernelGGL((onesweep_kernel<KeyT, true>), dim3(l.partitions), dim3(kSortThreads), 0, s,
+            hipLaunchKernelGGL((onesweep_kernel<KeyT, true, S>), dim3(l.partitions), dim3(S::kThreads), 0, s,
                                keys[cur], values[cur], keys[cur ^ 1], values[cur ^ 1], count, count_dev, strict,
                                shift, bits, hist + p * kRadix, status[p & 1],
                                (p + 1 < passes) ? status[(p + 1) & 1] : nullptr, tickets + p);
         else
-            hipLaunchKernelGGL((onesweep_kernel<KeyT, false>), dim3(l.partitions), dim3(kSortThreads), 0, s,
+            hipLaunchKernelGGL((onesweep_kernel<KeyT, false, S>), dim3(l.partitions), dim3(S::kThreads), 0, s,
                                keys[cur], nullptr, keys[cur ^ 1], nullptr, count, count_dev, strict, shift, bits,
                                hist + p * kRadix, status[p & 1], (p + 1 < passes) ? status[(p + 1) & 1] : nullptr,
                                tickets + p);
@@ -379,7 +411,7 @@ static hipError_t sort_pairs_t(KeyT* keys[2], int32_t* values[2], int32_t* sel, 
     if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
     hipError_t e = sort_check<KeyT>(count, begin_bit, end_bit, temp, temp_bytes);
     if (e != hipSuccess) return e;
-    const SortLayout l = sort_layout(count);
+    const SortLayout l = layout_for<KeyT>(count);
     char* tb = static_cast<char*>(temp);
     e = sort_prepare(l, tb, s);
     if (e != hipSuccess) return e;
@@ -404,46 +436,21 @@ hipError_t radix_sort_pairs_u32(uint32_t* keys[2], int32_t* values[2], int32_t* 
 }
 
 hipError_t radix_sort_prepare(int32_t count, void* temp, size_t temp_bytes, uint32_t** hist, hipStream_t s) {
-    if (count < 1) count = 1;
-    const SortLayout l = sort_layout(count);
+    const SortLayout l = layout_for<uint64_t>(count);
     if (temp_bytes < l.total || temp == nullptr || hist == nullptr) return hipErrorInvalidValue;
     *hist = reinterpret_cast<uint32_t*>(static_cast<char*>(temp) + l.hist_off);
     return sort_prepare(l, static_cast<char*>(temp), s);
 }
 
-template <typename KeyT>
-static hipError_t sort_prepared_t(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
-                                  const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
-                                  size_t temp_bytes, hipStream_t s, hipEvent_t passes_begin, hipEvent_t passes_end) {
-    if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
-    hipError_t e = sort_check<KeyT>(count, begin_bit, end_bit, temp, temp_bytes);
-    if (e != hipSuccess) return e;
-    return sort_passes_t<KeyT>(keys, values, sel, count, count_dev, begin_bit, end_bit, static_cast<char*>(temp), 1,
-                               s, passes_begin, passes_end);
-}
-
-hipError_t radix_sort_pairs_prepared(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
-                                     const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
-                                     size_t temp_bytes, hipStream_t s, hipEvent_t passes_begin,
-                                     hipEvent_t passes_end) {
-    return sort_prepared_t<uint64_t>(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s,
-                                     passes_begin, passes_end);
-}
-
 hipError_t radix_sort_keys_prepared(uint64_t* keys[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
                                     int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes, hipStream_t s,
                                     hipEvent_t passes_begin, hipEvent_t passes_end) {
+    if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
+    hipError_t e = sort_check<uint64_t>(count, begin_bit, end_bit, temp, temp_bytes);
+    if (e != hipSuccess) return e;
     int32_t* none[2] = {nullptr, nullptr};
-    return sort_prepared_t<uint64_t>(keys, none, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s,
-                                     passes_begin, passes_end);
-}
-
-hipError_t radix_sort_pairs_prepared_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
-                                         const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
-                                         size_t temp_bytes, hipStream_t s, hipEvent_t passes_begin,
-                                         hipEvent_t passes_end) {
-    return sort_prepared_t<uint32_t>(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s,
-                                     passes_begin, passes_end);
+    return sort_passes_t<uint64_t>(keys, none, sel, count, count_dev, begin_bit, end_bit, static_cast<char*>(temp),
+                                   1, s, passes_begin, passes_end);
 }
 
 }  // namespace gr

@@ -518,6 +518,9 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
 // exact entry count and a 64-bit mask whose bit l says whether candidate l
 // (row-major in the footprint) passed the exact test -- the whole answer for
 // footprints of at most 64 tiles, so the emit pass needs no tests for them.
+// (Queueing the candidates the two centre tests leave open and running the
+// edge tests on full waves of them measured slower: most candidates need the
+// edge tests anyway.)
 __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a) {
     constexpr int W = kBuildThreads / 64;
     __shared__ int s_excl[W][64];

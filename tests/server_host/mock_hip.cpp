@@ -382,12 +382,6 @@ hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel,
     return mock_sort(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
 }
 
-hipError_t radix_sort_pairs_prepared(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
-                                     const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
-                                     size_t temp_bytes, hipStream_t s, hipEvent_t, hipEvent_t) {
-    return mock_sort(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
-}
-
 hipError_t radix_sort_keys_prepared(uint64_t* keys[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
                                     int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes, hipStream_t s,
                                     hipEvent_t, hipEvent_t) {
@@ -403,12 +397,6 @@ hipError_t radix_sort_keys_prepared(uint64_t* keys[2], int32_t* sel, int32_t cou
     });
     if (passes & 1) *sel ^= 1;
     return hipSuccess;
-}
-
-hipError_t radix_sort_pairs_prepared_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
-                                         const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
-                                         size_t temp_bytes, hipStream_t s, hipEvent_t, hipEvent_t) {
-    return mock_sort(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
 }
 
 hipError_t radix_sort_pairs_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,

@@ -4,8 +4,8 @@
 For every frame: the renderer's sorted tile list (reference key form) and image
 against the FP32 oracle, bit-exact (on the decoded inputs for the precision
 form; the bar is max |diff| <= 1/255, bit-exact is stronger); the image
-against the FP64 render of the FP32 scene, PSNR >= 50 dB (40 dB for BF16
-storage), with max |diff| reported; and the tile list against the FP64
+against the FP64 render of the FP32 scene, PSNR >= 50 dB, with max |diff|
+reported; and the tile list against the FP64
 render's list -- pairs present in only one of them and, over the pairs both
 hold, positions whose order differs (FP32 and FP64 place footprint edges and
 equal-looking depths differently, so these are reported, not required to be
@@ -110,4 +110,4 @@ def test_c3_precision(torch_cuda, orc, c3, size, precision):
     say(f"[c3] {name} {size}: K {ref['k']} (FP64 of the FP32 scene {r64['k']}), tile list == oracle on the "
         f"decoded inputs, vs FP64: max|d| {dmax} LSB, PSNR {p:.2f} dB, pairs only {name} {only_p}, "
         f"only FP64 {only64}, order differences {order}")
-    assert p >= (50.0 if precision == 1 else 40.0)
+    assert p >= 50.0
