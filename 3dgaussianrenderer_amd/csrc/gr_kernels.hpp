@@ -96,7 +96,6 @@ struct CompactArgs {
     const float4* ellipse;
     float4* records;                // [visible][2]
     uint32_t* depth_keys;           // [visible]
-    int32_t* order;                 // [visible] record index (identity before the sort)
     int32_t count;
     int32_t batches;
     int32_t nviews;
@@ -114,6 +113,8 @@ struct BuildArgs {
     const uint32_t* record_count;// number of valid records (device)
     uint32_t* record_counts;     // count pass: exact entries per record
     uint64_t* record_masks;      // count pass: passing candidates 0..63 per record
+    uint32_t* depth_words;       // count pass: record index | entry count << index_bits (depth sort values)
+    int32_t index_bits;          // packed_index_bits(nviews * count)
     const uint32_t* record_offsets; // emit pass: output offset per record
     uint32_t* tile_hist;         // record emit: 256-bin histograms of the tile's 8-bit digits (NULL: none)
     int32_t tile_passes;         // digits counted into tile_hist (<= 3)
@@ -167,16 +168,17 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t stream);
 int scan_scratch_words(int32_t n);
 hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_t* scratch,
                              hipStream_t stream);
-// The same scan, but the exclusive prefix of element j goes to offsets[order[j]] (for j < *valid)
-// instead of back to data[j].
-hipError_t launch_build_scan_scatter(uint32_t* data, int32_t n, uint32_t* total, uint32_t* scratch,
-                                     const int32_t* order, const uint32_t* valid, uint32_t* offsets,
-                                     hipStream_t stream);
+// Bits of the record index in a packed record word for n records (>= 1, <= 31).
+int packed_index_bits(int64_t n);
+// Exclusive scan of the counts held by packed record words (record index in
+// bits [0, index_bits), count above it, saturated counts looked up in
+// counts[index]): the first *valid words, in their order; the prefix of word j
+// goes to offsets[index of word j] and the total to *total.
+hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint32_t* valid, const uint32_t* counts,
+                                    int32_t index_bits, uint32_t* total, uint32_t* scratch, uint32_t* offsets,
+                                    hipStream_t stream);
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t stream);
 hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t stream);
-// tmp[j] = counts[order[j]] for j < *valid, 0 for valid <= j < n
-hipError_t launch_gather_counts(const uint32_t* counts, const int32_t* order, const uint32_t* valid, uint32_t* tmp,
-                                int32_t n, hipStream_t stream);
 // entries (index << 32 | tile) -> keys[i] = tile << 32 | depth word of clip_z[index], values[i] = index
 hipError_t launch_expand_keys(const uint64_t* entries, const float* clip_z, uint64_t* keys, int32_t* values,
                               int64_t n, hipStream_t stream);

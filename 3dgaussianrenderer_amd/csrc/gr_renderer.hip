@@ -4,12 +4,13 @@
 //
 // Frame (one HIP stream; the only host sync is at the end, after the image):
 //   SH (degree > 0) -> project (+ per-64 visibility ballot)
-//   -> compaction of the visible Gaussians into build records and (depth
-//      word, record) pairs -> build count: exact entries and the mask of
-//      passing candidates per record
-//   -> 32-bit onesweep sort of the pairs by depth word -> the counts taken
-//      in depth order are scanned (K) and scattered back as per-record
-//      output offsets
+//   -> compaction of the visible Gaussians into build records and depth
+//      words -> build count: exact entries and the mask of passing
+//      candidates per record, and the depth sort's value (record index with
+//      the count packed above it)
+//   -> 32-bit onesweep sort of the (depth word, record | count) pairs -> the
+//      counts, now in depth order, are scanned (K) and scattered back as
+//      per-record output offsets
 //   -> emit (records in index order): each record's entries at its offset,
 //      + tile digit histograms
 //   -> onesweep sort of the tile bits only (bits [32, 32 + tile bits))
@@ -105,7 +106,7 @@ struct gr_renderer {
     DevBuf<uchar4> out;
     DevBuf<uint64_t> vis_bits;      // per 64 Gaussians: non-empty tile rect ballot
     DevBuf<uint32_t> vis_counts;    // popcounts -> exclusive offsets, total, scan scratch
-    DevBuf<uint32_t> batch_counts;  // record counts in depth order -> offsets, total K, scan scratch
+    DevBuf<uint32_t> batch_counts;  // total K, scan scratch
     DevBuf<uint32_t> rec_counts;    // entries per build record
     DevBuf<uint32_t> rec_offsets;   // output offset per build record
     DevBuf<uint64_t> rec_masks;     // passing candidates 0..63 per build record
@@ -253,7 +254,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     const size_t nn = (size_t)std::max(n, 1);
     R_CHECK(r->vis_bits.ensure((size_t)std::max(batches, 1)));
     R_CHECK(r->vis_counts.ensure((size_t)batches + 2 + scan_scratch_words(batches)));
-    R_CHECK(r->batch_counts.ensure(nn + 2 + scan_scratch_words((int32_t)nn)));
+    R_CHECK(r->batch_counts.ensure(2 + scan_scratch_words((int32_t)nn)));
     R_CHECK(r->rec_counts.ensure(nn));
     R_CHECK(r->rec_offsets.ensure(nn));
     R_CHECK(r->rec_masks.ensure(nn));
@@ -275,7 +276,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     vd.tiles = tiles;
     vd.out = outp;
     uint32_t* vis_total = r->vis_counts.p + batches;
-    uint32_t* k_total = r->batch_counts.p + n;
+    uint32_t* k_total = r->batch_counts.p;
 
     R_HIP(hipEventRecord(r->ev[0], stream));
     if (r->degree > 0 && n > 0) {
@@ -321,7 +322,6 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         a.ellipse = r->ellipse.p;
         a.records = r->records.p;
         a.depth_keys = r->dkeys[0].p;
-        a.order = r->dorder[0].p;
         a.count = n;
         a.batches = batches;
         a.nviews = 1;
@@ -333,6 +333,8 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     b.record_count = vis_total;
     b.record_counts = r->rec_counts.p;
     b.record_masks = r->rec_masks.p;
+    b.depth_words = reinterpret_cast<uint32_t*>(r->dorder[0].p);
+    b.index_bits = packed_index_bits(std::max(n, 1));
     b.entries = r->entries[0].p;
     b.count = n;
     b.capacity = capacity;
@@ -349,10 +351,10 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         R_HIP(radix_sort_pairs_u32(dk, dv, &dsel, n, vis_total, 0, 32, r->sort_temp, r->sort_temp_bytes, stream));
     }
     R_HIP(hipEventRecord(r->ev[4], stream));
-    // output offsets: counts in depth order, scanned (total K), scattered back to the records
-    R_HIP(launch_gather_counts(r->rec_counts.p, r->dorder[dsel].p, vis_total, r->batch_counts.p, n, stream));
-    R_HIP(launch_build_scan_scatter(r->batch_counts.p, n, k_total, k_total + 1, r->dorder[dsel].p, vis_total,
-                                    r->rec_offsets.p, stream));
+    // output offsets: the counts carried by the sorted words, scanned in depth order (total K)
+    // and scattered back to the records
+    R_HIP(launch_build_scan_packed(reinterpret_cast<const uint32_t*>(r->dorder[dsel].p), n, vis_total,
+                                   r->rec_counts.p, b.index_bits, k_total, k_total + 1, r->rec_offsets.p, stream));
     b.record_offsets = r->rec_offsets.p;
     const int tile_bits = tile_bits_for(tiles);
     R_HIP(radix_sort_prepare(capacity, r->sort_temp, r->sort_temp_bytes, &b.tile_hist, stream));

@@ -144,7 +144,7 @@ struct Workspace {
     Dev dkeys0, dkeys1, dorder0, dorder1;   // [<= V*N] depth words and record indices of visible Gaussians
     Dev records;                      // [<= V*N][2] float4 build records
     Dev dsort_temp;                   // depth sort scratch
-    Dev counts;                       // record counts in depth order -> offsets [V*N] + K + scratch
+    Dev counts;                       // K (device) + scan scratch
     Dev rec_counts, rec_offsets;      // [V*N] entries and output offset per record
     Dev rec_masks;                    // [V*N] passing candidates 0..63 per record
     Dev list0, list1;                 // tile lists: Gaussian index << 32 | global tile id
@@ -276,7 +276,7 @@ std::vector<PhaseANeed> phase_a_needs(Slot& sl, const Scene& sc, size_t nviews, 
             {&w.dorder1, V * N * 4},
             {&w.records, V * N * 32},
             {&w.dsort_temp, (int64_t)radix_sort_temp_bytes((int32_t)std::max<int64_t>(V * N, 1), 0, 32)},
-            {&w.counts, 4 * (V * N + 2 + scan_scratch_words((int32_t)(V * N)))},
+            {&w.counts, 4 * (2 + scan_scratch_words((int32_t)(V * N)))},
             {&w.rec_counts, V * N * 4},
             {&w.rec_offsets, V * N * 4},
             {&w.rec_masks, V * N * 8},
@@ -347,6 +347,8 @@ BuildArgs record_build_args(Slot& sl, const Scene& sc) {
     b.record_counts = w.rec_counts.as<uint32_t>();
     b.record_offsets = w.rec_offsets.as<uint32_t>();
     b.record_masks = w.rec_masks.as<uint64_t>();
+    b.depth_words = w.dorder0.as<uint32_t>();
+    b.index_bits = packed_index_bits(std::max<int64_t>((int64_t)V * N, 1));
     b.count = N;
     b.capacity = (int32_t)sl.capacity;
     b.batches = (int32_t)(((int64_t)V * N + kBuildBatch - 1) / kBuildBatch);
@@ -411,7 +413,6 @@ gr_status enqueue_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     c.ellipse = w.ell.as<float4>();
     c.records = w.records.as<float4>();
     c.depth_keys = w.dkeys0.as<uint32_t>();
-    c.order = w.dorder0.as<int32_t>();
     c.count = N;
     c.batches = batches;
     c.nviews = V;
@@ -425,11 +426,12 @@ gr_status enqueue_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
                                sl.stream));
     uint32_t* counts = w.counts.as<uint32_t>();
     const int vn = V * N;
-    // output offsets: counts in depth order, scanned (K), scattered back to the records
-    S_HIP(launch_gather_counts(w.rec_counts.as<uint32_t>(), dv[dsel], vis + nbv, counts, vn, sl.stream));
-    S_HIP(launch_build_scan_scatter(counts, vn, counts + vn, counts + vn + 2, dv[dsel], vis + nbv,
-                                    w.rec_offsets.as<uint32_t>(), sl.stream));
-    S_HIP(hipMemcpyAsync(w.k_host.p, counts + vn, 4, hipMemcpyDeviceToHost, sl.stream));
+    // output offsets: the counts carried by the sorted words, scanned in depth order (K) and
+    // scattered back to the records
+    S_HIP(launch_build_scan_packed(reinterpret_cast<const uint32_t*>(dv[dsel]), vn, vis + nbv,
+                                   w.rec_counts.as<uint32_t>(), b.index_bits, counts, counts + 2,
+                                   w.rec_offsets.as<uint32_t>(), sl.stream));
+    S_HIP(hipMemcpyAsync(w.k_host.p, counts, 4, hipMemcpyDeviceToHost, sl.stream));
     return GR_OK;
 }
 
@@ -440,21 +442,20 @@ gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
     const ViewDesc* vhost = w.views_host.as<ViewDesc>();
     ViewDesc* vdev = w.views.as<ViewDesc>();
     uint32_t* counts = w.counts.as<uint32_t>();
-    const int nb = V * N;   // the device-side K sits after the V*N scanned counts
     BuildArgs b = record_build_args(sl, sc);
     S_HIP(radix_sort_prepare((int32_t)sl.capacity, w.sort_temp.p, w.sort_temp.bytes, &b.tile_hist, sl.stream));
     b.tile_passes = (sl.tile_bits + 7) / 8;
     S_HIP(launch_build_emit(b, sl.stream));
     uint64_t* kb[2] = {w.list0.as<uint64_t>(), w.list1.as<uint64_t>()};
     int32_t sel = 0;
-    S_HIP(radix_sort_keys_prepared(kb, &sel, (int32_t)sl.capacity, counts + nb, 0, sl.tile_bits, w.sort_temp.p,
+    S_HIP(radix_sort_keys_prepared(kb, &sel, (int32_t)sl.capacity, counts, 0, sl.tile_bits, w.sort_temp.p,
                                    w.sort_temp.bytes, sl.stream));
     S_HIP(launch_fill_i32(w.ranges.as<int32_t>(), -1, 2 * (int64_t)sl.tiles_total, sl.stream));
     RangeArgs r{};
     r.keys = kb[sel];
     r.tile_shift = 0;
     r.tile_range = w.ranges.as<int32_t>();
-    r.size_dev = counts + nb;
+    r.size_dev = counts;
     r.size = (int32_t)sl.capacity;
     r.tiles = sl.tiles_total;
     S_HIP(launch_ranges(r, sl.stream));

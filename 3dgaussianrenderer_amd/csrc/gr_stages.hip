@@ -578,8 +578,12 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
         wave_sync();
     }
     if (pos < limit) {
-        a.record_counts[pos] = s_cnt[w][lane];
+        const uint32_t c = s_cnt[w][lane];
+        a.record_counts[pos] = c;
         a.record_masks[pos] = s_mask[w][lane];
+        // the depth sort's value: record index | count (saturated) in the bits above it
+        const uint32_t esc = 0xffffffffu >> a.index_bits;
+        a.depth_words[pos] = (uint32_t)pos | (min(c, esc) << a.index_bits);
     }
 }
 
@@ -719,7 +723,9 @@ static dim3 build_grid(const BuildArgs& a) {
 hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
     if (a.records) {
-        if (!a.record_counts || !a.record_masks) return hipErrorInvalidValue;
+        if (!a.record_counts || !a.record_masks || !a.depth_words || a.index_bits < 1 || a.index_bits > 31 ||
+            ((int64_t)1 << a.index_bits) < (int64_t)a.nviews * a.count)
+            return hipErrorInvalidValue;
         hipLaunchKernelGGL(record_count_kernel, build_grid(a), dim3(kBuildThreads), 0, s, a);
     } else {
         hipLaunchKernelGGL(build_kernel<false>, build_grid(a), dim3(kBuildThreads), 0, s, a);
@@ -747,7 +753,8 @@ static unsigned grid_for(int64_t n) {
 }
 
 // Compaction of the visible Gaussians (non-empty tile rect) of every view into
-// build records and (depth word, record index) pairs, in (view, index) order:
+// build records and depth words, in (view, index) order (the count pass adds
+// the depth sort's values):
 // one wave per 64-Gaussian batch, position = scanned batch offset + rank in
 // the ballot.
 __global__ __launch_bounds__(256) void compact_depth_kernel(CompactArgs a) {
@@ -767,21 +774,6 @@ __global__ __launch_bounds__(256) void compact_depth_kernel(CompactArgs a) {
     a.records[2 * (size_t)p] = make_float4(c.x, c.y, e.x, e.y);
     a.records[2 * (size_t)p + 1] = make_float4(e.z, e.w, __uint_as_float(dw), __int_as_float(v * a.count + i));
     a.depth_keys[p] = dw;
-    a.order[p] = (int32_t)p;
-}
-
-__global__ void gather_counts_kernel(const uint32_t* counts, const int32_t* order, const uint32_t* valid,
-                                     uint32_t* tmp, int32_t n) {
-    const uint32_t nv = min(*valid, (uint32_t)n);
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
-        tmp[j] = (uint32_t)j < nv ? counts[order[j]] : 0u;
-}
-
-hipError_t launch_gather_counts(const uint32_t* counts, const int32_t* order, const uint32_t* valid, uint32_t* tmp,
-                                int32_t n, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(gather_counts_kernel, dim3(grid_for(n)), dim3(256), 0, s, counts, order, valid, tmp, n);
-    return hipGetLastError();
 }
 
 hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t s) {
@@ -806,14 +798,28 @@ __device__ __forceinline__ uint32_t block_sum_256(uint32_t v, uint32_t* s_w) {
     return s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* data, int32_t n, uint32_t* sums) {
+// Packed record words (the depth sort's values): record index in bits [0, ib),
+// entry count in bits [ib, 32), saturated at 2^(32-ib) - 1, in which case
+// counts[index] holds it (1 <= ib <= 31).
+__device__ __forceinline__ uint32_t packed_count(uint32_t v, int ib, const uint32_t* counts) {
+    const uint32_t c = v >> ib;
+    return c == (0xffffffffu >> ib) ? counts[v & ((1u << ib) - 1u)] : c;
+}
+
+// PACKED: data holds packed record words, the first *valid of them meaningful,
+// and the values scanned are their counts.
+template <bool PACKED>
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* data, int32_t n, uint32_t* sums,
+                                                                   const uint32_t* valid, const uint32_t* counts,
+                                                                   int ib) {
     __shared__ uint32_t s_w[4];
     const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    const int64_t nv = PACKED ? (int64_t)min(*valid, (uint32_t)n) : (int64_t)n;
     uint32_t v = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const int64_t i = base + k * kScanThreads + threadIdx.x;
-        if (i < n) v += data[i];
+        if (i < nv) v += PACKED ? packed_count(data[i], ib, counts) : data[i];
     }
     const uint32_t tot = block_sum_256(v, s_w);
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
@@ -847,19 +853,22 @@ __global__ __launch_bounds__(1024) void scan_sums_kernel(uint32_t* sums, int32_t
     if (threadIdx.x == 0) *total = (uint32_t)(s_carry > 0xffffffffull ? 0xffffffffull : s_carry);
 }
 
-// SCATTER: the exclusive prefix of element j goes to offsets[order[j]] (j < *valid) instead of data[j]
-template <bool SCATTER>
+// PACKED: the exclusive prefix of packed word j (j < *valid) goes to
+// offsets[index of word j] instead of back to data[j].
+template <bool PACKED>
 __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data, int32_t n, const uint32_t* sums,
-                                                                 const int32_t* order, const uint32_t* valid,
-                                                                 uint32_t* offsets) {
+                                                                 const uint32_t* valid, const uint32_t* counts,
+                                                                 int ib, uint32_t* offsets) {
     __shared__ uint32_t s_w[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t my = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
+    const int64_t nv = PACKED ? (int64_t)min(*valid, (uint32_t)n) : (int64_t)n;
+    uint32_t word[kScanItems], v[kScanItems];
     uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (my + k < n) ? data[my + k] : 0u;
+        word[k] = (my + k < nv) ? data[my + k] : 0u;
+        v[k] = (my + k < nv) ? (PACKED ? packed_count(word[k], ib, counts) : word[k]) : 0u;
         sum += v[k];
     }
     uint32_t incl = sum;
@@ -872,11 +881,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data,
     __syncthreads();
     uint32_t run = sums[blockIdx.x] + incl - sum;
     for (int k = 0; k < w; ++k) run += s_w[k];
-    const int64_t nv = SCATTER ? (int64_t)min(*valid, (uint32_t)n) : 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        if constexpr (SCATTER) {
-            if (my + k < nv) offsets[order[my + k]] = run;
+        if constexpr (PACKED) {
+            if (my + k < nv) offsets[word[k] & ((1u << ib) - 1u)] = run;
         } else {
             if (my + k < n) data[my + k] = run;
         }
@@ -892,24 +900,35 @@ hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_
         return hipGetLastError();
     }
     const int nb = (n + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch);
+    hipLaunchKernelGGL(scan_reduce_kernel<false>, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch, nullptr,
+                       nullptr, 0);
     hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, scratch, nb, total);
     hipLaunchKernelGGL(scan_down_kernel<false>, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch, nullptr,
-                       nullptr, nullptr);
+                       nullptr, 0, nullptr);
     return hipGetLastError();
 }
 
-hipError_t launch_build_scan_scatter(uint32_t* data, int32_t n, uint32_t* total, uint32_t* scratch,
-                                     const int32_t* order, const uint32_t* valid, uint32_t* offsets, hipStream_t s) {
+int packed_index_bits(int64_t n) {
+    int b = 1;
+    while (b < 31 && ((int64_t)1 << b) < n) ++b;
+    return b;
+}
+
+hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint32_t* valid, const uint32_t* counts,
+                                    int32_t index_bits, uint32_t* total, uint32_t* scratch, uint32_t* offsets,
+                                    hipStream_t s) {
+    if (index_bits < 1 || index_bits > 31 || (n > 0 && ((int64_t)1 << index_bits) < n)) return hipErrorInvalidValue;
     if (n <= 0) {
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, total, 0u, (int64_t)1);
         return hipGetLastError();
     }
     const int nb = (n + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch);
+    uint32_t* data = const_cast<uint32_t*>(words);   // read only in the packed form
+    hipLaunchKernelGGL(scan_reduce_kernel<true>, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch, valid, counts,
+                       index_bits);
     hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, scratch, nb, total);
-    hipLaunchKernelGGL(scan_down_kernel<true>, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch, order, valid,
-                       offsets);
+    hipLaunchKernelGGL(scan_down_kernel<true>, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch, valid, counts,
+                       index_bits, offsets);
     return hipGetLastError();
 }
 

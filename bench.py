@@ -124,11 +124,13 @@ def serving_leg(gr, workload: str, n_req: int, rate: float):
         res = sv.run_load(srv, ids, reqs, rate=rate)
         out[f"poisson_{int(rate)}"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items()
                                        if k != "images"}
-    sat = sv.run_load(srv, ids, sv.make_requests(scenes, n_req, seed=12), rate=None)
+    sat_reqs = sv.make_requests(scenes, n_req, seed=12)
+    sat = sv.run_load(srv, ids, sat_reqs, rate=None)
     out["saturated"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in sat.items() if k != "images"}
     out["server_stats"] = srv.stats()
     srv.close()
-    seq = sv.run_sequential(scenes, reqs[: max(50, n_req // 8)], precision=0)
+    # the same requests (a prefix of the saturated list), one plain FP32 render at a time
+    seq = sv.run_sequential(scenes, sat_reqs[: max(50, n_req // 4)], precision=0)
     out["sequential_plain_fp32"] = {k: round(v, 3) for k, v in seq.items()}
     out["speedup_vs_sequential"] = round(sat["views_per_s"] / seq["views_per_s"], 2)
     return out, scenes, reqs

@@ -231,7 +231,6 @@ hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t s) {
                         rd(&a.ellipse[(size_t)v * a.count + 64 * b + l], 16);
                         touch(&a.records[2 * (size_t)p], 32);
                         a.depth_keys[p] = 7u;
-                        a.order[p] = (int32_t)p;
                         ++p;
                     }
             }
@@ -245,9 +244,12 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
         if (a.records) {
             const uint32_t nvis = std::min<uint32_t>(*a.record_count, (uint32_t)(a.nviews * a.count));
             rd(a.records, (size_t)nvis * 32);
+            if (a.index_bits < 1 || a.index_bits > 31 || ((int64_t)1 << a.index_bits) < (int64_t)a.nviews * a.count)
+                std::abort();
             for (uint32_t q = 0; q < nvis; ++q) {
                 a.record_counts[q] = 3u;
                 a.record_masks[q] = 0xbull;   // 4 candidates, 3 of them pass
+                a.depth_words[q] = q | (3u << a.index_bits);
             }
             return;
         }
@@ -260,23 +262,27 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
     return hipSuccess;
 }
 
-hipError_t launch_gather_counts(const uint32_t* counts, const int32_t* order, const uint32_t* valid, uint32_t* tmp,
-                                int32_t n, hipStream_t s) {
-    run_or_record(s, [=] {
-        const uint32_t nv = std::min<uint32_t>(*valid, (uint32_t)n);
-        for (int32_t j = 0; j < n; ++j) tmp[j] = (uint32_t)j < nv ? counts[order[j]] : 0u;
-    });
-    return hipSuccess;
+int packed_index_bits(int64_t n) {
+    int b = 1;
+    while (b < 31 && ((int64_t)1 << b) < n) ++b;
+    return b;
 }
 
-hipError_t launch_build_scan_scatter(uint32_t* data, int32_t n, uint32_t* total, uint32_t* scratch,
-                                     const int32_t* order, const uint32_t* valid, uint32_t* offsets, hipStream_t s) {
+hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint32_t* valid, const uint32_t* counts,
+                                    int32_t index_bits, uint32_t* total, uint32_t* scratch, uint32_t* offsets,
+                                    hipStream_t s) {
+    if (index_bits < 1 || index_bits > 31 || (n > 0 && ((int64_t)1 << index_bits) < n)) return hipErrorInvalidValue;
     run_or_record(s, [=] {
         const uint32_t nv = std::min<uint32_t>(*valid, (uint32_t)std::max(n, 0));
+        const uint32_t esc = 0xffffffffu >> index_bits;
         uint32_t acc = 0;
-        for (int i = 0; i < n; ++i) {
-            if ((uint32_t)i < nv) offsets[order[i]] = acc;
-            acc += data[i];
+        for (uint32_t i = 0; i < nv; ++i) {
+            const uint32_t w = words[i];
+            const uint32_t idx = w & ((1u << index_bits) - 1u);
+            if (idx >= nv) std::abort();   // a word names a record that does not exist
+            const uint32_t c = (w >> index_bits) == esc ? counts[idx] : (w >> index_bits);
+            offsets[idx] = acc;
+            acc += c;
         }
         *total = acc;
         touch(scratch, (size_t)scan_scratch_words(n) * 4);
