@@ -62,6 +62,9 @@ __global__ __launch_bounds__(256) void sh_kernel(ShArgs a) {
     const int i = blockIdx.x * kShGroup + threadIdx.x;
     if (i >= a.count) return;
     const float4 p = a.position[i];
+    // one 4-byte (2-byte) load per coefficient and lane, each wave-coalesced; staging the
+    // group's contiguous block through LDS with 16-byte loads measured slower (289 vs 227 us
+    // at 6M Gaussians)
     const T* sh = static_cast<const T*>(a.sh) + (size_t)blockIdx.x * kShGroup * a.sh_count + threadIdx.x;
     float c[NC];
 #pragma unroll
@@ -1017,10 +1020,24 @@ hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
 constexpr int kRasterThreads = 256;
 constexpr int kRasterBatch = 32;
 
+// blend_alpha with the clamp of e to [-26, 0] as one v_med3_f32: the same value
+// for every e that is not NaN, and a NaN e gives alpha 0 in both forms.
+__device__ __forceinline__ float blend_alpha_dev(float e, float opacity) {
+    const float ec = __builtin_amdgcn_fmed3f(e, -26.0f, 0.0f);
+    const float al = opacity * exp2_poly(ec);
+    return (e >= kMinExponent) ? al : 0.0f;
+}
+
+// One staged entry: all of it sits at one LDS address + immediate offsets.
+struct __align__(16) RasterEntry {
+    float4 a;   // cx, cy, qa, qb
+    float4 b;   // qc, opacity, r, g
+    float c;    // b
+    float pad[3];
+};
+
 __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
-    __shared__ float4 s_a[kRasterThreads];   // cx, cy, qa, qb
-    __shared__ float4 s_b[kRasterThreads];   // qc, opacity, r, g
-    __shared__ float s_c[kRasterThreads];    // b
+    __shared__ RasterEntry s_e[kRasterThreads];
     __shared__ int s_alive[2][4];
 
     const int bid = blockIdx.x;
@@ -1058,24 +1075,25 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
                 const float2 c = a.clip_xy[gi];
                 const float4 q = a.conic[gi];
                 const float4 col = a.color[gi];
-                s_a[threadIdx.x] = make_float4(c.x, c.y, q.x * kHalfLog2e, q.y * kLog2e);
-                s_b[threadIdx.x] = make_float4(q.z * kHalfLog2e, col.w, col.x, col.y);
-                s_c[threadIdx.x] = col.z;
+                s_e[threadIdx.x].a = make_float4(c.x, c.y, q.x * kHalfLog2e, q.y * kLog2e);
+                s_e[threadIdx.x].b = make_float4(q.z * kHalfLog2e, col.w, col.x, col.y);
+                s_e[threadIdx.x].c = col.z;
             }
             __syncthreads();
             for (int b0 = 0; b0 < nr; b0 += kRasterBatch) {
                 const int b1 = min(b0 + kRasterBatch, nr);
                 for (int j = b0; j < b1; ++j) {
-                    const float4 ea = s_a[j];
+                    const RasterEntry& en = s_e[j];
+                    const float4 ea = en.a;
                     const float dx = pcx - ea.x, dy = pcy - ea.y;
-                    const float4 eb = s_b[j];
+                    const float4 eb = en.b;
                     const float e = blend_exponent(ea.z, ea.w, eb.x, dx, dy);
                     if (__ballot(e >= kMinExponent) != 0ull) {
-                        const float al = blend_alpha(e, eb.y);
+                        const float al = blend_alpha_dev(e, eb.y);
                         const float wgt = T * al;
                         C0 = fmaf(eb.z, wgt, C0);
                         C1 = fmaf(eb.w, wgt, C1);
-                        C2 = fmaf(s_c[j], wgt, C2);
+                        C2 = fmaf(en.c, wgt, C2);
                         T = T - wgt;
                     }
                 }
