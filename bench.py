@@ -100,6 +100,72 @@ def copy_calibration(mib: int):
                       "launches": 5}), flush=True)
 
 
+def stage_path_fps(gr, torch, scene, cams, W, H, frames):
+    """The reference's frame sequence (app/main.cpp:326-433) through the stage-level C-ABI on
+    the same scene and poses: set the camera, SH, projection, build (its K readback sizes the
+    list), the 44-bit sort of the (tile << 32 | depth, index) pairs, ranges, raster, wait."""
+    import ctypes as C
+    n, shc = scene["count"], 48
+    t = torch
+    dev = lambda a: t.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    pos, sr, col = dev(scene["position"]), dev(scene["scale_rot"]), dev(scene["color"])
+    sh = dev(gr.realign_sh(scene["sh"], n, shc))
+    xy = t.empty((n, 2), dtype=t.float32, device="cuda")
+    z = t.empty(n, dtype=t.float32, device="cuda")
+    ell = t.empty((n, 4), dtype=t.float32, device="cuda")
+    con = t.empty((n, 4), dtype=t.float32, device="cuda")
+    tx, ty, _ = gr.tile_grid(W, H)
+    rng = t.empty(2 * tx * ty, dtype=t.int32, device="cuda")
+    img = t.empty((H, W, 4), dtype=t.uint8, device="cuda")
+    t.cuda.synchronize()
+    ctx = gr.Context(t.cuda.current_stream().cuda_stream)
+    ga = gr.GlobalArgs()
+    ga.position, ga.scale_and_rotation, ga.color = pos.data_ptr(), sr.data_ptr(), col.data_ptr()
+    ga.spherical_harmonics_degree, ga.spherical_harmonics_count = 3, shc
+    ga.spherical_harmonics = sh.data_ptr()
+    ga.position_clip_xy, ga.position_clip_z = xy.data_ptr(), z.data_ptr()
+    ga.conic, ga.screen_ellipse = con.data_ptr(), ell.data_ptr()
+    ga.splat_count, ga.back_buffer, ga.tile_range = n, img.data_ptr(), rng.data_ptr()
+    ga.width, ga.height, ga.sh_precision = W, H, 0
+    cap = 0
+    keys = vals = None
+    temp = gr.SortTemp()
+
+    def frame(i):
+        nonlocal cap, keys, vals
+        ga.camera_data = cams[i % len(cams)]
+        ctx.set_global_args(ga)
+        ctx.sh(n)
+        ctx.project(n)
+        while True:
+            ctx.set_tile_list_args(gr.TileListArgs(keys[0].data_ptr() if keys else None,
+                                                   vals[0].data_ptr() if vals else None, 0, cap))
+            size, k = ctx.build(cap)
+            if k <= cap:
+                break
+            cap = int(k * 1.25)   # grow-only, as the reference's doubling but sized from K
+            keys = [t.empty(cap, dtype=t.int64, device="cuda") for _ in range(2)]
+            vals = [t.empty(cap, dtype=t.int32, device="cuda") for _ in range(2)]
+        kb = gr.DoubleBufferU64((C.c_void_p * 2)(keys[0].data_ptr(), keys[1].data_ptr()), 0)
+        vb = gr.DoubleBufferI32((C.c_void_p * 2)(vals[0].data_ptr(), vals[1].data_ptr()), 0)
+        ctx.sort(k, kb, vb, temp)
+        ctx.set_tile_list_args(gr.TileListArgs(keys[kb.selector].data_ptr(), vals[kb.selector].data_ptr(), k, cap))
+        ctx.ranges(k)
+        ctx.raster()
+        ctx.sync()
+
+    for i in range(2):
+        frame(i)
+    t0 = time.perf_counter()
+    for i in range(frames):
+        frame(i)
+    fps = frames / (time.perf_counter() - t0)
+    t.cuda.synchronize()
+    temp.free()
+    ctx.close()
+    return fps
+
+
 def serving_leg(gr, workload: str, n_req: int, rate: float):
     """Continuous-batching server on the serving workload vs sequential plain renders."""
     from importlib import import_module
@@ -206,6 +272,11 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.traffic_profile),
             "algorithmic_bytes_per_launch": int(16.0 * K), "avg_launch_us": round(pass_avg_ms * 1e3, 2),
             "launches_per_frame": passes}
+    # the reference's frame sequence on the same kernels (stage-level C-ABI), for comparison
+    r.close()
+    stage_fps = stage_path_fps(gr, torch, scene, cams, W, H, min(args.steps, 16))
+    r = gr.Renderer(0)
+    r.upload(scene)
     serving = None
     if args.serving != "none":
         r.close()
@@ -284,6 +355,7 @@ def main():
         "stage_gbs": {k.replace("_ms", ""): round(v, 1) for k, v in gbs.items()},
         "tile_list_entries": int(K),
         "host_output_frames_per_s": round(host_fps, 2),
+        "stage_path_frames_per_s": round(stage_fps, 2),
         "roofline": roof,
         "cpu_baseline": cpu,
         "serving": serving,
