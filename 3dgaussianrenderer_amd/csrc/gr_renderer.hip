@@ -11,9 +11,10 @@
 //   -> 32-bit onesweep sort of the (depth word, record | count) pairs -> the
 //      counts, now in depth order, are scanned (K) and scattered back as
 //      per-record output offsets
-//   -> emit (records in index order): each record's entries at its offset,
-//      + tile digit histograms
-//   -> onesweep sort of the tile bits only (bits [32, 32 + tile bits))
+//   -> emit (records in index order): each record's entries, as 8-byte
+//      (Gaussian index << 32 | tile id) words, at its offset, + tile digit
+//      histograms
+//   -> keys-only onesweep sort of the tile bits (bits [0, tile bits))
 //   -> ranges -> raster -> image and K readback.
 // Entries sit in (depth word, Gaussian index) order before the last sort, so
 // the stable tile-bit sort yields exactly the order of the reference's
