@@ -29,13 +29,15 @@ namespace gr {
 
 // Partition shapes (lanes x pairs per lane) per sort: the 64-bit sorts (tile
 // lists) and the 32-bit depth sort of the visible Gaussians; measured on the
-// 6M-Gaussian 1080p frame (512 x 12 and 1024 x 8 fastest respectively; smaller
-// partitions were slower for both).  Build-time overrides exist for sweeps.
+// 6M-Gaussian 1080p frame (512 x 13 and 1024 x 8 fastest respectively; smaller
+// partitions were slower for both).  The look-back reads 4 predecessors per
+// step (16 or 32 per step measured slower).  Build-time overrides exist for
+// sweeps.
 #ifndef GR_SORT_THREADS
 #define GR_SORT_THREADS 512
 #endif
 #ifndef GR_SORT_ITEMS
-#define GR_SORT_ITEMS 12
+#define GR_SORT_ITEMS 13
 #endif
 #ifndef GR_DSORT_THREADS
 #define GR_DSORT_THREADS 1024
@@ -44,7 +46,7 @@ namespace gr {
 #define GR_DSORT_ITEMS 8
 #endif
 #ifndef GR_SORT_LOOKBACK
-#define GR_SORT_LOOKBACK 16
+#define GR_SORT_LOOKBACK 4
 #endif
 template <int THREADS, int ITEMS>
 struct SortShape {
