@@ -625,25 +625,43 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
                 for (int k = 0; k < a.tile_passes; ++k) atomicAdd(&s_hist[k][(tile >> (8 * k)) & 255u], 1u);
         }
     };
-    for (int first = (blockIdx.x * W + w) * kBuildBatch; first < limit; first += gridDim.x * W * kBuildBatch) {
+    // the next batch's record data is loaded while the current batch is written out
+    const int stride = gridDim.x * W * kBuildBatch;
+    int first = (blockIdx.x * W + w) * kBuildBatch;
+    float4 n_r0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), n_r1 = n_r0;
+    uint64_t n_mask = 0ull;
+    uint32_t n_off = 0u, n_cnt = 0u;
+    auto fetch = [&](int f) {
+        const int q = f + lane;
+        if (f < limit && q < limit) {
+            n_r0 = a.records[2 * (size_t)q];
+            n_r1 = a.records[2 * (size_t)q + 1];
+            n_mask = a.record_masks[q];
+            n_off = a.record_offsets[q];
+            n_cnt = a.record_counts[q];
+        }
+    };
+    fetch(first);
+    for (; first < limit; first += stride) {
         const int pos = first + lane;
+        const float4 r0 = n_r0, r1 = n_r1;
+        const uint64_t mask = n_mask;
+        const uint32_t off = n_off, rcnt = n_cnt;
+        fetch(first + stride);
         int ecnt = 0, big = 0;
         if (pos < limit) {
-            const float4 r0 = a.records[2 * (size_t)pos];
-            const float4 r1 = a.records[2 * (size_t)pos + 1];
             const int pg = __float_as_int(r1.w);
             const int view = pg / a.count;
             const ViewDesc& vd = view_of(view, a.v0, a.views);
             const Footprint f = footprint(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, vd.geom);
             const int cand = (f.nx > 0 && f.ny > 0) ? f.nx * f.ny : 0;
-            const uint32_t off = a.record_offsets[pos];
             s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, vd.geom.tiles_x);
             s_tb[w][lane] = vd.tile_base;
-            s_mask[w][lane] = a.record_masks[pos];
+            s_mask[w][lane] = mask;
             s_off[w][lane] = off;
             s_gi[w][lane] = pg - view * a.count;
             if (cand <= 64) {
-                ecnt = (int)a.record_counts[pos];
+                ecnt = (int)rcnt;
             } else {
                 big = cand;
                 s_cand[w][lane] = cand;
