@@ -142,7 +142,10 @@ gr_status gr_context_synchronize(gr_context* ctx) {
 gr_status gr_timer_create(gr_timer** out) {
     if (!out) return fail(GR_ERR_INVALID_ARGUMENT, "gr_timer_create: out is NULL");
     auto* t = new gr_timer();
-    if (hipEventCreate(&t->start) != hipSuccess || hipEventCreate(&t->stop) != hipSuccess) {
+    // timing only (the launchers' results reach the host through the caller's own copies and
+    // synchronisation), so the events skip the system-scope cache fence
+    if (hipEventCreateWithFlags(&t->start, hipEventDisableSystemFence) != hipSuccess ||
+        hipEventCreateWithFlags(&t->stop, hipEventDisableSystemFence) != hipSuccess) {
         delete t;
         return fail(GR_ERR_HIP, "hipEventCreate failed");
     }

@@ -166,7 +166,9 @@ GR_API gr_status gr_context_destroy(gr_context* ctx);
 GR_API gr_status gr_context_synchronize(gr_context* ctx);
 
 /* CudaTimer: start/stop events on the context stream; elapsed_ms waits for
- * the stop event (core/utilities.h:173-182) -- the stage launchers never do. */
+ * the stop event (core/utilities.h:173-182) -- the stage launchers never do.
+ * The events are for timing only: they do not make device writes visible to
+ * the host (use gr_context_synchronize or a copy for that). */
 GR_API gr_status gr_timer_create(gr_timer** out);
 GR_API gr_status gr_timer_destroy(gr_timer* timer);
 GR_API gr_status gr_timer_elapsed_ms(gr_timer* timer, float* out_ms);
