@@ -92,7 +92,6 @@ struct gr_renderer {
     int device = 0;
     gr_context* ctx = nullptr;
     hipEvent_t ev[9] = {};
-    hipEvent_t pass_ev[2] = {};     // around the tile-bit sort's onesweep passes
     int32_t last_passes = 0;
     int32_t n = 0, degree = 0, sh_count = 3, precision = GR_PRECISION_FP32;
     DevBuf<float4> position, scale_rot, color, conic, ellipse;
@@ -138,7 +137,6 @@ gr_status gr_renderer_create(gr_renderer** out, int32_t device, void* stream) {
     // separates); the last one, after the image and K copies, keeps it: the host reads those.
     for (int k = 0; k < 8; ++k) R_HIP(hipEventCreateWithFlags(&r->ev[k], hipEventDisableSystemFence));
     R_HIP(hipEventCreate(&r->ev[8]));
-    for (auto& e : r->pass_ev) R_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     *out = r;
     return GR_OK;
 }
@@ -157,7 +155,6 @@ gr_status gr_renderer_destroy(gr_renderer* r) {
     r->records.release(); r->rec_counts.release(); r->rec_offsets.release(); r->rec_masks.release();
     if (r->sort_temp) (void)hipFree(r->sort_temp);
     for (auto& e : r->ev) if (e) (void)hipEventDestroy(e);
-    for (auto& e : r->pass_ev) if (e) (void)hipEventDestroy(e);
     gr_context_destroy(r->ctx);
     delete r;
     return GR_OK;
@@ -365,16 +362,20 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     R_HIP(radix_sort_prepare(capacity, r->sort_temp, r->sort_temp_bytes, &b.tile_hist, stream));
     b.tile_passes = (tile_bits + 7) / 8;
     R_HIP(launch_build_emit(b, stream));
-    R_HIP(hipEventRecord(r->ev[5], stream));
-    // tile bits
+    // tile bits: the sort records ev[5] after its histogram scan and ev[6] after its passes, so
+    // the interval is the passes alone (each event leaves the GPU idle for a few microseconds;
+    // the same two serve the stage split and the pass timing)
     int32_t sel = 0;
     {
         uint64_t* kb[2] = {r->entries[0].p, r->entries[1].p};
         R_HIP(radix_sort_keys_prepared(kb, &sel, capacity, k_total, 0, tile_bits, r->sort_temp, r->sort_temp_bytes,
-                                       stream, r->pass_ev[0], r->pass_ev[1]));
+                                       stream, r->ev[5], r->ev[6]));
         r->last_passes = capacity > 1 ? (tile_bits + 7) / 8 : 0;
+        if (r->last_passes == 0) {   // no passes ran: the events still bracket the (empty) sort
+            R_HIP(hipEventRecord(r->ev[5], stream));
+            R_HIP(hipEventRecord(r->ev[6], stream));
+        }
     }
-    R_HIP(hipEventRecord(r->ev[6], stream));
     r->sel = sel;
     R_HIP(launch_fill_i32(r->tile_range.p, -1, 2 * (int64_t)tiles, stream));
     {
@@ -419,14 +420,14 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     r->last_w = width;
     r->last_h = height;
     if (stats) {
-        // events: 0 SH 1 project 2 compaction + count 3 depth sort 4 offsets + emit
-        // 5 tile-bit sort 6 ranges + raster 7
+        // events: 0 SH 1 project 2 compaction + count 3 depth sort 4 offsets + emit + the tile
+        // sort's histogram scan 5 tile-bit sort passes 6 ranges + raster 7
         float t[8];
         for (int k = 0; k < 7; ++k) R_HIP(hipEventElapsedTime(&t[k], r->ev[k], r->ev[k + 1]));
         R_HIP(hipEventElapsedTime(&t[7], r->ev[0], r->ev[7]));
         stats->sh_eval_ms = t[0];
         stats->project_ms = t[1];
-        stats->build_tile_ms = t[2] + t[4];   // compaction, count, offsets, emit
+        stats->build_tile_ms = t[2] + t[4];   // compaction, count, offsets, emit (+ histogram scan)
         stats->sort_ms = t[3] + t[5];         // depth sort + tile-bit sort
         stats->raster_ms = t[6];              // ranges + raster
         stats->total_ms = t[7];
@@ -442,7 +443,7 @@ gr_status gr_renderer_kernel_times(gr_renderer* r, gr_kernel_times* out) {
     out->tile_list_size = r->last_k;
     if (r->last_passes > 0) {
         float ms = 0.0f;
-        R_HIP(hipEventElapsedTime(&ms, r->pass_ev[0], r->pass_ev[1]));
+        R_HIP(hipEventElapsedTime(&ms, r->ev[5], r->ev[6]));
         out->tile_sort_pass_ms = ms;
     }
     return GR_OK;
