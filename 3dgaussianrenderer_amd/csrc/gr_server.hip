@@ -17,11 +17,18 @@
 // Tile ids are global over the batch's views, so one sort serves all views,
 // and within a tile entries keep (depth word, index) order.
 // Between A and B the host reads K and grows the slot's tile-list buffers
-// within the HBM budget (exact admission).  Both phases are captured once
-// per (scene, batch shape, workspace) as hipGraphs and replayed; the camera
-// block is the only per-batch input and travels through a memcpy node from
-// pinned memory.  Several slots run concurrently, so batches of different
-// scenes (or of one scene) overlap on the device.
+// within the HBM budget (exact admission).  A phase whose (scene, batch
+// shape, workspace) repeats is captured as a hipGraph and replayed from then
+// on (a shape seen once runs eagerly: capture and instantiation cost more
+// host time than the launches they save); the camera block is the only
+// per-batch input and travels through a memcpy node from pinned memory.
+// Several slots run concurrently, so batches of different scenes (or of one
+// scene) overlap on the device.
+// Copy-out threads move finished images from a slot's pinned staging into
+// the callers' buffers and post the completions, in chunks spread over the
+// threads; each slot has two staging buffers, so its next batch runs while
+// the last one is copied out.  The scheduler thread never touches image
+// bytes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -103,7 +110,7 @@ struct Dev {
 struct Pinned {
     void* p = nullptr;
     size_t bytes = 0;
-    gr_status ensure(size_t need) {
+    gr_status ensure(size_t need, bool* grew = nullptr) {
         if (need <= bytes && p) return GR_OK;
         const size_t want = std::max<size_t>(need + need / 4, 256);
         if (p) S_HIP(hipHostFree(p));
@@ -111,6 +118,7 @@ struct Pinned {
         bytes = 0;
         S_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
         bytes = want;
+        if (grew) *grew = true;
         return GR_OK;
     }
     void release() {
@@ -152,7 +160,8 @@ struct Workspace {
     Dev ranges;                       // 2 * tiles
     Dev images;                       // sum of W*H*4
     Dev views;                        // ViewDesc[V]
-    Pinned views_host, k_host, img_host;
+    Pinned views_host, k_host;
+    Pinned img_host[2];               // image staging, alternating between batches
     uint64_t generation = 0;          // bumps when any device pointer changes
     std::vector<Dev*> devs() {
         return {&color, &xy, &z, &ell, &con, &vis_bits, &vis_counts, &dkeys0, &dkeys1, &dorder0, &dorder1,
@@ -182,8 +191,28 @@ struct Slot {
     uint32_t k = 0;
     uint32_t capacity = 0;
     double start_ms = 0.0;
+    int img_cur = 0;                  // staging buffer of the batch in flight
     std::map<GraphKey, hipGraphExec_t> graphs;
+    std::map<GraphKey, int> seen;     // phase shapes run eagerly so far
 };
+
+// One view's copy-out: the chunks still to copy and the completion posted after the last.
+struct ViewCopy {
+    gr_completion c{};
+    int32_t scene = -1;
+    std::atomic<int> left{0};
+    std::atomic<int>* staging_busy = nullptr;   // views of the staging buffer still copying
+};
+
+struct CopyChunk {
+    const uint8_t* src;
+    uint8_t* dst;
+    size_t bytes;
+    std::shared_ptr<ViewCopy> view;
+};
+
+constexpr int kCopyThreads = 4;
+constexpr size_t kCopyChunk = size_t(2) << 20;
 
 }  // namespace
 
@@ -200,7 +229,15 @@ struct gr_server {
     uint64_t next_request = 1;
     std::atomic<int64_t> resident{0}, workspace{0};
     std::atomic<int64_t> peak{0}, batches{0}, views{0}, graph_launches{0}, budget_waits{0};
+    std::atomic<int64_t> graph_captures{0};
+    std::atomic<int64_t> capture_us{0}, copy_us{0};
     std::vector<Slot> slots;
+    std::unique_ptr<std::atomic<int>[]> staging_busy;   // [slot][2]
+    std::mutex copy_mu;
+    std::condition_variable copy_cv;
+    std::deque<CopyChunk> copy_q;
+    std::vector<std::thread> copiers;
+    bool copy_stop = false;                          // under copy_mu
     std::map<int32_t, std::deque<Pending>> queues;   // scheduler-owned
     std::map<int32_t, int> batch_limit;              // scheduler-owned: one-shot cap on a scene's next batch
     std::thread worker;
@@ -248,6 +285,37 @@ void complete(gr_server* s, Slot& sl, int32_t status) {
     auto it = s->scenes.find(sl.scene);
     if (it != s->scenes.end()) it->second->inflight -= (int)sl.batch.size();
     s->cv_done.notify_all();
+}
+
+// Posts a view whose image has been copied out; its staging buffer may be reused afterwards.
+void post_view(gr_server* s, ViewCopy& v) {
+    v.c.finish_ms = s->now_ms();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->done.push_back(v.c);
+        auto it = s->scenes.find(v.scene);
+        if (it != s->scenes.end()) it->second->inflight -= 1;
+    }
+    v.staging_busy->fetch_sub(1);
+    s->cv_done.notify_all();
+}
+
+// Copy-out thread: chunks in submission order until stopped with an empty queue.
+void copier(gr_server* s) {
+    for (;;) {
+        CopyChunk ch;
+        {
+            std::unique_lock<std::mutex> lk(s->copy_mu);
+            s->copy_cv.wait(lk, [&] { return s->copy_stop || !s->copy_q.empty(); });
+            if (s->copy_q.empty()) return;
+            ch = std::move(s->copy_q.front());
+            s->copy_q.pop_front();
+        }
+        const auto t = std::chrono::steady_clock::now();
+        std::memcpy(ch.dst, ch.src, ch.bytes);
+        s->copy_us += (int64_t)std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t).count();
+        if (ch.view->left.fetch_sub(1) == 1) post_view(s, *ch.view);
+    }
 }
 
 int64_t per_view_bytes(const Scene& sc) { return (int64_t)sc.n * (16 + 8 + 4 + 16 + 16); }
@@ -328,7 +396,7 @@ gr_status ensure_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
         S_CHECK(n.dev->ensure((size_t)n.bytes, s->workspace, &grew));
     S_CHECK(w.views_host.ensure(V * sizeof(ViewDesc)));
     S_CHECK(w.k_host.ensure(64));
-    S_CHECK(w.img_host.ensure((size_t)sl.img_bytes));
+    S_CHECK(w.img_host[sl.img_cur].ensure((size_t)sl.img_bytes));
     if (grew) ++w.generation;
     s->note_peak();
     return GR_OK;
@@ -472,7 +540,6 @@ gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
     ra.v0 = vhost[0];
     ra.views = vdev;
     S_HIP(launch_raster(ra, sl.stream));
-    S_HIP(hipMemcpyAsync(w.img_host.p, w.images.p, (size_t)sl.img_bytes, hipMemcpyDeviceToHost, sl.stream));
     return GR_OK;
 }
 
@@ -493,6 +560,7 @@ GraphKey graph_key(const Slot& sl, bool phase_b) {
 void drop_graphs(Slot& sl) {
     for (auto& kv : sl.graphs) (void)hipGraphExecDestroy(kv.second);
     sl.graphs.clear();
+    sl.seen.clear();
 }
 
 gr_status run_phase(gr_server* s, Slot& sl, const Scene& sc, bool phase_b) {
@@ -501,6 +569,12 @@ gr_status run_phase(gr_server* s, Slot& sl, const Scene& sc, bool phase_b) {
     const GraphKey key = graph_key(sl, phase_b);
     auto it = sl.graphs.find(key);
     if (it == sl.graphs.end()) {
+        // a shape seen for the first time runs eagerly; its first repeat is captured for the
+        // replays after it
+        if (sl.seen.size() > 1024) sl.seen.clear();
+        if (sl.seen[key]++ == 0) return enqueue();
+        sl.seen.erase(key);
+        const auto t0 = std::chrono::steady_clock::now();
         S_HIP(hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal));
         gr_status st = enqueue();
         hipGraph_t g = nullptr;
@@ -516,6 +590,9 @@ gr_status run_phase(gr_server* s, Slot& sl, const Scene& sc, bool phase_b) {
         if (e != hipSuccess) return sfail(GR_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         if (sl.graphs.size() > 64) drop_graphs(sl);
         it = sl.graphs.emplace(key, ex).first;
+        ++s->graph_captures;
+        s->capture_us +=
+            (int64_t)std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
     }
     S_HIP(hipGraphLaunch(it->second, sl.stream));
     ++s->graph_launches;
@@ -601,6 +678,12 @@ bool start_batch(gr_server* s, Slot& sl) {
     // a slot waiting for memory goes first: new batches would take the memory it waits for
     for (const Slot& o : s->slots)
         if (o.state == SlotState::WaitMemory) return false;
+    // a staging buffer whose images are no longer being copied out
+    std::atomic<int>* busy = &s->staging_busy[2 * (&sl - s->slots.data())];
+    if (busy[sl.img_cur].load() != 0) {
+        if (busy[sl.img_cur ^ 1].load() != 0) return false;
+        sl.img_cur ^= 1;
+    }
     if (!form_batch(s, sl)) return false;
     Scene* sc = scene_ptr(s, sl.scene);
     if (!sc) {
@@ -774,7 +857,11 @@ bool start_phase_b(gr_server* s, Slot& sl) {
     }
     s->note_peak();
     if (grew) ++w.generation;
-    if (run_phase(s, sl, *sc, true) != GR_OK || hipEventRecord(sl.ev_b, sl.stream) != hipSuccess) {
+    // the image copy stays out of the graph: the staging buffer alternates between batches
+    if (run_phase(s, sl, *sc, true) != GR_OK ||
+        hipMemcpyAsync(w.img_host[sl.img_cur].p, w.images.p, (size_t)sl.img_bytes, hipMemcpyDeviceToHost,
+                       sl.stream) != hipSuccess ||
+        hipEventRecord(sl.ev_b, sl.stream) != hipSuccess) {
         complete(s, sl, GR_ERR_HIP);
         sl.batch.clear();
         sl.state = SlotState::Idle;
@@ -784,18 +871,63 @@ bool start_phase_b(gr_server* s, Slot& sl) {
     return true;
 }
 
+// Hands the batch's images to the copy-out threads (views without an output buffer complete
+// here) and frees the slot; the slot's other staging buffer serves its next batch.
 void finish_batch(gr_server* s, Slot& sl) {
+    std::atomic<int>* busy = &s->staging_busy[2 * (&sl - s->slots.data()) + sl.img_cur];
+    const uint8_t* staging = sl.ws.img_host[sl.img_cur].as<uint8_t>();
+    std::vector<CopyChunk> chunks;
+    std::vector<gr_completion> now;
     int64_t off = 0;
     for (const Pending& p : sl.batch) {
         const size_t bytes = (size_t)p.req.width * p.req.height * 4;
-        if (p.req.out_rgba8) std::memcpy(p.req.out_rgba8, sl.ws.img_host.as<uint8_t>() + off, bytes);
+        gr_completion c{};
+        c.request_id = p.id;
+        c.user_tag = p.req.user_tag;
+        c.status = GR_OK;
+        c.batch_views = (int32_t)sl.batch.size();
+        c.submit_ms = p.submit_ms;
+        c.start_ms = sl.start_ms;
+        c.tile_list_size = sl.k;
+        if (!p.req.out_rgba8) {
+            c.finish_ms = s->now_ms();
+            now.push_back(c);
+        } else {
+            auto v = std::make_shared<ViewCopy>();
+            v->c = c;
+            v->scene = sl.scene;
+            v->staging_busy = busy;
+            const size_t n = (bytes + kCopyChunk - 1) / kCopyChunk;
+            v->left.store((int)n);
+            busy->fetch_add(1);
+            for (size_t k = 0; k < n; ++k) {
+                const size_t at = k * kCopyChunk;
+                chunks.push_back({staging + off + at, p.req.out_rgba8 + at, std::min(kCopyChunk, bytes - at), v});
+            }
+        }
         off += (int64_t)bytes;
     }
     ++s->batches;
     s->views += (int64_t)sl.batch.size();
-    complete(s, sl, GR_OK);
+    if (!now.empty()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (const gr_completion& c : now) s->done.push_back(c);
+            auto it = s->scenes.find(sl.scene);
+            if (it != s->scenes.end()) it->second->inflight -= (int)now.size();
+        }
+        s->cv_done.notify_all();
+    }
+    if (!chunks.empty()) {
+        {
+            std::lock_guard<std::mutex> lk(s->copy_mu);
+            for (CopyChunk& ch : chunks) s->copy_q.push_back(std::move(ch));
+        }
+        s->copy_cv.notify_all();
+    }
     sl.batch.clear();
     sl.state = SlotState::Idle;
+    sl.img_cur ^= 1;
 }
 
 void scheduler(gr_server* s) {
@@ -846,11 +978,14 @@ gr_status gr_server_create(gr_server** out, const gr_server_config* cfg) {
     S_HIP(hipGetDevice(&s->device));
     s->t0 = std::chrono::steady_clock::now();
     s->slots.resize(cfg->slots);
+    s->staging_busy.reset(new std::atomic<int>[2 * cfg->slots]);
+    for (int i = 0; i < 2 * cfg->slots; ++i) s->staging_busy[i].store(0);
     for (Slot& sl : s->slots) {
         S_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
         S_HIP(hipEventCreateWithFlags(&sl.ev_a, hipEventDisableTiming));
         S_HIP(hipEventCreateWithFlags(&sl.ev_b, hipEventDisableTiming));
     }
+    for (int i = 0; i < kCopyThreads; ++i) s->copiers.emplace_back(copier, s);
     s->worker = std::thread(scheduler, s);
     *out = s;
     return GR_OK;
@@ -861,6 +996,12 @@ gr_status gr_server_destroy(gr_server* s) {
     s->stop = true;
     s->cv_submit.notify_all();
     if (s->worker.joinable()) s->worker.join();
+    {
+        std::lock_guard<std::mutex> lk(s->copy_mu);
+        s->copy_stop = true;
+    }
+    s->copy_cv.notify_all();
+    for (std::thread& t : s->copiers) t.join();   // drains the copy queue first
     (void)hipSetDevice(s->device);
     for (Slot& sl : s->slots) {
         (void)hipStreamSynchronize(sl.stream);
@@ -870,7 +1011,8 @@ gr_status gr_server_destroy(gr_server* s) {
             d->release(s->workspace);
         w.views_host.release();
         w.k_host.release();
-        w.img_host.release();
+        w.img_host[0].release();
+        w.img_host[1].release();
         (void)hipEventDestroy(sl.ev_a);
         (void)hipEventDestroy(sl.ev_b);
         (void)hipStreamDestroy(sl.stream);
@@ -1002,6 +1144,9 @@ gr_status gr_server_get_stats(gr_server* s, gr_server_stats* o) {
     o->views = s->views.load();
     o->graph_launches = s->graph_launches.load();
     o->budget_waits = s->budget_waits.load();
+    o->graph_captures = s->graph_captures.load();
+    o->capture_ms = (double)s->capture_us.load() / 1000.0;
+    o->copy_out_ms = (double)s->copy_us.load() / 1000.0;
     return GR_OK;
 }
 

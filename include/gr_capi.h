@@ -349,7 +349,9 @@ GR_API gr_status gr_renderer_kernel_times(gr_renderer* r, gr_kernel_times* out);
  * scheduler thread forms batches of pending requests of one scene and renders
  * each batch with ONE launch per stage (views fused: SH and projection read
  * the scene once for all views, the tile lists of all views go through one
- * sort), optionally replaying hipGraph-captured stage sequences.  Up to
+ * sort), optionally replaying hipGraph-captured stage sequences (a batch
+ * shape is captured when it repeats).  Images reach the callers' buffers
+ * through copy-out threads, which post each completion after its copy.  Up to
  * `slots` batches are in flight on their own streams, so requests that
  * arrive while others run are admitted as soon as a slot and HBM budget are
  * free, and their work overlaps on the device.  Memory admission: resident
@@ -364,7 +366,7 @@ typedef struct gr_server_config {
     int32_t device;               /* -1: current device */
     int32_t slots;                /* batches in flight (>= 1) */
     int32_t max_views_per_batch;  /* >= 1 */
-    int32_t use_graphs;           /* capture + replay stage sequences with hipGraph */
+    int32_t use_graphs;           /* capture + replay repeated stage sequences with hipGraph */
     int64_t hbm_budget_bytes;     /* resident scenes + workspaces; 0 = no limit */
 } gr_server_config;
 
@@ -397,6 +399,9 @@ typedef struct gr_server_stats {
     int64_t views;                /* views rendered in them */
     int64_t graph_launches;
     int64_t budget_waits;         /* times a batch waited for memory */
+    int64_t graph_captures;       /* phase shapes captured as graphs (on their first repeat) */
+    double capture_ms;            /* scheduler time spent capturing and instantiating graphs */
+    double copy_out_ms;           /* copy-out thread time moving images into callers' buffers */
 } gr_server_stats;
 
 GR_API gr_status gr_server_create(gr_server** out, const gr_server_config* config);
