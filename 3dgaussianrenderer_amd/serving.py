@@ -148,3 +148,41 @@ def run_sequential(scenes, requests, precision: int = 0):
     for r in rs.values():
         r.close()
     return {"views_per_s": len(requests) / wall, "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99)}
+
+
+def main(argv=None) -> int:
+    """Load generator: one workload against a Server, one JSON line of results.
+
+    python3 -m 3dgaussianrenderer_amd.serving --workload c4 --requests 600 --rate 0
+    (rate 0: every request submitted at once, the saturated case)."""
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description=main.__doc__.splitlines()[0])
+    ap.add_argument("--workload", default="c4", choices=["c4", "c5"])
+    ap.add_argument("--requests", type=int, default=600)
+    ap.add_argument("--rate", type=float, default=0.0, help="Poisson arrivals per second; 0 = all at once")
+    ap.add_argument("--slots", type=int, default=3)
+    ap.add_argument("--max-views", type=int, default=16)
+    ap.add_argument("--graphs", type=int, default=1)
+    ap.add_argument("--precision", type=int, default=1, help="0 FP32, 1 FP16, 2 BF16 storage")
+    ap.add_argument("--budget-gib", type=float, default=0.0)
+    ap.add_argument("--seed", type=int, default=12)
+    args = ap.parse_args(argv)
+    scenes = make_scenes(args.workload)
+    srv = capi.Server(slots=args.slots, max_views=args.max_views, use_graphs=bool(args.graphs),
+                      budget_bytes=int(args.budget_gib * (1 << 30)))
+    ids = [srv.add_scene(sc, args.precision) for sc in scenes]
+    run_load(srv, ids, make_requests(scenes, 64, seed=99), rate=None)   # warm-up
+    time.sleep(0.5)                                                      # marks the run's start in a trace
+    res = run_load(srv, ids, make_requests(scenes, args.requests, seed=args.seed), rate=args.rate or None)
+    out = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items() if k != "images"}
+    out.update(workload=args.workload, slots=args.slots, graphs=bool(args.graphs), precision=args.precision,
+               server_stats=srv.stats())
+    srv.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
