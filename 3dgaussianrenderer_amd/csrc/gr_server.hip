@@ -211,7 +211,10 @@ struct CopyChunk {
     std::shared_ptr<ViewCopy> view;
 };
 
-constexpr int kCopyThreads = 4;
+#ifndef GR_COPY_THREADS
+#define GR_COPY_THREADS 4   // build-time override for sweeps
+#endif
+constexpr int kCopyThreads = GR_COPY_THREADS;
 constexpr size_t kCopyChunk = size_t(2) << 20;
 
 }  // namespace
