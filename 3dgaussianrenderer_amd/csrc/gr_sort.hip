@@ -48,16 +48,24 @@ namespace gr {
 #ifndef GR_SORT_LOOKBACK
 #define GR_SORT_LOOKBACK 4
 #endif
-template <int THREADS, int ITEMS>
+// waves per SIMD the compiler is asked to fit (register budget); 0 = its own choice
+#ifndef GR_SORT_WPE
+#define GR_SORT_WPE 0
+#endif
+#ifndef GR_DSORT_WPE
+#define GR_DSORT_WPE 0
+#endif
+template <int THREADS, int ITEMS, int WPE = 0>
 struct SortShape {
     static constexpr int kThreads = THREADS;
     static constexpr int kWaves = THREADS / 64;
     static constexpr int kItems = ITEMS;
     static constexpr int kTile = THREADS * ITEMS;   // pairs per partition
+    static constexpr int kWavesPerEU = WPE;
     static_assert(THREADS % 64 == 0 && THREADS >= 256, "one lane per digit in the look-back");
 };
-template <typename KeyT> struct ShapeOf { using type = SortShape<GR_SORT_THREADS, GR_SORT_ITEMS>; };
-template <> struct ShapeOf<uint32_t> { using type = SortShape<GR_DSORT_THREADS, GR_DSORT_ITEMS>; };
+template <typename KeyT> struct ShapeOf { using type = SortShape<GR_SORT_THREADS, GR_SORT_ITEMS, GR_SORT_WPE>; };
+template <> struct ShapeOf<uint32_t> { using type = SortShape<GR_DSORT_THREADS, GR_DSORT_ITEMS, GR_DSORT_WPE>; };
 constexpr int kMinTile = GR_SORT_THREADS * GR_SORT_ITEMS < GR_DSORT_THREADS * GR_DSORT_ITEMS
                              ? GR_SORT_THREADS * GR_SORT_ITEMS
                              : GR_DSORT_THREADS * GR_DSORT_ITEMS;
@@ -202,8 +210,17 @@ __device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
 
 // PAIRS = false: keys only (vin/vout unused), e.g. tile-list entries that carry
 // their payload in the key bits above the sorted ones.
+// LDS of a pass: the per-wave digit counters are dead once every pair's
+// position in the partition is known, so the reordered partition overlays them.
 template <typename KeyT, bool PAIRS, class S>
-__global__ __launch_bounds__(S::kThreads) void onesweep_kernel(
+struct SortLds {
+    static constexpr size_t kPairBytes = (size_t)S::kTile * (sizeof(KeyT) + (PAIRS ? sizeof(int32_t) : 0));
+    static constexpr size_t kCountBytes = (size_t)S::kWaves * kRadix * sizeof(uint32_t);
+    static constexpr size_t kBytes = kPairBytes > kCountBytes ? kPairBytes : kCountBytes;
+};
+
+template <typename KeyT, bool PAIRS, class S>
+__global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::kWavesPerEU))) void onesweep_kernel(
     const KeyT* __restrict__ kin, const int32_t* __restrict__ vin, KeyT* __restrict__ kout,
     int32_t* __restrict__ vout, int32_t n, const uint32_t* count_dev, int strict, int shift, int bits,
     const uint32_t* __restrict__ digit_off, uint32_t* status, uint32_t* status_next, uint32_t* ticket) {
@@ -212,12 +229,13 @@ __global__ __launch_bounds__(S::kThreads) void onesweep_kernel(
     constexpr int kSortItems = S::kItems;
     constexpr int kSortTile = S::kTile;
     __shared__ uint32_t s_part;
-    __shared__ uint32_t s_wcnt[kSortWaves][kRadix];
     __shared__ uint32_t s_dstart[kRadix];
     __shared__ uint32_t s_gbase[kRadix];
     __shared__ uint32_t s_wsum[kRadix / 64];
-    __shared__ KeyT s_keys[kSortTile];
-    __shared__ int32_t s_vals[PAIRS ? kSortTile : 1];
+    __shared__ __align__(16) unsigned char s_raw[SortLds<KeyT, PAIRS, S>::kBytes];
+    auto s_wcnt = reinterpret_cast<uint32_t(*)[kRadix]>(s_raw);       // [waves][digit], ranking phase
+    KeyT* s_keys = reinterpret_cast<KeyT*>(s_raw);                      // reorder phase
+    int32_t* s_vals = reinterpret_cast<int32_t*>(s_raw + (size_t)kSortTile * sizeof(KeyT));
 
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t == 0) s_part = atomicAdd(ticket, 1u);
@@ -321,13 +339,18 @@ __global__ __launch_bounds__(S::kThreads) void onesweep_kernel(
     }
     __syncthreads();
 
+    // positions in the partition, then (counters dead) the reorder over the same LDS
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const uint32_t d = (uint32_t)(key[i] >> shift) & dmask;
+        rank[i] += s_dstart[d] + s_wcnt[w][d];
+    }
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
         if (wbase + i * 64 < n) {
-            const uint32_t d = (uint32_t)(key[i] >> shift) & dmask;
-            const uint32_t lp = s_dstart[d] + s_wcnt[w][d] + rank[i];
-            s_keys[lp] = key[i];
-            if constexpr (PAIRS) s_vals[lp] = val[i];
+            s_keys[rank[i]] = key[i];
+            if constexpr (PAIRS) s_vals[rank[i]] = val[i];
         }
     }
     __syncthreads();
