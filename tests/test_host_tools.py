@@ -180,3 +180,31 @@ def test_headless_ply_end_to_end(torch_cuda, orc, tmp_path):
                            capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stdout + r.stderr
         assert read_ppm(out).max() > 0
+
+
+def test_serving_load_generator_host_side():
+    """The serving workloads' request generators (seeded, resolutions from the workload's set,
+    cameras looking at the scene centroid from radius 4-10) and the load-generator CLI's
+    argument handling; nothing here touches a GPU."""
+    sv = importlib.import_module("3dgaussianrenderer_amd.serving")
+    scenes = [{"centroid": (0.5 * i, -0.25 * i, 1.0)} for i in range(3)]
+    a = sv.make_requests(scenes, 40, seed=5)
+    b = sv.make_requests(scenes, 40, seed=5)
+    assert len(a) == 40 and [(i, W, H) for i, _, W, H in a] == [(i, W, H) for i, _, W, H in b]
+    assert all(np.array_equal(np.asarray(x[1].packed()), np.asarray(y[1].packed())) for x, y in zip(a, b))
+    assert {(W, H) for _, _, W, H in a} <= set(sv.RESOLUTIONS) and len({i for i, _, _, _ in a}) == 3
+    for i, cam, W, H in a:
+        eye = np.asarray(cam.position[:3], np.float64)
+        r = np.linalg.norm(eye - np.asarray(scenes[i]["centroid"], np.float64))
+        assert 4.0 - 1e-3 <= r <= 10.0 + 1e-3
+    bursts = sv.make_bursty(scenes, 3, burst=8, singles_per_burst=2, seed=1)
+    assert len(bursts) == 3 * (8 + 2)
+    for k in range(3):
+        blk = bursts[10 * k:10 * k + 10]
+        same = blk[:8]   # one scene, one resolution, one arrival time
+        assert len({(x[0], x[1], x[3], x[4]) for x in same}) == 1
+        singles = blk[8:]   # single 4K requests shortly after the burst
+        assert all((x[3], x[4]) == (3840, 2160) and x[0] > same[0][0] for x in singles)
+    r = subprocess.run([__import__("sys").executable, "-m", "3dgaussianrenderer_amd.serving", "--help"],
+                       cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "--workload" in r.stdout and "--requests" in r.stdout
