@@ -111,7 +111,7 @@ class ServerStats(C.Structure):
     _fields_ = [("resident_bytes", C.c_int64), ("workspace_bytes", C.c_int64), ("peak_bytes", C.c_int64),
                 ("batches", C.c_int64), ("views", C.c_int64), ("graph_launches", C.c_int64),
                 ("budget_waits", C.c_int64), ("graph_captures", C.c_int64), ("capture_ms", C.c_double),
-                ("copy_out_ms", C.c_double)]
+                ("copy_out_ms", C.c_double), ("workspace_reallocs", C.c_int64), ("realloc_ms", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -76,7 +76,7 @@ gr_status sfail(gr_status st, const std::string& msg) {
 struct Dev {
     void* p = nullptr;
     size_t bytes = 0;
-    gr_status ensure(size_t need, std::atomic<int64_t>& acct, bool* grew = nullptr) {
+    gr_status ensure(size_t need, std::atomic<int64_t>& acct, int* grows = nullptr) {
         if (need <= bytes && p) return GR_OK;
         const size_t want = std::max<size_t>(need + need / 4, 256);
         if (p) {
@@ -88,7 +88,7 @@ struct Dev {
         S_HIP(hipMalloc(&p, want));
         bytes = want;
         acct += (int64_t)want;
-        if (grew) *grew = true;
+        if (grows) ++*grows;
         return GR_OK;
     }
     // Net change of the accounted bytes if ensure(need) ran now (the old block is freed first).
@@ -228,12 +228,14 @@ struct gr_server {
     std::deque<Pending> incoming;
     std::deque<gr_completion> done;
     std::map<int32_t, Scene*> scenes;
+    std::atomic<int32_t> largest_n{0};               // most Gaussians of any scene added so far
     int32_t next_scene = 0;
     uint64_t next_request = 1;
     std::atomic<int64_t> resident{0}, workspace{0};
     std::atomic<int64_t> peak{0}, batches{0}, views{0}, graph_launches{0}, budget_waits{0};
     std::atomic<int64_t> graph_captures{0};
     std::atomic<int64_t> capture_us{0}, copy_us{0};
+    std::atomic<int64_t> reallocs{0}, realloc_us{0};
     std::vector<Slot> slots;
     std::unique_ptr<std::atomic<int>[]> staging_busy;   // [slot][2]
     std::mutex copy_mu;
@@ -257,12 +259,26 @@ struct gr_server {
         while (u > p && !peak.compare_exchange_weak(p, u)) {
         }
     }
+    // `n` buffers were (re)allocated since t0
+    void note_realloc(int n, std::chrono::steady_clock::time_point t0) {
+        if (n <= 0) return;
+        reallocs += n;
+        realloc_us += (int64_t)std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+    }
     bool over_budget(int64_t extra) const {
         return cfg.hbm_budget_bytes > 0 && used() + extra > cfg.hbm_budget_bytes;
     }
 };
 
 namespace {
+
+constexpr int32_t kBatchTiles = 1 << 16;   // tiles per batch beyond which views go to the next batch
+constexpr int64_t kMaxRecords = (1 << 30) - 1;   // views x Gaussians of a batch (depth sort's count)
+
+// Views of an n-Gaussian scene one batch may hold: each view adds n depth-sort records.
+int max_views_for(const gr_server* s, int32_t n) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(s->cfg.max_views_per_batch, kMaxRecords / std::max(n, 1)));
+}
 
 Scene* scene_ptr(gr_server* s, int32_t id) {
     std::lock_guard<std::mutex> lk(s->mu);
@@ -391,16 +407,33 @@ void release_workspace(gr_server* s, Slot& sl) {
     ++w.generation;
 }
 
+// Without an HBM budget the slot's phase-A buffers are sized for a full batch of the largest
+// scene added so far (as many views as a batch of it may hold, up to the tile and image
+// sizes a full batch can reach), so neither fuller batches nor larger scenes reallocate them
+// later: each reallocation holds the scheduler for about half a millisecond.  Under a budget
+// every buffer is sized for the batch at hand (exact admission).
 gr_status ensure_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     Workspace& w = sl.ws;
     const size_t V = sl.batch.size();
-    bool grew = false;
-    for (const PhaseANeed& n : phase_a_needs(sl, sc, V, sl.img_bytes, sl.tiles_total))
-        S_CHECK(n.dev->ensure((size_t)n.bytes, s->workspace, &grew));
-    S_CHECK(w.views_host.ensure(V * sizeof(ViewDesc)));
-    S_CHECK(w.k_host.ensure(64));
-    S_CHECK(w.img_host[sl.img_cur].ensure((size_t)sl.img_bytes));
-    if (grew) ++w.generation;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<PhaseANeed> needs = phase_a_needs(sl, sc, V, sl.img_bytes, sl.tiles_total);
+    if (s->cfg.hbm_budget_bytes <= 0) {
+        Scene largest;   // sizes only
+        largest.n = std::max(sc.n, s->largest_n.load());
+        const std::vector<PhaseANeed> full =
+            phase_a_needs(sl, largest, (size_t)max_views_for(s, largest.n),
+                          (int64_t)kBatchTiles * kTileSize * kTileSize * 4, kBatchTiles);
+        for (size_t i = 0; i < needs.size(); ++i) needs[i].bytes = std::max(needs[i].bytes, full[i].bytes);
+    }
+    int grows = 0;
+    for (const PhaseANeed& n : needs)
+        S_CHECK(n.dev->ensure((size_t)n.bytes, s->workspace, &grows));
+    bool pinned_grew = false;
+    S_CHECK(w.views_host.ensure(V * sizeof(ViewDesc), &pinned_grew));
+    S_CHECK(w.k_host.ensure(64, &pinned_grew));
+    S_CHECK(w.img_host[sl.img_cur].ensure((size_t)sl.img_bytes, &pinned_grew));
+    if (grows) ++w.generation;
+    s->note_realloc(grows + (pinned_grew ? 1 : 0), t0);
     s->note_peak();
     return GR_OK;
 }
@@ -603,7 +636,6 @@ gr_status run_phase(gr_server* s, Slot& sl, const Scene& sc, bool phase_b) {
 }
 
 // Takes up to max_views pending requests of the scene with the oldest request.
-constexpr int32_t kBatchTiles = 1 << 16;   // tiles per batch beyond which views go to the next batch
 
 bool form_batch(gr_server* s, Slot& sl) {
     int32_t best = -1;
@@ -617,7 +649,8 @@ bool form_batch(gr_server* s, Slot& sl) {
     }
     if (best < 0) return false;
     auto& q = s->queues[best];
-    int limit = s->cfg.max_views_per_batch;
+    const Scene* sc = scene_ptr(s, best);
+    int limit = sc ? max_views_for(s, sc->n) : 1;
     auto lim = s->batch_limit.find(best);
     if (lim != s->batch_limit.end()) {
         limit = std::min(limit, lim->second);
@@ -841,9 +874,10 @@ bool start_phase_b(gr_server* s, Slot& sl) {
             return true;
         }
     }
-    bool grew = false;
-    if (w.list0.ensure(need * 8, s->workspace, &grew) != GR_OK ||
-        w.list1.ensure(need * 8, s->workspace, &grew) != GR_OK) {
+    const auto t0 = std::chrono::steady_clock::now();
+    int grows = 0;
+    if (w.list0.ensure(need * 8, s->workspace, &grows) != GR_OK ||
+        w.list1.ensure(need * 8, s->workspace, &grows) != GR_OK) {
         complete(s, sl, GR_ERR_OUT_OF_MEMORY);
         sl.batch.clear();
         sl.state = SlotState::Idle;
@@ -852,14 +886,15 @@ bool start_phase_b(gr_server* s, Slot& sl) {
     sl.capacity = (uint32_t)(std::min(w.list0.bytes, w.list1.bytes) / 8);
     sl.capacity = std::min<uint32_t>(sl.capacity, (1u << 30) - 1);
     const size_t sort_bytes = radix_sort_temp_bytes((int32_t)sl.capacity, 0, sl.tile_bits);
-    if (w.sort_temp.ensure(sort_bytes, s->workspace, &grew) != GR_OK) {
+    if (w.sort_temp.ensure(sort_bytes, s->workspace, &grows) != GR_OK) {
         complete(s, sl, GR_ERR_OUT_OF_MEMORY);
         sl.batch.clear();
         sl.state = SlotState::Idle;
         return true;
     }
     s->note_peak();
-    if (grew) ++w.generation;
+    s->note_realloc(grows, t0);
+    if (grows) ++w.generation;
     // the image copy stays out of the graph: the staging buffer alternates between batches
     if (run_phase(s, sl, *sc, true) != GR_OK ||
         hipMemcpyAsync(w.img_host[sl.img_cur].p, w.images.p, (size_t)sl.img_bytes, hipMemcpyDeviceToHost,
@@ -1082,6 +1117,7 @@ gr_status gr_server_add_scene(gr_server* s, const gr_model_host* m, int32_t* sce
     sc->bytes = bytes;
     *scene_id = sc->id;
     const int32_t id = sc->id;   // (C++17 sequences the right operand of = first)
+    if (n > s->largest_n.load()) s->largest_n.store(n);   // writers hold mu
     s->scenes[id] = sc.release();
     return GR_OK;
 }
@@ -1150,6 +1186,8 @@ gr_status gr_server_get_stats(gr_server* s, gr_server_stats* o) {
     o->graph_captures = s->graph_captures.load();
     o->capture_ms = (double)s->capture_us.load() / 1000.0;
     o->copy_out_ms = (double)s->copy_us.load() / 1000.0;
+    o->workspace_reallocs = s->reallocs.load();
+    o->realloc_ms = (double)s->realloc_us.load() / 1000.0;
     return GR_OK;
 }
 

@@ -402,6 +402,8 @@ typedef struct gr_server_stats {
     int64_t graph_captures;       /* phase shapes captured as graphs (on their first repeat) */
     double capture_ms;            /* scheduler time spent capturing and instantiating graphs */
     double copy_out_ms;           /* copy-out thread time moving images into callers' buffers */
+    int64_t workspace_reallocs;   /* slot buffers (re)allocated to fit larger batches */
+    double realloc_ms;            /* scheduler time spent in those allocations */
 } gr_server_stats;
 
 GR_API gr_status gr_server_create(gr_server** out, const gr_server_config* config);
