@@ -431,7 +431,9 @@ gr_status ensure_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     bool pinned_grew = false;
     S_CHECK(w.views_host.ensure(V * sizeof(ViewDesc), &pinned_grew));
     S_CHECK(w.k_host.ensure(64, &pinned_grew));
-    S_CHECK(w.img_host[sl.img_cur].ensure((size_t)sl.img_bytes, &pinned_grew));
+    // image staging (pinned host memory, outside the HBM budget): sized once for a full batch
+    const size_t staging_full = (size_t)kBatchTiles * kTileSize * kTileSize * 4;
+    S_CHECK(w.img_host[sl.img_cur].ensure(std::max<size_t>((size_t)sl.img_bytes, staging_full), &pinned_grew));
     if (grows) ++w.generation;
     s->note_realloc(grows + (pinned_grew ? 1 : 0), t0);
     s->note_peak();
