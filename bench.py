@@ -205,8 +205,9 @@ def serving_leg(gr, workload: str, n_req: int, rate: float):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
-                                   "(see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "the renderer and the server run on one GPU; sharding a frame by screen "
+                                   "tiles or a scene by Gaussians over several GPUs is not built, so --gpus "
+                                   "must be 1"}))
         sys.exit(2)
     import torch
 
