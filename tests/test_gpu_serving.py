@@ -9,7 +9,8 @@
   GR_ERR_BUDGET (status 5) without disturbing other requests; a tight budget
   makes batches wait and still completes them;
 * a burst of requests over several scenes completes, fused into fewer
-  batches than views.
+  batches than views;
+* graph replays of a repeated batch shape render each batch's own cameras.
 """
 import numpy as np
 import pytest
@@ -150,4 +151,24 @@ def test_burst_many_scenes(torch_cuda):
     assert min(lat) > 0
     st = srv.stats()
     assert st["views"] == 120 and st["batches"] < 120
+    srv.close()
+
+
+def test_graph_replays_take_each_batch_cameras(torch_cuda):
+    """Single-view batches of two alternating resolutions, each with a new camera: from a
+    shape's first repeat its phases replay a captured graph, and every replay must render
+    its own request's camera (the camera block is the graph's only per-batch input)."""
+    say("serving graph replays with new cameras")
+    res = [(640, 360), (1280, 720)]
+    s, views = scene_and_views(2, 60_000, 61, 12, res)
+    ref = renderer_images(s, views)
+    srv = gr.Server(slots=1, max_views=2, use_graphs=True)
+    sid = srv.add_scene(s)
+    for k, (cam, W, H) in enumerate(views):
+        rid = srv.submit(sid, cam, W, H)
+        d, img = srv.wait_all([rid])[rid]
+        assert d["status"] == 0 and d["batch_views"] == 1
+        assert np.array_equal(img, ref[k]), f"request {k} ({W}x{H}) differs from its own camera's image"
+    st = srv.stats()
+    assert st["graph_captures"] >= 2 and st["graph_launches"] > st["graph_captures"], st
     srv.close()
