@@ -125,6 +125,10 @@ __global__ void sort_init_kernel(uint32_t* hist, uint32_t* tickets, uint32_t* st
 // would put many lanes of one instruction on one LDS address.  The staging
 // array is skewed by one slot per 16 keys to spread the per-lane walks over
 // the LDS banks.
+#ifndef GR_HIST_BLOCKS
+#define GR_HIST_BLOCKS 1024
+#endif
+constexpr int kHistBlocks = GR_HIST_BLOCKS;   // workgroups of the histogram read (each flushes once)
 constexpr int kHistThreads = 256;
 constexpr int kHistSeg = 16;
 constexpr int kHistChunk = kHistThreads * kHistSeg;
@@ -441,7 +445,7 @@ static hipError_t sort_pairs_t(KeyT* keys[2], int32_t* values[2], int32_t* sel, 
     e = sort_prepare(l, tb, s);
     if (e != hipSuccess) return e;
     int hist_blocks = (count + kHistChunk - 1) / kHistChunk;
-    if (hist_blocks > 1024) hist_blocks = 1024;
+    if (hist_blocks > kHistBlocks) hist_blocks = kHistBlocks;
     hipLaunchKernelGGL(sort_hist_kernel<KeyT>, dim3(hist_blocks), dim3(kHistThreads), 0, s, keys[*sel], count,
                        count_dev, begin_bit, end_bit, sort_passes(begin_bit, end_bit),
                        reinterpret_cast<uint32_t*>(tb + l.hist_off));
