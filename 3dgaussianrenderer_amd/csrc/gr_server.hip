@@ -162,7 +162,7 @@ struct Workspace {
     Dev views;                        // ViewDesc[V]
     Pinned views_host, k_host;
     Pinned img_host[2];               // image staging, alternating between batches
-    uint64_t generation = 0;          // bumps when any device pointer changes
+    uint64_t generation = 0;          // bumps when a pointer a captured graph holds changes
     std::vector<Dev*> devs() {
         return {&color, &xy, &z, &ell, &con, &vis_bits, &vis_counts, &dkeys0, &dkeys1, &dorder0, &dorder1,
                 &records, &rec_counts, &rec_offsets, &rec_masks, &dsort_temp, &counts, &list0, &list1, &sort_temp, &ranges, &images, &views};
@@ -428,14 +428,17 @@ gr_status ensure_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     int grows = 0;
     for (const PhaseANeed& n : needs)
         S_CHECK(n.dev->ensure((size_t)n.bytes, s->workspace, &grows));
-    bool pinned_grew = false;
-    S_CHECK(w.views_host.ensure(V * sizeof(ViewDesc), &pinned_grew));
-    S_CHECK(w.k_host.ensure(64, &pinned_grew));
+    // the camera block and the K word are copy nodes of the captured graphs: sized once (for the
+    // largest batch), and a reallocation still retires every graph of the slot
+    bool captured_grew = false, staging_grew = false;
+    S_CHECK(w.views_host.ensure(std::max<size_t>(V, (size_t)s->cfg.max_views_per_batch) * sizeof(ViewDesc),
+                                &captured_grew));
+    S_CHECK(w.k_host.ensure(64, &captured_grew));
     // image staging (pinned host memory, outside the HBM budget): sized once for a full batch
     const size_t staging_full = (size_t)kBatchTiles * kTileSize * kTileSize * 4;
-    S_CHECK(w.img_host[sl.img_cur].ensure(std::max<size_t>((size_t)sl.img_bytes, staging_full), &pinned_grew));
-    if (grows) ++w.generation;
-    s->note_realloc(grows + (pinned_grew ? 1 : 0), t0);
+    S_CHECK(w.img_host[sl.img_cur].ensure(std::max<size_t>((size_t)sl.img_bytes, staging_full), &staging_grew));
+    if (grows || captured_grew) ++w.generation;
+    s->note_realloc(grows + (captured_grew ? 1 : 0) + (staging_grew ? 1 : 0), t0);
     s->note_peak();
     return GR_OK;
 }
