@@ -1033,7 +1033,10 @@ hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
 // the one of gr::blend_exponent / gr::blend_alpha:
 //   w = T * alpha;  C = fma(rgb, w, C);  T = T - w    (= C += rgb*T*alpha, T *= 1-alpha)
 // A wave skips an entry outright when every lane's exponent is below -25,
-// which is exactly the per-lane rule (alpha = 0 leaves C and T unchanged).
+// which is exactly the per-lane rule (alpha = 0 leaves C and T unchanged) as
+// long as the entry's colour is finite: a NaN or infinite channel times a zero
+// weight is NaN, which the reference's blend (:729-736) adds to every pixel it
+// reaches, so such an entry's skip threshold is -inf (never skipped).
 // ---------------------------------------------------------------------------
 constexpr int kRasterThreads = 256;
 constexpr int kRasterBatch = 32;
@@ -1049,9 +1052,9 @@ __device__ __forceinline__ float blend_alpha_dev(float e, float opacity) {
 // One staged entry: all of it sits at one LDS address + immediate offsets.
 struct __align__(16) RasterEntry {
     float4 a;   // cx, cy, qa, qb
-    float4 b;   // qc, opacity, r, g
-    float c;    // b
-    float pad[3];
+    float4 b;   // qc, skip threshold (-25, or -inf for a colour that is not finite), opacity, r
+    float2 c;   // g, b
+    float pad[2];
 };
 
 __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
@@ -1093,9 +1096,10 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
                 const float2 c = a.clip_xy[gi];
                 const float4 q = a.conic[gi];
                 const float4 col = a.color[gi];
+                const bool finite = isfinite(col.x) && isfinite(col.y) && isfinite(col.z);
                 s_e[threadIdx.x].a = make_float4(c.x, c.y, q.x * kHalfLog2e, q.y * kLog2e);
-                s_e[threadIdx.x].b = make_float4(q.z * kHalfLog2e, col.w, col.x, col.y);
-                s_e[threadIdx.x].c = col.z;
+                s_e[threadIdx.x].b = make_float4(q.z * kHalfLog2e, finite ? kMinExponent : -INFINITY, col.w, col.x);
+                s_e[threadIdx.x].c = make_float2(col.y, col.z);
             }
             __syncthreads();
             for (int b0 = 0; b0 < nr; b0 += kRasterBatch) {
@@ -1106,12 +1110,13 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
                     const float dx = pcx - ea.x, dy = pcy - ea.y;
                     const float4 eb = en.b;
                     const float e = blend_exponent(ea.z, ea.w, eb.x, dx, dy);
-                    if (__ballot(e >= kMinExponent) != 0ull) {
-                        const float al = blend_alpha_dev(e, eb.y);
+                    if (__ballot(!(e < eb.y)) != 0ull) {   // some lane not below the threshold (or NaN)
+                        const float al = blend_alpha_dev(e, eb.z);
                         const float wgt = T * al;
-                        C0 = fmaf(eb.z, wgt, C0);
-                        C1 = fmaf(eb.w, wgt, C1);
-                        C2 = fmaf(en.c, wgt, C2);
+                        const float2 gb = en.c;
+                        C0 = fmaf(eb.w, wgt, C0);
+                        C1 = fmaf(gb.x, wgt, C1);
+                        C2 = fmaf(gb.y, wgt, C2);
                         T = T - wgt;
                     }
                 }

@@ -1,8 +1,10 @@
 """Edge cases of the frame path on the GPU: one renderer across frames of changing size (the
 tile list grows, the frame overflowing it is rendered again, grown buffers are reused by
 smaller frames); the stage-level build truncating at the caller's capacity as the reference
-does (core/gaussian_render.cu:566-589, app/main.cpp:330-338); and frames with too many tiles
-rejected before anything runs."""
+does (core/gaussian_render.cu:566-589, app/main.cpp:330-338); frames with too many tiles
+rejected before anything runs; and degenerate Gaussians (non-finite positions, zero, tiny and
+overflowing scales, Gaussians behind or in the camera plane, extreme quaternion bytes, NaN and
+saturating SH) through the stage path and the renderer, against the oracle."""
 import importlib
 
 import numpy as np
@@ -82,3 +84,88 @@ def test_too_many_tiles_rejected(torch_cuda):
                                       0.01, 100.0), 64, 64)
     assert st["tile_list_size"] > 0
     r.close()
+
+
+def adversarial_scene(count=3000, seed=5):
+    """A 10k-style scene (camera at z = 3 looking at the origin) with degenerate Gaussians
+    planted at its front: non-finite positions, zero / tiny / screen-covering / needle /
+    overflowing scales, Gaussians behind the camera, in its plane and before the near plane,
+    opacity 0 and 1, the extreme quaternion bytes, saturating and NaN SH coefficients,
+    far-away and duplicate Gaussians."""
+    s = gr.generate_scene(1, count, 3, seed)
+    p, sr, sh = s["position"], s["scale_rot"], s["sh"]
+    q = sr.view(np.uint32)
+    nan, inf = np.float32("nan"), np.float32("inf")
+    p[0, 0] = nan
+    p[1, :3] = (inf, 0.0, 0.0)
+    sr[2, :3] = 0.0
+    sr[3, :3] = 1e-30
+    sr[4, :3], p[4, :3] = 4.0, 0.0
+    sr[5, :3], p[5, :3] = (3.0, 0.01, 0.01), 0.0
+    p[6, :3] = (0.0, 0.0, 5.0)
+    p[7, :3] = (0.2, 0.1, 3.0)
+    p[8, :3], sr[8, :3] = (0.0, 0.0, 2.995), 0.001
+    p[9, 3] = 0.0
+    p[10, 3], sr[10, :3], p[10, :3] = 1.0, 0.3, (0.1, -0.1, 0.5)
+    q[11, 3], q[12, 3], q[13, 3] = 0x00000000, 0xFFFFFFFF, 0x80808080
+    sh[14, :], sh[15, :] = 1e30, -1e30
+    sh[16, 5] = nan
+    p[17, :3] = (1e20, 1e20, -1e20)
+    sr[18, :3] = (1e20, 1.0, 1.0)
+    p[19, :3] = p[20, :3] = (0.05, 0.05, 0.0)
+    sr[19, :3] = sr[20, :3] = 0.05
+    return s
+
+
+def same_floats(a, b):
+    """Bit-equal, except that any NaN matches any NaN (payloads are not meaningful)."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    both_nan = np.isnan(a) & np.isnan(b)
+    return a.shape == b.shape and np.array_equal(np.where(both_nan, 0, a.view(np.uint32)),
+                                                 np.where(both_nan, 0, b.view(np.uint32)))
+
+
+def test_degenerate_gaussians_stage_path_and_renderer(torch_cuda, orc):
+    say("degenerate Gaussians: stage path and renderer vs the oracle")
+    s = adversarial_scene()
+    n = s["count"]
+    W, H = 320, 240
+    cam = gr.camera_make((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), float(np.deg2rad(60.0)), W, H, 0.01, 100.0)
+    camp = cam.packed()
+    p = StagePipeline(torch_cuda, s, cam, W, H)
+    try:
+        col_g = p.run_sh()
+        col_o = orc.sh(s["position"], p.sh_aligned, n, 3, camp)
+        assert same_floats(col_g, col_o), "SH colour differs"
+        xy_g, z_g, ell_g, con_g = p.run_project()
+        xy_o, z_o, ell_o, con_o = orc.project(s["position"], s["scale_rot"], n, camp, W, H)
+        for name, g, o in (("clip xy", xy_g, xy_o), ("clip z", z_g, z_o), ("ellipse", ell_g, ell_o),
+                           ("conic", con_g, con_o)):
+            assert same_floats(g, o), f"{name} differs"
+        keys_g, vals_g = p.run_build()
+        keys_o, vals_o = orc.build(xy_o, z_o, ell_o, n, W, H)
+        assert np.array_equal(keys_g, keys_o) and np.array_equal(vals_g, vals_o), "tile list differs"
+        # the planted screen-covering and needle Gaussians take the large-footprint path
+        assert (vals_o == 4).sum() > 64 and (vals_o == 5).sum() > 0
+        sk_g, sv_g = p.run_sort()
+        sk_o, sv_o = orc.sort(keys_o, vals_o, 0, orc.sort_end_bit(W, H))
+        assert np.array_equal(sk_g, sk_o) and np.array_equal(sv_g, sv_o), "sorted list differs"
+        rng_g = p.run_ranges()
+        assert np.array_equal(rng_g, orc.ranges(sk_o, p.tiles)), "ranges differ"
+        img_g = p.run_raster()
+        img_o = orc.raster(orc.ranges(sk_o, p.tiles), sv_o, xy_o, con_o, np.ascontiguousarray(col_o, np.float32),
+                           W, H)
+        assert np.array_equal(img_g, img_o), f"stage-path image differs at {np.argwhere(img_g != img_o)[:5]}"
+    finally:
+        p.close()
+    r = gr.Renderer()
+    r.upload(s)
+    img, st = r.render(cam, W, H)
+    keys, vals, _ = frame_list(r)
+    r.close()
+    shal = orc.realign_sh(s["sh"], n, 48)
+    ref = orc.render(s["position"], s["scale_rot"], shal, None, n, 3, camp, W, H)
+    assert st["tile_list_size"] == ref["k"] == len(sk_o)
+    assert np.array_equal(keys, ref["keys"]) and np.array_equal(vals, ref["values"]), "renderer list differs"
+    assert np.array_equal(img, ref["image"]), "renderer image differs"
