@@ -4,7 +4,8 @@ smaller frames); the stage-level build truncating at the caller's capacity as th
 does (core/gaussian_render.cu:566-589, app/main.cpp:330-338); frames with too many tiles
 rejected before anything runs; and degenerate Gaussians (non-finite positions, zero, tiny and
 overflowing scales, Gaussians behind or in the camera plane, extreme quaternion bytes, NaN and
-saturating SH) through the stage path and the renderer, against the oracle."""
+saturating SH) through the stage path, the renderer (FP32 and the FP16/BF16 storage form) and
+the server, against the oracle and the renderer."""
 import importlib
 
 import numpy as np
@@ -169,3 +170,51 @@ def test_degenerate_gaussians_stage_path_and_renderer(torch_cuda, orc):
     assert st["tile_list_size"] == ref["k"] == len(sk_o)
     assert np.array_equal(keys, ref["keys"]) and np.array_equal(vals, ref["values"]), "renderer list differs"
     assert np.array_equal(img, ref["image"]), "renderer image differs"
+
+
+@pytest.mark.parametrize("precision", [1, 2], ids=["fp16", "bf16"])
+def test_degenerate_gaussians_precision_form(torch_cuda, orc, precision):
+    """The same scene stored as FP16 / BF16 (the 1e30 SH coefficients overflow to infinity in
+    FP16): the renderer against the oracle run on the decoded inputs."""
+    say(f"degenerate Gaussians, precision form {precision}")
+    s = adversarial_scene()
+    n = s["count"]
+    W, H = 320, 240
+    cam = gr.camera_make((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), float(np.deg2rad(60.0)), W, H, 0.01, 100.0)
+    r = gr.Renderer()
+    r.upload(s, precision)
+    img, st = r.render(cam, W, H)
+    keys, vals, _ = frame_list(r)
+    r.close()
+    pos_p, packed, sh16 = gr.pack_precision(s, precision)
+    pos, sr, shd = orc.decode_precision(pos_p, packed, sh16, n, 3, precision)
+    ref = orc.render(pos, sr, shd, None, n, 3, cam.packed(), W, H)
+    assert st["tile_list_size"] == ref["k"]
+    assert np.array_equal(keys, ref["keys"]) and np.array_equal(vals, ref["values"]), "tile list differs"
+    assert np.array_equal(img, ref["image"]), f"image differs at {np.argwhere(img != ref['image'])[:5]}"
+
+
+def test_degenerate_gaussians_served(torch_cuda):
+    """The scene served alone and fused with a second view: each image equals the renderer's."""
+    say("degenerate Gaussians, served")
+    s = adversarial_scene()
+    views = [(gr.camera_make((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), float(np.deg2rad(60.0)), 320, 240, 0.01, 100.0),
+              320, 240),
+             (gr.camera_make((0.5, 0.3, 2.5), (0.0, 0.0, 0.0), float(np.deg2rad(50.0)), 200, 136, 0.01, 100.0),
+              200, 136)]
+    r = gr.Renderer()
+    r.upload(s)
+    ref = [r.render(cam, W, H)[0] for cam, W, H in views]
+    r.close()
+    srv = gr.Server(slots=2, max_views=4, use_graphs=False)
+    sid = srv.add_scene(s)
+    rid = srv.submit(sid, *views[0])
+    d, img = srv.wait_all([rid])[rid]
+    assert d["status"] == 0 and np.array_equal(img, ref[0]), "served alone differs"
+    for _ in range(3):
+        ids = [srv.submit(sid, *v) for v in views]
+        got = srv.wait_all(ids)
+        for k, i in enumerate(ids):
+            d, img = got[i]
+            assert d["status"] == 0 and np.array_equal(img, ref[k]), f"view {k} served in a batch differs"
+    srv.close()
