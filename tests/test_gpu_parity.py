@@ -39,6 +39,7 @@ CASES = [
     ("c1_800x600", 1, 10000, 3, 800, 600, (0.0, 0.0, 3.0), FOV60),
     ("c1_1024sq_deg2", 1, 10000, 2, 1024, 1024, (0.3, 0.4, 2.5), FOV60),
     ("c1_odd_803x601_deg1", 1, 7001, 1, 803, 601, (-1.0, 0.5, 2.8), FOV50),
+    ("c1_deg0_640x480", 1, 10000, 0, 640, 480, (0.2, -0.3, 2.7), FOV60),
     ("clusters_1080p", 2, 60000, 3, 1920, 1080, None, FOV50),
 ]
 
