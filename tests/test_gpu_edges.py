@@ -218,3 +218,40 @@ def test_degenerate_gaussians_served(torch_cuda):
             d, img = got[i]
             assert d["status"] == 0 and np.array_equal(img, ref[k]), f"view {k} served in a batch differs"
     srv.close()
+
+
+def test_two_renderers_on_two_threads(torch_cuda):
+    """Two renderers (each with its own stream) driven from two host threads at once, different
+    scenes and sizes: every frame equals the same frame rendered with nothing else running."""
+    import threading
+    say("two renderers on two threads")
+    jobs = []
+    for kind, count, seed, (W, H) in ((1, 20_000, 71, (640, 480)), (2, 60_000, 72, (1280, 720))):
+        s = gr.generate_scene(kind, count, 3, seed)
+        c = s["centroid"] if kind == 2 else np.zeros(3, np.float32)
+        cams = [gr.camera_make(gr.pose_eye(seed, i, c, 3.0 if kind == 1 else 8.0), c, FOV50, W, H, 0.01, 100.0)
+                for i in range(4)]
+        r = gr.Renderer()
+        r.upload(s)
+        jobs.append((r, cams, W, H, [r.render(cam, W, H)[0] for cam in cams]))
+    errors = []
+
+    def work(r, cams, W, H, ref):
+        try:
+            for rep in range(6):
+                for i, cam in enumerate(cams):
+                    img, _ = r.render(cam, W, H)
+                    if not np.array_equal(img, ref[i]):
+                        errors.append(f"{W}x{H} pose {i} repeat {rep} differs")
+        except Exception as e:   # reported below, on the test's thread
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=work, args=j) for j in jobs]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in threads), "a render thread did not finish"
+    for r, *_ in jobs:
+        r.close()
+    assert not errors, errors[:5]
