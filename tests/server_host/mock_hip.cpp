@@ -340,10 +340,16 @@ hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t s) 
     return hipSuccess;
 }
 
+constexpr int32_t kNoRanges = -7;   // marks a list that did not fit (the real ranges stay empty)
+
 hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
     run_or_record(s, [=] {
-        const uint32_t n = a.size_dev ? std::min<uint32_t>(*a.size_dev, (uint32_t)a.size) : (uint32_t)a.size;
         if (a.tile_shift != 0 && a.tile_shift != 32) std::abort();
+        if (a.size_dev && *a.size_dev > (uint32_t)a.size) {   // K beyond the list: nothing is ranged
+            a.tile_range[0] = kNoRanges;
+            return;
+        }
+        const uint32_t n = a.size_dev ? *a.size_dev : (uint32_t)a.size;
         rd(a.keys, (size_t)n * 8);
         touch(a.tile_range, (size_t)a.tiles * 8);
     });
@@ -353,12 +359,13 @@ hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
 hipError_t launch_raster(const RasterArgs& a, hipStream_t s) {
     run_or_record(s, [=] {
         rd(a.tile_range, (size_t)a.tiles * 8);
+        const bool empty = a.tile_range[0] == kNoRanges;   // every tile empty: RGBA 0
         int tiles = 0;
         for (int v = 0; v < a.nviews; ++v) {
             const ViewDesc& d = view_of(v, a.v0, a.views);
             if (d.tile_base != tiles) std::abort();   // views must tile the global tile space in order
             tiles += d.tiles;
-            std::memset(d.out, (int)(uint8_t)d.cam.position.x, (size_t)d.geom.width * d.geom.height * 4);
+            std::memset(d.out, empty ? 0 : (int)(uint8_t)d.cam.position.x, (size_t)d.geom.width * d.geom.height * 4);
         }
         if (tiles != a.tiles) std::abort();
     });
