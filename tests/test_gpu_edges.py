@@ -5,7 +5,8 @@ does (core/gaussian_render.cu:566-589, app/main.cpp:330-338); frames with too ma
 rejected before anything runs; and degenerate Gaussians (non-finite positions, zero, tiny and
 overflowing scales, Gaussians behind or in the camera plane, extreme quaternion bytes, NaN and
 saturating SH) through the stage path, the renderer (FP32 and the FP16/BF16 storage form) and
-the server, against the oracle and the renderer."""
+the server, against the oracle and the renderer; two renderers on two host threads; and entry
+counts too large for the count pass's packed field."""
 import importlib
 
 import numpy as np
@@ -255,3 +256,47 @@ def test_two_renderers_on_two_threads(torch_cuda):
     for r, *_ in jobs:
         r.close()
     assert not errors, errors[:5]
+
+
+def test_saturated_entry_counts(torch_cuda, orc):
+    """The count pass packs each record's entry count above its index bits (1.2M records: 21
+    index bits, counts up to 2047); three Gaussians covering most of a 1080p frame (4,800-8,160
+    tiles each) overflow that field and take the escape through the full counts.  The renderer
+    against the oracle, and two views served in one batch (2.4M records) against the renderer."""
+    say("entry counts beyond the packed field")
+    s = gr.generate_scene(2, 1_200_000, 3, 81)
+    c = s["centroid"]
+    for k, sc in enumerate((2.5, 3.0, 1.5)):
+        s["position"][k, :3] = c + np.float32(0.1 * k)
+        s["scale_rot"][k, :3] = sc
+        s["position"][k, 3] = 0.3
+    W, H = 1920, 1080
+    cams = [gr.camera_make(gr.pose_eye(81, i, c, 8.0), c, FOV50, W, H, 0.01, 100.0) for i in range(2)]
+    n = s["count"]
+    r = gr.Renderer()
+    r.upload(s)
+    img, st = r.render(cams[0], W, H)
+    keys, vals, _ = frame_list(r)
+    second = r.render(cams[1], W, H)[0]
+    r.close()
+    shal = orc.realign_sh(s["sh"], n, 48)
+    ref = orc.render(s["position"], s["scale_rot"], shal, None, n, 3, cams[0].packed(), W, H)
+    assert min(int((ref["values"] == k).sum()) for k in range(3)) > 2047
+    assert st["tile_list_size"] == ref["k"]
+    assert np.array_equal(keys, ref["keys"]) and np.array_equal(vals, ref["values"]), "tile list differs"
+    assert np.array_equal(img, ref["image"]), "image differs"
+    srv = gr.Server(slots=1, max_views=2, use_graphs=False)
+    sid = srv.add_scene(s)
+    fused = 0
+    blocker_cam = gr.camera_make(gr.pose_eye(81, 5, c, 8.0), c, FOV50, 3840, 2160, 0.01, 100.0)
+    for _ in range(3):
+        # a 4K view first keeps the slot busy, so the two views queue behind it and share a batch
+        ids = [srv.submit(sid, blocker_cam, 3840, 2160)] + [srv.submit(sid, cam, W, H) for cam in cams]
+        got = srv.wait_all(ids, timeout_s=120)
+        assert got[ids[0]][0]["status"] == 0
+        for want, rid in zip((img, second), ids[1:]):
+            d, out = got[rid]
+            fused = max(fused, d["batch_views"])
+            assert d["status"] == 0 and np.array_equal(out, want), "served view differs"
+    srv.close()
+    say(f"[saturated counts] largest batch of the two 1080p views: {fused}")
