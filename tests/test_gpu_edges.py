@@ -5,8 +5,9 @@ does (core/gaussian_render.cu:566-589, app/main.cpp:330-338); frames with too ma
 rejected before anything runs; and degenerate Gaussians (non-finite positions, zero, tiny and
 overflowing scales, Gaussians behind or in the camera plane, extreme quaternion bytes, NaN and
 saturating SH) through the stage path, the renderer (FP32 and the FP16/BF16 storage form) and
-the server, against the oracle and the renderer; two renderers on two host threads; and entry
-counts too large for the count pass's packed field."""
+the server, against the oracle and the renderer; two renderers on two host threads; entry
+counts too large for the count pass's packed field; and frames of 2^16 and more tiles (a third
+tile-sort pass)."""
 import importlib
 
 import numpy as np
@@ -300,3 +301,34 @@ def test_saturated_entry_counts(torch_cuda, orc):
             assert d["status"] == 0 and np.array_equal(out, want), "served view differs"
     srv.close()
     say(f"[saturated counts] largest batch of the two 1080p views: {fused}")
+
+
+@pytest.mark.parametrize("size", [(4096, 4096), (7680, 4320)], ids=["65536_tiles", "129600_tiles"])
+def test_tile_ids_beyond_16_bits(torch_cuda, orc, size):
+    """Exactly 2^16 tiles (tile ids fit 16 bits: two sort passes) and 480 x 270 tiles (17 bits: a
+    third pass over every entry, three digit histograms counted by the emit): the renderer
+    against the oracle, and the view served alone against the renderer."""
+    W, H = size
+    say(f"tile ids at {W}x{H}")
+    s = gr.generate_scene(2, 80_000, 3, 91)
+    c = s["centroid"]
+    n = s["count"]
+    cam = gr.camera_make(gr.pose_eye(91, 0, c, 7.0), c, FOV50, W, H, 0.01, 100.0)
+    r = gr.Renderer()
+    r.upload(s)
+    img, st = r.render(cam, W, H)
+    keys, vals, _ = frame_list(r)
+    kt = r.kernel_times()
+    r.close()
+    assert kt["tile_sort_passes"] == (2 if W * H <= 4096 * 4096 else 3)
+    shal = orc.realign_sh(s["sh"], n, 48)
+    ref = orc.render(s["position"], s["scale_rot"], shal, None, n, 3, cam.packed(), W, H)
+    assert st["tile_list_size"] == ref["k"]
+    assert np.array_equal(keys, ref["keys"]) and np.array_equal(vals, ref["values"]), "tile list differs"
+    assert np.array_equal(img, ref["image"]), "image differs"
+    srv = gr.Server(slots=1, max_views=4, use_graphs=False)
+    sid = srv.add_scene(s)
+    rid = srv.submit(sid, cam, W, H)
+    d, out = srv.wait_all([rid], timeout_s=120)[rid]
+    srv.close()
+    assert d["status"] == 0 and np.array_equal(out, img), "served view differs"
