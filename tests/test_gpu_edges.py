@@ -332,3 +332,27 @@ def test_tile_ids_beyond_16_bits(torch_cuda, orc, size):
     d, out = srv.wait_all([rid], timeout_s=120)[rid]
     srv.close()
     assert d["status"] == 0 and np.array_equal(out, img), "served view differs"
+
+
+def test_reupload_switches_precision(torch_cuda):
+    """One renderer re-uploaded as FP32, FP16, BF16, FP32 and a smaller scene (SH degree 1):
+    each frame equals a fresh renderer's frame of that upload."""
+    say("renderer re-uploads across precisions and scenes")
+    a = gr.generate_scene(2, 50_000, 3, 95)
+    b = gr.generate_scene(1, 7_000, 1, 96)
+    c = a["centroid"]
+    cam = gr.camera_make(gr.pose_eye(95, 0, c, 7.0), c, FOV50, 800, 600, 0.01, 100.0)
+
+    def fresh(scene, precision):
+        r = gr.Renderer()
+        r.upload(scene, precision)
+        img = r.render(cam, 800, 600)[0]
+        r.close()
+        return img
+
+    r = gr.Renderer()
+    for scene, precision in ((a, 0), (a, 1), (a, 2), (a, 0), (b, 0), (a, 1)):
+        r.upload(scene, precision)
+        img = r.render(cam, 800, 600)[0]
+        assert np.array_equal(img, fresh(scene, precision)), f"upload ({scene['count']}, precision {precision})"
+    r.close()
