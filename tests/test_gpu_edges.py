@@ -356,3 +356,44 @@ def test_reupload_switches_precision(torch_cuda):
         img = r.render(cam, 800, 600)[0]
         assert np.array_equal(img, fresh(scene, precision)), f"upload ({scene['count']}, precision {precision})"
     r.close()
+
+
+def test_sort_scratch_reused_across_sizes(torch_cuda):
+    """One context and one caller-owned sort scratch across sorts of growing and shrinking size
+    (the scratch grows as CUB's does, core/gaussian_render.cu:610-618): each result equals numpy's
+    stable argsort on the key bits."""
+    import ctypes as C
+    say("sort scratch reused across sizes")
+    t = torch_cuda
+    rng = np.random.default_rng(97)
+    ctx = gr.Context(t.cuda.current_stream().cuda_stream)
+    temp = gr.SortTemp()
+    grown = []
+    try:
+        for n, end_bit in ((1000, 44), (200_000, 44), (50, 40), (3_000_000, 47), (10, 44), (700_000, 33)):
+            tiles = rng.integers(0, 1 << max(end_bit - 32, 1), n, dtype=np.uint64)
+            depth = rng.integers(0, 1 << 32, n, dtype=np.uint64) & np.uint64(0xFFFFFF00)   # ties
+            keys = (tiles << np.uint64(32)) | depth
+            vals = np.arange(n, dtype=np.int32)
+            kd = [t.empty(n, dtype=t.int64, device="cuda") for _ in range(2)]
+            vd = [t.empty(n, dtype=t.int32, device="cuda") for _ in range(2)]
+            kd[0][:] = t.from_numpy(keys.view(np.int64)).to("cuda")
+            vd[0][:] = t.from_numpy(vals).to("cuda")
+            t.cuda.synchronize()
+            kb = gr.DoubleBufferU64((C.c_void_p * 2)(kd[0].data_ptr(), kd[1].data_ptr()), 0)
+            vb = gr.DoubleBufferI32((C.c_void_p * 2)(vd[0].data_ptr(), vd[1].data_ptr()), 0)
+            ctx.radix_sort(n, 0, end_bit, kb, vb, temp)
+            ctx.sync()
+            grown.append(temp.nbytes.value)
+            s = kb.selector
+            got_k = kd[s].cpu().numpy().view(np.uint64)
+            got_v = vd[s].cpu().numpy()
+            masked = keys & np.uint64((1 << end_bit) - 1)
+            order = np.argsort(masked, kind="stable")
+            assert np.array_equal(got_v, vals[order]), f"n={n}: order differs"
+            assert np.array_equal(got_k, keys[order]), f"n={n}: keys differ"
+    finally:
+        t.cuda.synchronize()
+        temp.free()
+        ctx.close()
+    assert grown == sorted(grown), f"the scratch shrank: {grown}"
