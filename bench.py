@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--serving", default="c4", help="serving workload: c4, c5 or none")
     ap.add_argument("--serving-requests", type=int, default=1200)
     ap.add_argument("--serving-rate", type=float, default=400.0)
+    ap.add_argument("--serving-rate-high", type=float, default=1200.0,
+                    help="second Poisson rate (about three quarters of the saturated views/s)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU oracle sample length (frames cycle poses)")
     ap.add_argument("--traffic-profile", default=str(ROOT / "profiles" / "tile_sort_pass_6M_1080p.md"),
                     help="rocprofv3 summary holding the measured HBM bytes per tile-sort pass")
@@ -166,7 +168,7 @@ def stage_path_fps(gr, torch, scene, cams, W, H, frames):
     return fps
 
 
-def serving_leg(gr, workload: str, n_req: int, rate: float):
+def serving_leg(gr, workload: str, n_req: int, rate: float, rate_high: float = 0.0):
     """Continuous-batching server on the serving workload vs sequential plain renders."""
     from importlib import import_module
     sv = import_module("3dgaussianrenderer_amd.serving")
@@ -188,6 +190,10 @@ def serving_leg(gr, workload: str, n_req: int, rate: float):
         out["bursty"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items() if k != "images"}
     else:
         res = sv.run_load(srv, ids, reqs, rate=rate)
+        if rate_high > 0:
+            hi = sv.run_load(srv, ids, sv.make_requests(scenes, n_req, seed=13), rate=rate_high)
+            out[f"poisson_{int(rate_high)}"] = {k: (round(v, 3) if isinstance(v, float) else v)
+                                                for k, v in hi.items() if k != "images"}
         out[f"poisson_{int(rate)}"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items()
                                        if k != "images"}
     sat_reqs = sv.make_requests(scenes, n_req, seed=12)
@@ -281,7 +287,8 @@ def main():
     serving = None
     if args.serving != "none":
         r.close()
-        serving, sv_scenes, sv_reqs = serving_leg(gr, args.serving, args.serving_requests, args.serving_rate)
+        serving, sv_scenes, sv_reqs = serving_leg(gr, args.serving, args.serving_requests, args.serving_rate,
+                                                  args.serving_rate_high)
         r = gr.Renderer(0)
         r.upload(scene)
     cpu = None
