@@ -3,7 +3,7 @@
 //
 //   server_host_test mix <graphs> <budget_bytes>   alone + fused rounds over two scenes
 //   server_host_test add_budget                    a scene larger than the budget is refused
-//   server_host_test threads <graphs>              concurrent submitters, a scene removed mid-run
+//   server_host_test threads <graphs> [budget]     concurrent submitters, a scene removed mid-run
 //
 // Each request's camera position.x carries a tag in [1, 250]; the mocked
 // raster paints that view with it, so every completed image is checked to be
@@ -163,8 +163,8 @@ int run_add_budget() {
     return 0;
 }
 
-int run_threads(int graphs) {
-    gr_server_config cfg{-1, 3, 16, graphs, 0};
+int run_threads(int graphs, long long budget) {
+    gr_server_config cfg{-1, 3, 16, graphs, budget};
     gr_server* s = nullptr;
     if (gr_server_create(&s, &cfg) != GR_OK) return 2;
     Model m1(50000, 3, 0), m2(20000, 2, 2), m3(30000, 0, 0);
@@ -210,6 +210,6 @@ int main(int argc, char** argv) {
     const std::string mode = argv[1];
     if (mode == "mix") return run_mix(argc > 2 ? atoi(argv[2]) : 0, argc > 3 ? atoll(argv[3]) : 0);
     if (mode == "add_budget") return run_add_budget();
-    if (mode == "threads") return run_threads(argc > 2 ? atoi(argv[2]) : 0);
+    if (mode == "threads") return run_threads(argc > 2 ? atoi(argv[2]) : 0, argc > 3 ? atoll(argv[3]) : 0);
     return 1;
 }

@@ -100,3 +100,15 @@ def test_concurrent_submitters_and_removal(harness, graphs):
     assert r["remove_status"] == 0
     assert r["submitted"] + r["refused"] == 240
     assert r["ok"] == r["submitted"] and r["bad_image"] == 0 and r["other"] == 0
+
+
+@pytest.mark.parametrize("graphs", [0, 1], ids=["eager", "graphs"])
+def test_concurrent_submitters_under_budget(harness, graphs):
+    # room for the scenes and about one batch: batches wait, halve and requeue while four
+    # threads submit and a scene is removed
+    budget = 60_000_000
+    r = run(harness, "threads", graphs, budget)
+    assert r["remove_status"] == 0
+    assert r["submitted"] + r["refused"] == 240
+    assert r["ok"] + r["budget"] == r["submitted"] and r["bad_image"] == 0 and r["other"] == 0
+    assert r["peak"] <= budget
