@@ -208,3 +208,13 @@ def test_serving_load_generator_host_side():
     r = subprocess.run([__import__("sys").executable, "-m", "3dgaussianrenderer_amd.serving", "--help"],
                        cwd=ROOT, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "--workload" in r.stdout and "--requests" in r.stdout
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus 2 exits with status 2 and one JSON line saying why, before any GPU work."""
+    import sys
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], cwd=ROOT, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "--gpus must be 1" in msg["error"]
