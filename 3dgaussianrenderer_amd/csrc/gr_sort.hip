@@ -71,6 +71,12 @@ constexpr int kMinTile = GR_SORT_THREADS * GR_SORT_ITEMS < GR_DSORT_THREADS * GR
                              : GR_DSORT_THREADS * GR_DSORT_ITEMS;
 constexpr int kRadix = 256;
 constexpr int kMaxPasses = 8;
+#ifndef GR_HIST_REPLICAS
+#define GR_HIST_REPLICAS 16
+#endif
+// copies of the digit histograms the histogram read's workgroups flush into (workgroup b into
+// copy b mod kHistReplicas, fewer atomics per address); the scan sums them into copy 0
+constexpr int kHistReplicas = GR_HIST_REPLICAS;
 constexpr int kLookback = GR_SORT_LOOKBACK;            // predecessors read per look-back step
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagIncl = 2u << 30;
@@ -90,7 +96,7 @@ static SortLayout sort_layout(int32_t count, int tile) {
     const int64_t max_parts = ((int64_t)count + kMinTile - 1) / kMinTile;
     size_t off = 0;
     l.hist_off = off;
-    off += sizeof(uint32_t) * kMaxPasses * kRadix;
+    off += sizeof(uint32_t) * kHistReplicas * kMaxPasses * kRadix;
     l.ticket_off = off;
     off += 256;
     off = (off + 255) & ~(size_t)255;
@@ -113,7 +119,7 @@ size_t radix_sort_temp_bytes(int32_t count, int32_t, int32_t) {
 
 __global__ void sort_init_kernel(uint32_t* hist, uint32_t* tickets, uint32_t* status0, int64_t status_n) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid < kMaxPasses * kRadix) hist[tid] = 0u;
+    if (tid < kHistReplicas * kMaxPasses * kRadix) hist[tid] = 0u;
     if (tid < kMaxPasses) tickets[tid] = 0u;
     for (int64_t i = tid; i < status_n; i += (int64_t)gridDim.x * blockDim.x) status0[i] = 0u;
 }
@@ -128,6 +134,7 @@ __global__ void sort_init_kernel(uint32_t* hist, uint32_t* tickets, uint32_t* st
 #ifndef GR_HIST_BLOCKS
 #define GR_HIST_BLOCKS 1024
 #endif
+
 constexpr int kHistBlocks = GR_HIST_BLOCKS;   // workgroups of the histogram read (each flushes once)
 constexpr int kHistThreads = 256;
 constexpr int kHistSeg = 16;
@@ -182,7 +189,7 @@ __global__ __launch_bounds__(kHistThreads) void sort_hist_kernel(const KeyT* __r
     __syncthreads();
     for (int i = threadIdx.x; i < passes * kRadix; i += blockDim.x) {
         const uint32_t v = (&s_h[0][0])[i];
-        if (v) atomicAdd(&hist[i], v);
+        if (v) atomicAdd(&hist[(blockIdx.x % kHistReplicas) * kMaxPasses * kRadix + i], v);
     }
 }
 
@@ -191,7 +198,8 @@ __global__ __launch_bounds__(256) void sort_hist_scan_kernel(uint32_t* hist) {
     __shared__ uint32_t s_w[4];
     uint32_t* h = hist + blockIdx.x * kRadix;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const uint32_t v = h[t];
+    uint32_t v = h[t];
+    for (int r = 1; r < kHistReplicas; ++r) v += h[(size_t)r * kMaxPasses * kRadix + t];
     uint32_t incl = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -377,7 +385,8 @@ static hipError_t sort_prepare(const SortLayout& l, char* tb, hipStream_t s) {
     uint32_t* status0 = reinterpret_cast<uint32_t*>(tb + l.status_off[0]);
     const int64_t status_n = (int64_t)l.partitions * kRadix;
     int64_t init_blocks = (status_n + 255) / 256;
-    if (init_blocks < (kMaxPasses * kRadix + 255) / 256) init_blocks = (kMaxPasses * kRadix + 255) / 256;
+    if (init_blocks < (kHistReplicas * kMaxPasses * kRadix + 255) / 256)
+        init_blocks = (kHistReplicas * kMaxPasses * kRadix + 255) / 256;
     if (init_blocks > 4096) init_blocks = 4096;
     hipLaunchKernelGGL(sort_init_kernel, dim3((unsigned)init_blocks), dim3(256), 0, s, hist, tickets, status0,
                        status_n);
