@@ -196,10 +196,14 @@ size_t radix_sort_temp_bytes(int32_t count, int32_t begin_bit, int32_t end_bit);
 hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                             const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                             size_t temp_bytes, hipStream_t stream);
+// Copies of the sort's digit histograms: a producer may add its counts to any
+// copy (copy c starts kSortHistCopyStride words after copy 0); the scan sums them.
+constexpr int kSortHistCopies = 16;
+constexpr int kSortHistCopyStride = 8 * 256;
 // Two-step form of the keys-only 64-bit sort for a producer that accumulates
 // the digit histograms itself: radix_sort_prepare zeroes the histograms
 // (passes x 256 uint32 at *hist, pass p counting digit
-// (key >> (begin_bit + 8p)) & 255), the partition tickets and the first
+// (key >> (begin_bit + 8p)) & 255, in each of the kSortHistCopies copies), the partition tickets and the first
 // look-back region of `temp` sized for `count`; after the producer has added
 // exactly the keys the sort will see, radix_sort_keys_prepared scans them and
 // runs the passes.  A device-side count above `count` then means the producer

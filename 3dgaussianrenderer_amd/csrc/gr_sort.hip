@@ -71,12 +71,11 @@ constexpr int kMinTile = GR_SORT_THREADS * GR_SORT_ITEMS < GR_DSORT_THREADS * GR
                              : GR_DSORT_THREADS * GR_DSORT_ITEMS;
 constexpr int kRadix = 256;
 constexpr int kMaxPasses = 8;
-#ifndef GR_HIST_REPLICAS
-#define GR_HIST_REPLICAS 16
-#endif
-// copies of the digit histograms the histogram read's workgroups flush into (workgroup b into
-// copy b mod kHistReplicas, fewer atomics per address); the scan sums them into copy 0
-constexpr int kHistReplicas = GR_HIST_REPLICAS;
+// copies of the digit histograms the histogram read's workgroups (and the emit's) flush into
+// (workgroup b into copy b mod kHistReplicas, fewer atomics per address); the scan sums them
+// into copy 0
+constexpr int kHistReplicas = kSortHistCopies;
+static_assert(kSortHistCopyStride == kMaxPasses * kRadix, "histogram copy layout");
 constexpr int kLookback = GR_SORT_LOOKBACK;            // predecessors read per look-back step
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagIncl = 2u << 30;

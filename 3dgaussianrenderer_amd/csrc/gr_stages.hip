@@ -730,7 +730,7 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
         __syncthreads();
         for (int k = 0; k < a.tile_passes; ++k) {
             const uint32_t v = s_hist[k][threadIdx.x];
-            if (v) atomicAdd(&a.tile_hist[k * 256 + threadIdx.x], v);
+            if (v) atomicAdd(&a.tile_hist[(blockIdx.x % kSortHistCopies) * kSortHistCopyStride + k * 256 + threadIdx.x], v);
         }
     }
 }
