@@ -219,6 +219,37 @@ __device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Publishes partition `part`'s count of digit t and returns in `excl` the count of that digit
+// in all earlier partitions (decoupled look-back over kLookback predecessors per step).
+__device__ __forceinline__ void lookback(uint32_t* status, int part, int t, uint32_t total, uint32_t& excl) {
+    uint32_t* my = status + (size_t)part * kRadix + t;
+    if (part == 0) {
+        st_status(my, kFlagIncl | total);
+        return;
+    }
+    st_status(my, kFlagAgg | total);
+    int j = part - 1;
+    bool done = false;
+    while (!done) {
+        uint32_t sv[kLookback];
+#pragma unroll
+        for (int q = 0; q < kLookback; ++q)
+            sv[q] = (j - q >= 0) ? ld_status(status + (size_t)(j - q) * kRadix + t) : kFlagIncl;
+        int q = 0;
+        for (; q < kLookback; ++q) {
+            const uint32_t flag = sv[q] & ~kCountMask;
+            if (flag == 0u) break;               // not yet published: poll again from here
+            excl += sv[q] & kCountMask;
+            if (flag == kFlagIncl) {
+                done = true;
+                break;
+            }
+        }
+        if (!done) j -= q;
+    }
+    st_status(my, kFlagIncl | (excl + total));
+}
+
 // PAIRS = false: keys only (vin/vout unused), e.g. tile-list entries that carry
 // their payload in the key bits above the sorted ones.
 // LDS of a pass: the per-wave digit counters are dead once every pair's
@@ -302,32 +333,7 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::
             s_wcnt[k][t] = total;
             total += c;
         }
-        uint32_t* my = status + (size_t)part * kRadix + t;
-        if (part == 0) {
-            st_status(my, kFlagIncl | total);
-        } else {
-            st_status(my, kFlagAgg | total);
-            int j = part - 1;
-            bool done = false;
-            while (!done) {
-                uint32_t sv[kLookback];
-#pragma unroll
-                for (int q = 0; q < kLookback; ++q)
-                    sv[q] = (j - q >= 0) ? ld_status(status + (size_t)(j - q) * kRadix + t) : kFlagIncl;
-                int q = 0;
-                for (; q < kLookback; ++q) {
-                    const uint32_t flag = sv[q] & ~kCountMask;
-                    if (flag == 0u) break;               // not yet published: poll again from here
-                    excl += sv[q] & kCountMask;
-                    if (flag == kFlagIncl) {
-                        done = true;
-                        break;
-                    }
-                }
-                if (!done) j -= q;
-            }
-            st_status(my, kFlagIncl | (excl + total));
-        }
+        lookback(status, part, t, total, excl);
     }
     // exclusive scan of the partition totals over the 256 digits (waves 0..3)
     if (t < kRadix) {
