@@ -327,7 +327,7 @@ def main():
                               "sample": f"{len(one)} frames (poses 0..{len(one) - 1}), {sum(one):.1f} s"},
                "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
         cam = cams[0]
-        # accuracy gates (the oracle as checker): the benchmarked frame vs the FP64 render,
+        # accuracy gates (the oracle as reference): the benchmarked frame vs the FP64 render,
         # and served FP16-storage views vs the FP64 render of their FP32 scenes
         img, _ = r.render(cam, W, H)
         r64 = orc.render64(scene["position"], scene["scale_rot"], shal, scene["color"], N, 3, cam.packed(), W, H)

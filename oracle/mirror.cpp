@@ -1,7 +1,7 @@
 // oracle/mirror.cpp -- TEST INFRASTRUCTURE ONLY.
 //
 // CPU restatement of the reference's per-frame splat pipeline
-// (core/gaussian_render.cu), used as the checker for the HIP kernels.  Only
+// (core/gaussian_render.cu), against which the HIP kernels are compared.  Only
 // the tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
 // it; the product never links, imports or calls anything here.
 //

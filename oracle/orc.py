@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/mirror.cpp) -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
-module, and only as the checker / the timed CPU baseline.
+module, and only to compare the HIP path's results against / as the timed CPU baseline.
 """
 from __future__ import annotations
 

@@ -1,6 +1,6 @@
 """CPU restatement of the reference's PLY scene reader and writer -- TEST INFRASTRUCTURE ONLY.
 
-Only tests/ may import this module, as the checker of the product's gr_ply_* entry points.
+Only tests/ may import this module, to check the product's gr_ply_* entry points against.
 
 Follows core/ply_parser.cpp:
   * parse_header  -- parseHeader (:15-100): token lines, "ply" skipped (not counted),

@@ -1,6 +1,6 @@
 """GPU parity: every stage of the HIP path against the CPU oracle, through the C-ABI.
 
-Bar (north star): SH colours, projections, tile lists, sort order and ranges
+Bar: SH colours, projections, tile lists, sort order and ranges
 bit-exact; images bit-exact against the FP32 oracle (the oracle shares the
 blend arithmetic defined by this build), max |diff| <= 1 LSB and PSNR >= 50 dB
 against the FP64 render.
