@@ -154,9 +154,11 @@ struct RasterArgs {
 // consecutive Gaussians.
 constexpr int kBuildBatch = 64;
 constexpr int kBuildThreads = 256;
-// Workgroups of the record emit pass (its waves take batches grid-stride).
+// Workgroups of the record emit pass (its waves take batches grid-stride).  At 6M Gaussians
+// and 1080p, 4096 workgroups build a frame's list in 0.535 ms against 0.559 at 2048, 0.564
+// at 1024 and 0.538 at 8192.
 #ifndef GR_EMIT_BLOCKS
-#define GR_EMIT_BLOCKS 2048
+#define GR_EMIT_BLOCKS 4096
 #endif
 constexpr int kEmitBlocks = GR_EMIT_BLOCKS;
 
