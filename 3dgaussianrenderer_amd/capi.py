@@ -86,9 +86,14 @@ class FrameView(C.Structure):
                 ("tiles_x", C.c_int32), ("tiles_y", C.c_int32), ("splat_count", C.c_int32)]
 
 
+KERNEL_TIMES_MAX = 24
+TIMING = {"none": 0, "sort_passes": 1, "kernels": 2}
+
+
 class KernelTimes(C.Structure):
-    _fields_ = [("tile_sort_pass_ms", C.c_double), ("tile_sort_passes", C.c_int32), ("pad", C.c_int32),
-                ("tile_list_size", C.c_int64)]
+    _fields_ = [("tile_sort_pass_ms", C.c_double), ("tile_sort_passes", C.c_int32), ("kernels", C.c_int32),
+                ("tile_list_size", C.c_int64), ("visible", C.c_int64),
+                ("kernel_ms", C.c_double * KERNEL_TIMES_MAX), ("kernel_name", (C.c_char * 24) * KERNEL_TIMES_MAX)]
 
 
 class ServerConfig(C.Structure):
@@ -165,6 +170,7 @@ SIGNATURES = [
                                      C.POINTER(RenderStats)]),
     ("gr_renderer_frame_view", C.c_int, [_P, C.POINTER(FrameView)]),
     ("gr_renderer_kernel_times", C.c_int, [_P, C.POINTER(KernelTimes)]),
+    ("gr_renderer_set_timing", C.c_int, [_P, C.c_int32]),
     ("gr_server_create", C.c_int, [_PP, C.POINTER(ServerConfig)]),
     ("gr_server_destroy", C.c_int, [_P]),
     ("gr_server_add_scene", C.c_int, [_P, C.POINTER(ModelHost), C.POINTER(C.c_int32)]),
@@ -173,6 +179,7 @@ SIGNATURES = [
     ("gr_server_poll", C.c_int, [_P, C.POINTER(Completion), C.c_int32, C.POINTER(C.c_int32), C.c_int32]),
     ("gr_server_get_stats", C.c_int, [_P, C.POINTER(ServerStats)]),
     ("gr_server_clock_ms", C.c_double, [_P]),
+    ("gr_server_last_error", C.c_int, [_P, C.c_char_p, C.c_int32]),
 ]
 
 _lib = None
@@ -413,22 +420,29 @@ class Renderer:
         m = model_host(scene, precision)
         check(lib().gr_renderer_upload_model(self.h, C.byref(m)))
 
-    def render(self, cam: CameraData, width: int, height: int, out=None):
-        """Render into host memory (numpy, returned) or into a device pointer (int)."""
+    def render(self, cam: CameraData, width: int, height: int, out=None, stats: bool = True):
+        """Render into host memory (numpy, returned) or into a device pointer (int).  With
+        stats=False the frame records no per-stage events and the returned stats are None."""
         st = RenderStats()
+        sp = C.byref(st) if stats else None
         if out is None or isinstance(out, np.ndarray):
             img = out if out is not None else np.empty((height, width, 4), np.uint8)
-            check(lib().gr_renderer_render(self.h, C.byref(cam), width, height, _fp(img), 0, C.byref(st)))
-            return img, st.as_dict()
-        check(lib().gr_renderer_render(self.h, C.byref(cam), width, height, C.c_void_p(int(out)), 1, C.byref(st)))
-        return None, st.as_dict()
+            check(lib().gr_renderer_render(self.h, C.byref(cam), width, height, _fp(img), 0, sp))
+            return img, (st.as_dict() if stats else None)
+        check(lib().gr_renderer_render(self.h, C.byref(cam), width, height, C.c_void_p(int(out)), 1, sp))
+        return None, (st.as_dict() if stats else None)
+
+    def set_timing(self, level: str):
+        """'none', 'sort_passes' (two events around the tile-sort passes) or 'kernels'."""
+        check(lib().gr_renderer_set_timing(self.h, TIMING[level]))
 
     def kernel_times(self) -> dict:
-        """Tile-bit sort pass timing of the last frame (HIP events on the renderer's stream)."""
+        """Kernel timing of the last frame (HIP events on the renderer's stream)."""
         t = KernelTimes()
         check(lib().gr_renderer_kernel_times(self.h, C.byref(t)))
         return {"tile_sort_pass_ms": t.tile_sort_pass_ms, "tile_sort_passes": t.tile_sort_passes,
-                "tile_list_size": t.tile_list_size}
+                "tile_list_size": t.tile_list_size, "visible": t.visible,
+                "kernel_ms": {t.kernel_name[i].value.decode(): t.kernel_ms[i] for i in range(t.kernels)}}
 
     def frame_view(self) -> FrameView:
         v = FrameView()
@@ -513,3 +527,9 @@ class Server:
 
     def clock_ms(self) -> float:
         return lib().gr_server_clock_ms(self.h)
+
+    def last_error(self) -> str:
+        """Message of the last request that completed with an error status."""
+        buf = C.create_string_buffer(1024)
+        check(lib().gr_server_last_error(self.h, buf, len(buf)))
+        return buf.value.decode()

@@ -64,6 +64,7 @@ gr_status grow(T*& p, size_t& cap, size_t need, double headroom = 1.0) {
 
 namespace gr_detail {
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+std::string get_last_error() { return g_last_error; }
 }  // namespace gr_detail
 
 static gr_status timer_start(gr_context* ctx, gr_timer* t) {
@@ -361,7 +362,11 @@ gr_status gr_rasterize_tile(gr_context* ctx, gr_timer* timer) {
         return fail(GR_ERR_INVALID_ARGUMENT, "global/tile-list args not set");
     const gr_global_args& g = ctx->args;
     if (!g.back_buffer || !g.tile_range) return fail(GR_ERR_INVALID_ARGUMENT, "raster buffers are NULL");
-    RasterArgs a;
+    // the kernel gathers every entry's Gaussian through values[] and reads its clip position,
+    // conic and colour: a missing one is refused here, before anything is launched
+    if (!ctx->tl.values || !g.position_clip_xy || !g.conic || !g.color)
+        return fail(GR_ERR_INVALID_ARGUMENT, "raster needs tile-list values, clip positions, conics and colours");
+    RasterArgs a{};
     a.v0 = ctx_view(ctx);
     a.views = nullptr;
     a.nviews = 1;

@@ -28,6 +28,8 @@ struct gr_context {
 namespace gr_detail {
 // Records the message returned by gr_last_error() on this thread.
 void set_last_error(const std::string& msg);
+// The message gr_last_error() returns on this thread.
+std::string get_last_error();
 }  // namespace gr_detail
 
 // Camera block of the C-ABI -> kernel camera arguments.

@@ -194,7 +194,7 @@ GR_HD int tile_bits_for(int tiles) {
 // own polynomial so that the GPU and the host oracle agree bit for bit:
 //   n = rint(e), f = e - n in [-1/2, 1/2], 2^f ~ p(f), p(0) = 1 exactly, a
 //   degree-5 near-minimax relative-error fit (Lawson-weighted least squares on
-//   Chebyshev nodes, csrc/fit_exp2.py; max relative error 1.7e-7 with FP32
+//   Chebyshev nodes, tests/golden/fit_exp2.py; max relative error 1.7e-7 with FP32
 //   Horner/FMA evaluation), result = ldexp(p(f), n).  p(f) <= 1 for f <= 0,
 //   so the density never exceeds 1.
 // Callers pass e in [-26, 0] (see blend_alpha below), so n fits an int

@@ -88,10 +88,21 @@ gr_status rfail(gr_status st, const std::string& msg) {
 }
 }  // namespace
 
+constexpr int kMaxMarks = 24;
+
 struct gr_renderer {
     int device = 0;
     gr_context* ctx = nullptr;
-    hipEvent_t ev[9] = {};
+    hipEvent_t ev[9] = {};          // per-stage events (only when stats are requested) + frame end
+    hipEvent_t pass_ev[2] = {};     // around the tile-sort passes (timing level >= 1, no stats)
+    hipEvent_t mark_ev[kMaxMarks] = {};   // per-kernel marks (timing level 2)
+    int32_t timing = GR_TIMING_NONE;
+    // per-kernel marks of the last frame: event handle and the name of the work before it
+    int32_t marks = 0;
+    hipEvent_t mark_at[kMaxMarks] = {};
+    const char* mark_name[kMaxMarks] = {};
+    hipEvent_t pass_begin = nullptr, pass_end = nullptr;   // events that bracketed the last passes
+    int64_t last_visible = -1;
     int32_t last_passes = 0;
     int32_t n = 0, degree = 0, sh_count = 3, precision = GR_PRECISION_FP32;
     DevBuf<float4> position, scale_rot, color, conic, ellipse;
@@ -137,6 +148,8 @@ gr_status gr_renderer_create(gr_renderer** out, int32_t device, void* stream) {
     // separates); the last one, after the image and K copies, keeps it: the host reads those.
     for (int k = 0; k < 8; ++k) R_HIP(hipEventCreateWithFlags(&r->ev[k], hipEventDisableSystemFence));
     R_HIP(hipEventCreate(&r->ev[8]));
+    for (auto& e : r->pass_ev) R_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    for (auto& e : r->mark_ev) R_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     *out = r;
     return GR_OK;
 }
@@ -155,6 +168,8 @@ gr_status gr_renderer_destroy(gr_renderer* r) {
     r->records.release(); r->rec_counts.release(); r->rec_offsets.release(); r->rec_masks.release();
     if (r->sort_temp) (void)hipFree(r->sort_temp);
     for (auto& e : r->ev) if (e) (void)hipEventDestroy(e);
+    for (auto& e : r->pass_ev) if (e) (void)hipEventDestroy(e);
+    for (auto& e : r->mark_ev) if (e) (void)hipEventDestroy(e);
     gr_context_destroy(r->ctx);
     delete r;
     return GR_OK;
@@ -280,7 +295,22 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     uint32_t* vis_total = r->vis_counts.p + batches;
     uint32_t* k_total = r->batch_counts.p;
 
-    R_HIP(hipEventRecord(r->ev[0], stream));
+    // Timing.  Per-stage events run only when stats are requested; per-kernel marks only at
+    // GR_TIMING_KERNELS; the tile-sort passes are bracketed at GR_TIMING_SORT_PASSES and up (and
+    // by the stage events, which share them).  Each event leaves the GPU idle for a few
+    // microseconds, so a frame asked for none of them records none.
+    const bool stages = stats != nullptr;
+    const bool kmarks = r->timing >= GR_TIMING_KERNELS;
+    r->marks = 0;
+    auto stage = [&](int k) -> hipError_t { return stages ? hipEventRecord(r->ev[k], stream) : hipSuccess; };
+    auto mark = [&](const char* name) -> hipError_t {
+        if (!kmarks || r->marks >= kMaxMarks) return hipSuccess;
+        r->mark_at[r->marks] = r->mark_ev[r->marks];
+        r->mark_name[r->marks] = name;
+        return hipEventRecord(r->mark_ev[r->marks++], stream);
+    };
+    R_HIP(stage(0));
+    R_HIP(mark("begin"));
     if (r->degree > 0 && n > 0) {
         ShArgs a{};
         a.position = r->position.p;
@@ -293,7 +323,8 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         a.v0 = vd;
         R_HIP(launch_sh(a, r->degree, r->precision, stream));
     }
-    R_HIP(hipEventRecord(r->ev[1], stream));
+    R_HIP(stage(1));
+    R_HIP(mark("sh"));
     {
         ProjectArgs a{};
         a.position = r->position.p;
@@ -312,7 +343,8 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         a.v0 = vd;
         R_HIP(launch_project(a, stream));
     }
-    R_HIP(hipEventRecord(r->ev[2], stream));
+    R_HIP(stage(2));
+    R_HIP(mark("project"));
     // visible Gaussians -> build records + (depth word, record) pairs
     R_HIP(launch_build_scan(r->vis_counts.p, batches, vis_total, vis_total + 1, stream));
     {
@@ -329,6 +361,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         a.nviews = 1;
         R_HIP(launch_compact_depth(a, stream));
     }
+    R_HIP(mark("compact"));
     // exact entry count and candidate mask of every record (index order)
     BuildArgs b{};
     b.records = r->records.p;
@@ -344,7 +377,8 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     b.nviews = 1;
     b.v0 = vd;
     R_HIP(launch_build_count(b, stream));
-    R_HIP(hipEventRecord(r->ev[3], stream));
+    R_HIP(stage(3));
+    R_HIP(mark("record_count"));
     // depth order of the records
     int32_t dsel = 0;
     {
@@ -352,28 +386,47 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         int32_t* dv[2] = {r->dorder[0].p, r->dorder[1].p};
         R_HIP(radix_sort_pairs_u32(dk, dv, &dsel, n, vis_total, 0, 32, r->sort_temp, r->sort_temp_bytes, stream));
     }
-    R_HIP(hipEventRecord(r->ev[4], stream));
+    R_HIP(stage(4));
+    R_HIP(mark("depth_sort"));
     // output offsets: the counts carried by the sorted words, scanned in depth order (total K)
     // and scattered back to the records
     R_HIP(launch_build_scan_packed(reinterpret_cast<const uint32_t*>(r->dorder[dsel].p), n, vis_total,
                                    r->rec_counts.p, b.index_bits, k_total, k_total + 1, r->rec_offsets.p, stream));
+    R_HIP(mark("offsets"));
     b.record_offsets = r->rec_offsets.p;
     const int tile_bits = tile_bits_for(tiles);
     R_HIP(radix_sort_prepare(capacity, r->sort_temp, r->sort_temp_bytes, &b.tile_hist, stream));
     b.tile_passes = (tile_bits + 7) / 8;
     R_HIP(launch_build_emit(b, stream));
-    // tile bits: the sort records ev[5] after its histogram scan and ev[6] after its passes, so
-    // the interval is the passes alone (each event leaves the GPU idle for a few microseconds;
-    // the same two serve the stage split and the pass timing)
+    R_HIP(mark("emit"));
+    // tile bits: the sort records its begin event after its histogram scan and its end event
+    // after its passes, so the interval is the passes alone (the stage split shares the two)
     int32_t sel = 0;
     {
+        hipEvent_t pb = nullptr, pe = nullptr;
+        if (stages) {
+            pb = r->ev[5];
+            pe = r->ev[6];
+        } else if (r->timing >= GR_TIMING_SORT_PASSES) {
+            pb = r->pass_ev[0];
+            pe = r->pass_ev[1];
+        }
         uint64_t* kb[2] = {r->entries[0].p, r->entries[1].p};
         R_HIP(radix_sort_keys_prepared(kb, &sel, capacity, k_total, 0, tile_bits, r->sort_temp, r->sort_temp_bytes,
-                                       stream, r->ev[5], r->ev[6]));
+                                       stream, pb, pe));
         r->last_passes = capacity > 1 ? (tile_bits + 7) / 8 : 0;
-        if (r->last_passes == 0) {   // no passes ran: the events still bracket the (empty) sort
-            R_HIP(hipEventRecord(r->ev[5], stream));
-            R_HIP(hipEventRecord(r->ev[6], stream));
+        if (r->last_passes == 0 && pb) {   // no passes ran: the events still bracket the (empty) sort
+            R_HIP(hipEventRecord(pb, stream));
+            R_HIP(hipEventRecord(pe, stream));
+        }
+        r->pass_begin = pb;
+        r->pass_end = pe;
+        if (kmarks && pb && r->marks + 2 <= kMaxMarks) {
+            // the pass events double as marks: histogram scan, then the passes
+            r->mark_at[r->marks] = pb;
+            r->mark_name[r->marks++] = "tile_sort_scan";
+            r->mark_at[r->marks] = pe;
+            r->mark_name[r->marks++] = "tile_sort_passes";
         }
     }
     r->sel = sel;
@@ -388,6 +441,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         ra.tiles = tiles;
         R_HIP(launch_ranges(ra, stream));
     }
+    R_HIP(mark("ranges"));
     {
         RasterArgs a{};
         a.tile_range = r->tile_range.p;
@@ -402,10 +456,12 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         a.v0 = vd;
         R_HIP(launch_raster(a, stream));
     }
-    R_HIP(hipEventRecord(r->ev[7], stream));
+    R_HIP(stage(7));
+    R_HIP(mark("raster"));
     if (!out_is_device)
         R_HIP(hipMemcpyAsync(out_rgba8, outp, (size_t)width * height * 4, hipMemcpyDeviceToHost, stream));
     R_HIP(hipMemcpyAsync(ctx->pinned, k_total, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    if (kmarks) R_HIP(hipMemcpyAsync(ctx->pinned + 1, vis_total, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     R_HIP(hipEventRecord(r->ev[8], stream));
     R_HIP(hipEventSynchronize(r->ev[8]));
     const int64_t need = (int64_t)ctx->pinned[0];
@@ -417,6 +473,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         return gr_renderer_render(r, cam, width, height, out_rgba8, out_is_device, stats);
     }
     r->last_k = need;
+    r->last_visible = kmarks ? (int64_t)ctx->pinned[1] : -1;
     r->last_w = width;
     r->last_h = height;
     if (stats) {
@@ -436,15 +493,34 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     return GR_OK;
 }
 
+gr_status gr_renderer_set_timing(gr_renderer* r, int32_t level) {
+    if (!r || level < GR_TIMING_NONE || level > GR_TIMING_KERNELS)
+        return rfail(GR_ERR_INVALID_ARGUMENT, "gr_renderer_set_timing: bad argument");
+    r->timing = level;
+    return GR_OK;
+}
+
 gr_status gr_renderer_kernel_times(gr_renderer* r, gr_kernel_times* out) {
     if (!r || !out) return rfail(GR_ERR_INVALID_ARGUMENT, "gr_renderer_kernel_times: NULL argument");
     *out = gr_kernel_times{};
     out->tile_sort_passes = r->last_passes;
     out->tile_list_size = r->last_k;
-    if (r->last_passes > 0) {
+    out->visible = r->last_visible;
+    if (r->last_passes > 0 && r->pass_begin && r->pass_end) {
         float ms = 0.0f;
-        R_HIP(hipEventElapsedTime(&ms, r->ev[5], r->ev[6]));
+        R_HIP(hipEventElapsedTime(&ms, r->pass_begin, r->pass_end));
         out->tile_sort_pass_ms = ms;
+    } else {
+        out->tile_sort_pass_ms = -1.0;
+    }
+    // mark k closes the work named mark_name[k] that began at mark k - 1
+    out->kernels = 0;
+    for (int k = 1; k < r->marks && out->kernels < GR_KERNEL_TIMES_MAX; ++k) {
+        float ms = 0.0f;
+        R_HIP(hipEventElapsedTime(&ms, r->mark_at[k - 1], r->mark_at[k]));
+        const int i = out->kernels++;
+        out->kernel_ms[i] = ms;
+        std::snprintf(out->kernel_name[i], sizeof(out->kernel_name[i]), "%s", r->mark_name[k]);
     }
     return GR_OK;
 }

@@ -72,13 +72,14 @@ gr_status sfail(gr_status st, const std::string& msg) {
         if (s_ != GR_OK) return s_;  \
     } while (0)
 
-// device allocation with byte accounting
+// device allocation with byte accounting; workspaces grow with 25 % headroom, resident
+// scene buffers (exact = true) take exactly the bytes admitted for them
 struct Dev {
     void* p = nullptr;
     size_t bytes = 0;
-    gr_status ensure(size_t need, std::atomic<int64_t>& acct, int* grows = nullptr) {
+    gr_status ensure(size_t need, std::atomic<int64_t>& acct, int* grows = nullptr, bool exact = false) {
         if (need <= bytes && p) return GR_OK;
-        const size_t want = std::max<size_t>(need + need / 4, 256);
+        const size_t want = exact ? std::max<size_t>(need, 1) : std::max<size_t>(need + need / 4, 256);
         if (p) {
             S_HIP(hipFree(p));
             acct -= (int64_t)bytes;
@@ -136,6 +137,9 @@ struct Scene {
     int64_t bytes = 0;
     int inflight = 0;
     bool removing = false;
+    void release(std::atomic<int64_t>& acct) {
+        for (Dev* d : {&position, &scale_rot, &packed, &sh, &color0}) d->release(acct);
+    }
 };
 
 struct Pending {
@@ -248,6 +252,14 @@ struct gr_server {
     std::thread worker;
     std::atomic<bool> stop{false};
     std::string fatal;
+    // Admission is check-then-allocate: the scheduler (workspaces) and add_scene callers
+    // (resident scenes) each hold this lock from the budget check through the allocations it
+    // admits, so concurrent admissions cannot both pass against the same headroom.
+    std::mutex acct_mu;
+    // message of the last request that completed with an error (the scheduler thread's
+    // failures never reach the submitting thread's gr_last_error)
+    std::mutex err_mu;
+    std::string last_error;
 
     double now_ms() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -288,6 +300,11 @@ Scene* scene_ptr(gr_server* s, int32_t id) {
 
 void complete(gr_server* s, Slot& sl, int32_t status) {
     const double t = s->now_ms();
+    if (status != GR_OK) {
+        // the failing step left its message on this (the scheduler's) thread
+        std::lock_guard<std::mutex> lk(s->err_mu);
+        s->last_error = gr_detail::get_last_error();
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     for (const Pending& p : sl.batch) {
         gr_completion c{};
@@ -751,6 +768,7 @@ bool start_batch(gr_server* s, Slot& sl) {
     sl.tiles_total = tiles;
     sl.img_bytes = img;
     sl.tile_bits = tile_bits_for(tiles);
+    std::unique_lock<std::mutex> acct(s->acct_mu);   // budget check through the allocations it admits
     const int64_t growth = phase_a_growth(sl, *sc, sl.batch.size(), img, tiles);
     if (s->over_budget(growth)) {
         if (other_slots_busy(s, sl)) {
@@ -797,6 +815,7 @@ bool start_batch(gr_server* s, Slot& sl) {
         sl.batch.clear();
         return false;
     }
+    acct.unlock();
     int64_t off = 0;
     for (size_t v = 0; v < vds.size(); ++v) {
         vds[v].out = reinterpret_cast<uchar4*>(sl.ws.images.as<uint8_t>() + off);
@@ -836,6 +855,7 @@ bool start_phase_b(gr_server* s, Slot& sl) {
         return w.list0.growth(need * 8) + w.list1.growth(need * 8) + w.sort_temp.growth(sort_after);
     };
     size_t need = std::max<size_t>(sl.k, 1);
+    std::unique_lock<std::mutex> acct(s->acct_mu);   // budget check through the allocations it admits
     int64_t grow = growth_for(need);
     if (grow > 0 && s->over_budget(grow)) {
         if (other_slots_running(s, sl)) {
@@ -897,6 +917,7 @@ bool start_phase_b(gr_server* s, Slot& sl) {
         sl.state = SlotState::Idle;
         return true;
     }
+    acct.unlock();
     s->note_peak();
     s->note_realloc(grows, t0);
     if (grows) ++w.generation;
@@ -1061,9 +1082,8 @@ gr_status gr_server_destroy(gr_server* s) {
         (void)hipStreamDestroy(sl.stream);
     }
     for (auto& kv : s->scenes) {
-        Scene* sc = kv.second;
-        for (Dev* d : {&sc->position, &sc->scale_rot, &sc->packed, &sc->sh, &sc->color0}) d->release(s->resident);
-        delete sc;
+        kv.second->release(s->resident);
+        delete kv.second;
     }
     delete s;
     return GR_OK;
@@ -1083,39 +1103,51 @@ gr_status gr_server_add_scene(gr_server* s, const gr_model_host* m, int32_t* sce
     sc->precision = m->precision;
     const size_t groups = ((size_t)n + kShGroup - 1) / kShGroup;
     const size_t shn = groups * kShGroup * (size_t)sc->sh_count;
-    int64_t bytes = (int64_t)n * 16 + (m->precision ? (int64_t)n * 8 : (int64_t)n * 16) +
-                    (m->sh_degree > 0 ? (int64_t)shn * (m->precision ? 2 : 4) : (int64_t)n * 16);
-    if (s->over_budget(bytes)) return sfail(GR_ERR_BUDGET, "scene does not fit the HBM budget");
+    const size_t pos_b = (size_t)n * 16;
+    const size_t sr_b = m->precision ? 0 : (size_t)n * 16;
+    const size_t pk_b = m->precision ? (size_t)n * 8 : 0;
+    const size_t sh_b = m->sh_degree > 0 ? shn * (m->precision ? 2 : 4) : 0;
+    const size_t c0_b = m->sh_degree == 0 ? (size_t)n * 16 : 0;
+    const int64_t bytes = (int64_t)(pos_b + sr_b + pk_b + sh_b + c0_b);
     std::atomic<int64_t>& acct = s->resident;
+    // frees whatever was allocated when a later step fails (and un-counts it)
+    struct Guard {
+        Scene* sc;
+        std::atomic<int64_t>& acct;
+        ~Guard() {
+            if (sc) sc->release(acct);
+        }
+    } guard{sc.get(), acct};
+    {
+        // the buffers take exactly the admitted bytes; the check and the allocations run under
+        // the admission lock, so a concurrent workspace growth cannot use the same headroom
+        std::lock_guard<std::mutex> alk(s->acct_mu);
+        if (s->over_budget(bytes)) return sfail(GR_ERR_BUDGET, "scene does not fit the HBM budget");
+        S_CHECK(sc->position.ensure(pos_b, acct, nullptr, true));
+        if (sr_b) S_CHECK(sc->scale_rot.ensure(sr_b, acct, nullptr, true));
+        if (pk_b) S_CHECK(sc->packed.ensure(pk_b, acct, nullptr, true));
+        if (sh_b) S_CHECK(sc->sh.ensure(sh_b, acct, nullptr, true));
+        if (c0_b) S_CHECK(sc->color0.ensure(c0_b, acct, nullptr, true));
+    }
     if (m->precision) {
         std::vector<float> pos((size_t)4 * n);
         std::vector<uint16_t> pk((size_t)4 * n);
         std::vector<uint16_t> sh16(m->sh_degree > 0 ? shn : 1);
         S_CHECK(gr_pack_precision_model(m, m->precision, pos.data(), pk.data(), sh16.data()));
-        S_CHECK(sc->position.ensure((size_t)n * 16, acct));
-        S_CHECK(sc->packed.ensure((size_t)n * 8, acct));
-        S_HIP(hipMemcpy(sc->position.p, pos.data(), (size_t)n * 16, hipMemcpyHostToDevice));
-        S_HIP(hipMemcpy(sc->packed.p, pk.data(), (size_t)n * 8, hipMemcpyHostToDevice));
-        if (m->sh_degree > 0) {
-            S_CHECK(sc->sh.ensure(shn * 2, acct));
-            S_HIP(hipMemcpy(sc->sh.p, sh16.data(), shn * 2, hipMemcpyHostToDevice));
-        }
+        S_HIP(hipMemcpy(sc->position.p, pos.data(), pos_b, hipMemcpyHostToDevice));
+        S_HIP(hipMemcpy(sc->packed.p, pk.data(), pk_b, hipMemcpyHostToDevice));
+        if (sh_b) S_HIP(hipMemcpy(sc->sh.p, sh16.data(), sh_b, hipMemcpyHostToDevice));
     } else {
-        S_CHECK(sc->position.ensure((size_t)n * 16, acct));
-        S_CHECK(sc->scale_rot.ensure((size_t)n * 16, acct));
-        S_HIP(hipMemcpy(sc->position.p, m->position, (size_t)n * 16, hipMemcpyHostToDevice));
-        S_HIP(hipMemcpy(sc->scale_rot.p, m->scale_and_rotation, (size_t)n * 16, hipMemcpyHostToDevice));
-        if (m->sh_degree > 0) {
+        S_HIP(hipMemcpy(sc->position.p, m->position, pos_b, hipMemcpyHostToDevice));
+        S_HIP(hipMemcpy(sc->scale_rot.p, m->scale_and_rotation, sr_b, hipMemcpyHostToDevice));
+        if (sh_b) {
             std::vector<float> al(shn);
             S_CHECK(gr_realign_spherical_harmonics(m->sh, al.data(), kShGroup, sc->sh_count, n));
-            S_CHECK(sc->sh.ensure(shn * 4, acct));
-            S_HIP(hipMemcpy(sc->sh.p, al.data(), shn * 4, hipMemcpyHostToDevice));
+            S_HIP(hipMemcpy(sc->sh.p, al.data(), sh_b, hipMemcpyHostToDevice));
         }
     }
-    if (m->sh_degree == 0) {
-        S_CHECK(sc->color0.ensure((size_t)n * 16, acct));
-        S_HIP(hipMemcpy(sc->color0.p, m->color, (size_t)n * 16, hipMemcpyHostToDevice));
-    }
+    if (c0_b) S_HIP(hipMemcpy(sc->color0.p, m->color, c0_b, hipMemcpyHostToDevice));
+    guard.sc = nullptr;
     s->note_peak();
     std::lock_guard<std::mutex> lk(s->mu);
     sc->id = s->next_scene++;
@@ -1140,7 +1172,7 @@ gr_status gr_server_remove_scene(gr_server* s, int32_t id) {
         s->scenes.erase(it);
     }
     // graphs are keyed by scene id and ids are never reused, so stale graphs are never replayed
-    for (Dev* d : {&sc->position, &sc->scale_rot, &sc->packed, &sc->sh, &sc->color0}) d->release(s->resident);
+    sc->release(s->resident);
     delete sc;
     return GR_OK;
 }
@@ -1197,5 +1229,14 @@ gr_status gr_server_get_stats(gr_server* s, gr_server_stats* o) {
 }
 
 double gr_server_clock_ms(gr_server* s) { return s ? s->now_ms() : 0.0; }
+
+gr_status gr_server_last_error(gr_server* s, char* buf, int32_t buf_bytes) {
+    if (!s || !buf || buf_bytes <= 0) return sfail(GR_ERR_INVALID_ARGUMENT, "gr_server_last_error: bad argument");
+    std::lock_guard<std::mutex> lk(s->err_mu);
+    const size_t n = std::min<size_t>(s->last_error.size(), (size_t)buf_bytes - 1);
+    std::memcpy(buf, s->last_error.data(), n);
+    buf[n] = '\0';
+    return GR_OK;
+}
 
 }  // extern "C"

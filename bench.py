@@ -8,15 +8,22 @@ the centroid, 8 seeded poses cycled frame by frame.  One step = one full frame
 HBM and the image left in HBM.
 
 Output: one JSON line (the driver's contract) with the per-stage breakdown, a
-roofline object for the dominant kernel and the CPU oracle timed on a bounded
-sample of the same workload.
+roofline object for the dominant kernel, a per-kernel roofline table, the
+serving workloads (C4 Poisson sweep, C5 bursty under its HBM budget) and the
+CPU oracle timed on a bounded sample of the same workload.
 
-Roofline: the dominant kernel is one pass of the tile-bit onesweep sort (the
-largest single-kernel time of the frame).  The renderer's tile list holds
-8-byte entries (Gaussian index << 32 | tile id; the entries are in depth
-order before the tile sort, so the depth word needs no storage), sorted on
-the tile bits, and a pass reads and writes each entry once: 16 B per
-tile-list entry, so
+Timed loops: the headline frames record no per-stage events (stats off); the
+only events in them are the two around the tile-sort passes that the roofline
+object reads.  A second loop renders the same frames with per-stage stats
+(RenderStats) and a third with an event after every kernel (the per-kernel
+table); both are reported beside the headline, never as it.
+
+Roofline: the dominant stage is the tile-bit onesweep sort, two launches per
+frame (one launch is shorter than SH, record_count or raster, but the pair
+takes the most time of any stage).  The renderer's tile list holds 8-byte
+entries (Gaussian index << 32 | tile id; the entries are in depth order before
+the tile sort, so the depth word needs no storage), sorted on the tile bits,
+and a pass reads and writes each entry once: 16 B per tile-list entry, so
 achieved = 16 * K / (average pass duration), the duration measured with HIP
 events around the passes on the renderer's stream in every timed frame.
 `traffic` is the measured HBM bytes per pass launch, taken from the committed
@@ -51,13 +58,18 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--poses", type=int, default=8)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--kernel-frames", type=int, default=8, help="frames timed kernel by kernel")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="time the CPU oracle on a bounded sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--serving", default="c4", help="serving workload: c4, c5 or none")
+    ap.add_argument("--serving", default="c4,c5", help="serving workloads, comma separated (c4, c5), or none")
     ap.add_argument("--serving-requests", type=int, default=1200)
-    ap.add_argument("--serving-rate", type=float, default=400.0)
-    ap.add_argument("--serving-rate-high", type=float, default=1200.0,
-                    help="second Poisson rate (about three quarters of the saturated views/s)")
+    ap.add_argument("--serving-rates", default="400,800,1200,1600,2000,2400,3200",
+                    help="C4 Poisson arrival rates (req/s), swept upwards until p99 passes --p99-bound-ms")
+    ap.add_argument("--p99-bound-ms", type=float, default=50.0)
+    ap.add_argument("--c5-binding-budget-gib", type=float, default=24.0,
+                    help="C5 again under resident scenes + this many GiB (a budget that binds)")
+    ap.add_argument("--serving-accuracy", type=int, default=1,
+                    help="one served view per resolution against the FP64 render")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU oracle sample length (frames cycle poses)")
     ap.add_argument("--traffic-profile", default=str(ROOT / "profiles" / "tile_sort_pass_6M_1080p.md"),
                     help="rocprofv3 summary holding the measured HBM bytes per tile-sort pass")
@@ -168,44 +180,135 @@ def stage_path_fps(gr, torch, scene, cams, W, H, frames):
     return fps
 
 
-def serving_leg(gr, workload: str, n_req: int, rate: float, rate_high: float = 0.0):
-    """Continuous-batching server on the serving workload vs sequential plain renders."""
-    from importlib import import_module
-    sv = import_module("3dgaussianrenderer_amd.serving")
+def rounded(d: dict) -> dict:
+    return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items() if k != "images"}
+
+
+def served_accuracy(gr, scenes, reqs, precision: int):
+    """One served view per resolution of the workload against the FP64 render of its FP32 scene."""
+    from oracle import orc
+    srv = gr.Server(slots=1, max_views=4, use_graphs=True)
+    per_res = {}
+    for (i, c, Wv, Hv) in reqs:
+        key = f"{Wv}x{Hv}"
+        if key in per_res:
+            continue
+        sc = scenes[i]
+        sid = srv.add_scene(sc, precision)
+        rid = srv.submit(sid, c, Wv, Hv)
+        _, simg = srv.wait_all([rid])[rid]
+        al = orc.realign_sh(sc["sh"], sc["count"], 48)
+        ref = orc.render64(sc["position"], sc["scale_rot"], al, sc["color"], sc["count"], 3, c.packed(), Wv, Hv)
+        dmax = int(np.abs(simg[..., :3].astype(int) - ref["image"][..., :3].astype(int)).max())
+        per_res[key] = {"psnr_db": round(orc.psnr(simg, ref["image"]), 2), "max_abs_diff_lsb": dmax,
+                        "gaussians": int(sc["count"])}
+        srv.remove_scene(sid)
+        if len(per_res) == 4:
+            break
+    srv.close()
+    return per_res
+
+
+def serving_leg(gr, workload: str, args):
+    """Continuous-batching server on one serving workload vs sequential plain renders."""
+    sv = importlib.import_module("3dgaussianrenderer_amd.serving")
+    n_req = args.serving_requests
     t0 = time.perf_counter()
     scenes = sv.make_scenes(workload)
     gen_s = time.perf_counter() - t0
     precision = 1  # FP16 storage, FP32 arithmetic (the mixed-precision serving form)
     budget = 200 * (1 << 30) if workload == "c5" else 0
-    srv = gr.Server(slots=3, max_views=64 if workload == "c5" else 16, use_graphs=True, budget_bytes=budget)
+    max_views = 64 if workload == "c5" else 16
+    srv = gr.Server(slots=3, max_views=max_views, use_graphs=True, budget_bytes=budget)
+    t0 = time.perf_counter()
     ids = [srv.add_scene(sc, precision) for sc in scenes]
-    reqs = sv.make_requests(scenes, n_req, seed=11)
+    add_s = time.perf_counter() - t0
     warm = sv.make_requests(scenes, 64, seed=99)
     sv.run_load(srv, ids, warm, rate=None)
     out = {"workload": workload, "scenes": len(scenes), "gaussians": int(sum(s["count"] for s in scenes)),
-           "precision": "fp16 storage, fp32 arithmetic", "setup_generate_s": round(gen_s, 2)}
+           "precision": "fp16 storage, fp32 arithmetic", "hbm_budget_bytes": budget,
+           "setup_s": {"generate": round(gen_s, 2), "add_scenes": round(add_s, 2)}}
     if workload == "c5":
         arr = sv.make_bursty(scenes, n_bursts=max(1, n_req // 68))
         res = sv.run_load(srv, ids, [(i, c, W, H) for (_, i, c, W, H) in arr], arrivals=[a[0] for a in arr])
-        out["bursty"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items() if k != "images"}
+        out["bursty"] = rounded(res)
     else:
-        res = sv.run_load(srv, ids, reqs, rate=rate)
-        if rate_high > 0:
-            hi = sv.run_load(srv, ids, sv.make_requests(scenes, n_req, seed=13), rate=rate_high)
-            out[f"poisson_{int(rate_high)}"] = {k: (round(v, 3) if isinstance(v, float) else v)
-                                                for k, v in hi.items() if k != "images"}
-        out[f"poisson_{int(rate)}"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items()
-                                       if k != "images"}
+        # Poisson arrivals, swept upwards until p99 passes the bound or the server falls behind
+        sweep = []
+        for k, rate in enumerate(float(x) for x in args.serving_rates.split(",")):
+            res = rounded(sv.run_load(srv, ids, sv.make_requests(scenes, n_req, seed=13 + k), rate=rate))
+            res["rate"] = rate
+            sweep.append(res)
+            if res["p99_ms"] > args.p99_bound_ms or res["completed_ok"] < n_req:
+                break
+        out["poisson_sweep"] = sweep
+        ok = [x for x in sweep if x["p99_ms"] <= args.p99_bound_ms and x["completed_ok"] == n_req]
+        out["p99_bound_ms"] = args.p99_bound_ms
+        out["best_views_per_s_within_p99_bound"] = max((x["views_per_s"] for x in ok), default=None)
     sat_reqs = sv.make_requests(scenes, n_req, seed=12)
     sat = sv.run_load(srv, ids, sat_reqs, rate=None)
-    out["saturated"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in sat.items() if k != "images"}
+    out["saturated"] = rounded(sat)
     out["server_stats"] = srv.stats()
     srv.close()
-    # the same requests (a prefix of the saturated list), one plain FP32 render at a time
+    if workload == "c5" and args.c5_binding_budget_gib > 0:
+        # the same saturated requests under a budget that binds: resident scenes + a few GiB
+        resident = out["server_stats"]["resident_bytes"]
+        tight = int(resident + args.c5_binding_budget_gib * (1 << 30))
+        srv = gr.Server(slots=3, max_views=max_views, use_graphs=True, budget_bytes=tight)
+        ids = [srv.add_scene(sc, precision) for sc in scenes]
+        res = rounded(sv.run_load(srv, ids, sat_reqs, rate=None))
+        st = srv.stats()
+        srv.close()
+        res.update(hbm_budget_bytes=tight, peak_bytes=st["peak_bytes"], budget_waits=st["budget_waits"],
+                   within_budget=st["peak_bytes"] <= tight)
+        out["saturated_binding_budget"] = res
+    # the same requests (a prefix of the saturated list), one plain FP32 render at a time into a
+    # reused pinned host buffer per resolution
     seq = sv.run_sequential(scenes, sat_reqs[: max(50, n_req // 4)], precision=0)
-    out["sequential_plain_fp32"] = {k: round(v, 3) for k, v in seq.items()}
+    out["sequential_plain_fp32"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in seq.items()}
     out["speedup_vs_sequential"] = round(sat["views_per_s"] / seq["views_per_s"], 2)
-    return out, scenes, reqs
+    if args.serving_accuracy:
+        per_res = served_accuracy(gr, scenes, sat_reqs, precision)
+        out["served_fp16_vs_fp64_per_resolution"] = per_res
+        out["served_fp16_min_psnr_vs_fp64_db"] = min(v["psnr_db"] for v in per_res.values())
+    return out
+
+
+# algorithmic bytes per frame of each timed interval (FP32 storage, SH degree 3): N Gaussians,
+# V visible (non-empty tile rect), K tile-list entries, T tiles, P tile-sort passes
+def kernel_bytes(N, V, K, T, P, W, H):
+    return {
+        "sh": 224.0 * N,                      # position 16 + 48 coefficients 192 in, colour 16 out
+        "project": 72.0 * N,                  # position + scale/rotation 32 in; xy, z, ellipse, conic 40 out
+        "compact": 64.0 * V,                  # xy, z, ellipse 28 in; 32-byte record + depth word out
+        "record_count": 48.0 * V,             # record 32 in; count, mask, packed depth value 16 out
+        "depth_sort": (4.0 + 16.0 * 4) * V,   # histogram read + 4 passes over (depth word, value)
+        "offsets": 12.0 * V,                  # packed words read twice, offsets written
+        "emit": 48.0 * V + 8.0 * K,           # record, mask, offset, count in; K 8-byte entries out
+        "tile_sort_passes": 16.0 * K * P,     # each pass reads and writes every entry
+        "ranges": 8.0 * K + 8.0 * T,
+        "raster": 48.0 * K + 4.0 * W * H,     # entry + xy + conic + colour gathers (upper bound), RGBA8 out
+    }
+
+
+KERNEL_BOUND = {"record_count": "valu", "raster": "valu", "tile_sort_passes": "hbm (look-back latency)",
+                "depth_sort": "hbm (look-back latency)", "emit": "hbm (latency)", "offsets": "hbm (scattered stores)"}
+
+
+def kernel_table(frames, N, K, T, P, W, H):
+    names = list(frames[0]["kernel_ms"])
+    V = float(np.mean([f["visible"] for f in frames]))
+    alg = kernel_bytes(N, V, K, T, P, W, H)
+    rows = {}
+    for nm in names:
+        ms = float(np.median([f["kernel_ms"].get(nm, np.nan) for f in frames]))
+        row = {"us": round(ms * 1e3, 1)}
+        if nm in alg and ms > 0:
+            gbs = alg[nm] / (ms * 1e-3) / 1e9
+            row.update(algorithmic_bytes=int(alg[nm]), achieved_gbs=round(gbs, 1),
+                       frac_hbm=round(gbs / HBM_PEAK_GBS, 3), bound=KERNEL_BOUND.get(nm, "hbm"))
+        rows[nm] = row
+    return rows, V
 
 
 def main():
@@ -229,36 +332,64 @@ def main():
     center = scene["centroid"]
     cams = [gr.camera_make(gr.pose_eye(args.seed, i, center, 8.0), center, float(np.deg2rad(50.0)), W, H, 0.01,
                            100.0) for i in range(args.poses)]
+    tx, ty, _ = gr.tile_grid(W, H)
+    T = tx * ty
     r = gr.Renderer(0)
     t0 = time.perf_counter()
     r.upload(scene)
     upload_s = time.perf_counter() - t0
     out = torch.empty((H, W, 4), dtype=torch.uint8, device="cuda")
-    stages = {k: [] for k in ("sh_eval_ms", "project_ms", "build_tile_ms", "sort_ms", "raster_ms", "total_ms")}
-    ks, pass_ms = [], []
+    optr = out.data_ptr()
+    print(f"[bench] scene ready ({gen_s:.1f} s generate, {upload_s:.1f} s upload)", file=sys.stderr, flush=True)
+
+    # headline: stats off, events only around the tile-sort passes (the live roofline)
+    r.set_timing("sort_passes")
     for i in range(args.warmup):
-        r.render(cams[i % len(cams)], W, H, out=out.data_ptr())
+        r.render(cams[i % len(cams)], W, H, out=optr, stats=False)
     torch.cuda.synchronize()
+    ks, pass_ms = [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        _, st = r.render(cams[i % len(cams)], W, H, out=out.data_ptr())
-        for k in stages:
-            stages[k].append(st[k])
-        ks.append(st["tile_list_size"])
+        r.render(cams[i % len(cams)], W, H, out=optr, stats=False)
         kt = r.kernel_times()
+        ks.append(kt["tile_list_size"])
         pass_ms.append(kt["tile_sort_pass_ms"])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     fps = args.steps / elapsed
+    passes = kt["tile_sort_passes"]
+    K = float(np.mean(ks))
+
+    # the same frames with per-stage stats (RenderStats events; reported, not the metric)
+    r.set_timing("none")
+    stages = {k: [] for k in ("sh_eval_ms", "project_ms", "build_tile_ms", "sort_ms", "raster_ms", "total_ms")}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        _, st = r.render(cams[i % len(cams)], W, H, out=optr)
+        for k in stages:
+            stages[k].append(st[k])
+    torch.cuda.synchronize()
+    stats_fps = args.steps / (time.perf_counter() - t0)
+    mean = {k: float(np.mean(v)) for k, v in stages.items()}
+
+    # per-kernel intervals (an event after every kernel; reported, not the metric)
+    r.set_timing("kernels")
+    kframes = []
+    for i in range(args.kernel_frames):
+        r.render(cams[i % len(cams)], W, H, out=optr, stats=False)
+        kframes.append(r.kernel_times())
+    r.set_timing("none")
+    ktab, V = kernel_table(kframes, N, float(np.mean([f["tile_list_size"] for f in kframes])), T, passes, W, H)
+
     # the same frames with the image copied to host memory (PCIe-inclusive; not the metric)
     host_n = min(args.steps, 16)
+    himg = np.empty((H, W, 4), np.uint8)
     t0 = time.perf_counter()
     for i in range(host_n):
-        r.render(cams[i % len(cams)], W, H)
+        r.render(cams[i % len(cams)], W, H, out=himg, stats=False)
     host_fps = host_n / (time.perf_counter() - t0)
-    mean = {k: float(np.mean(v)) for k, v in stages.items()}
-    K = float(np.mean(ks))
-    passes = kt["tile_sort_passes"]
+
     # algorithmic bytes per stage (FP32, degree 3): SH reads position + 48 coefficients and
     # writes colour; projection reads position + scale/rotation and writes xy, z, ellipse, conic;
     # build reads xy/z/ellipse and writes K 8-byte entries; sort = depth words of the
@@ -279,20 +410,17 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.traffic_profile),
             "algorithmic_bytes_per_launch": int(16.0 * K), "avg_launch_us": round(pass_avg_ms * 1e3, 2),
             "launches_per_frame": passes}
+    print(f"[bench] frames: {fps:.1f}/s headline, {stats_fps:.1f}/s with stats", file=sys.stderr, flush=True)
     # the reference's frame sequence on the same kernels (stage-level C-ABI), for comparison
     r.close()
     stage_fps = stage_path_fps(gr, torch, scene, cams, W, H, min(args.steps, 16))
-    r = gr.Renderer(0)
-    r.upload(scene)
-    serving = None
-    if args.serving != "none":
-        r.close()
-        serving, sv_scenes, sv_reqs = serving_leg(gr, args.serving, args.serving_requests, args.serving_rate,
-                                                  args.serving_rate_high)
-        r = gr.Renderer(0)
-        r.upload(scene)
+    serving = {}
+    for wl in [w for w in args.serving.split(",") if w and w != "none"]:
+        print(f"[bench] serving {wl}", file=sys.stderr, flush=True)
+        serving[wl] = serving_leg(gr, wl, args)
     cpu = None
     if args.cpu_baseline:
+        print("[bench] CPU oracle", file=sys.stderr, flush=True)
         from oracle import orc
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         orc.set_threads(threads)
@@ -326,38 +454,15 @@ def main():
                "one_thread": {"value": round(len(one) / sum(one), 4), "unit": "frames/s",
                               "sample": f"{len(one)} frames (poses 0..{len(one) - 1}), {sum(one):.1f} s"},
                "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
-        cam = cams[0]
-        # accuracy gates (the oracle as reference): the benchmarked frame vs the FP64 render,
-        # and served FP16-storage views vs the FP64 render of their FP32 scenes
-        img, _ = r.render(cam, W, H)
-        r64 = orc.render64(scene["position"], scene["scale_rot"], shal, scene["color"], N, 3, cam.packed(), W, H)
-        acc = {"frame_psnr_vs_fp64_db": round(orc.psnr(img, r64["image"]), 2),
-               "frame_max_abs_diff_lsb": int(np.abs(img[..., :3].astype(int) - r64["image"][..., :3].astype(int)).max())}
-        if serving is not None:
-            # one served FP16-storage view per resolution of the workload vs the FP64 render of its
-            # FP32 scene
-            srv = gr.Server(slots=1, max_views=4, use_graphs=True)
-            per_res = {}
-            for (i, c, Wv, Hv) in sv_reqs:
-                key = f"{Wv}x{Hv}"
-                if key in per_res:
-                    continue
-                sc = sv_scenes[i]
-                sid = srv.add_scene(sc, 1)
-                rid = srv.submit(sid, c, Wv, Hv)
-                _, simg = srv.wait_all([rid])[rid]
-                al = orc.realign_sh(sc["sh"], sc["count"], 48)
-                ref = orc.render64(sc["position"], sc["scale_rot"], al, sc["color"], sc["count"], 3, c.packed(), Wv, Hv)
-                dmax = int(np.abs(simg[..., :3].astype(int) - ref["image"][..., :3].astype(int)).max())
-                per_res[key] = {"psnr_db": round(orc.psnr(simg, ref["image"]), 2), "max_abs_diff_lsb": dmax,
-                                "gaussians": int(sc["count"])}
-                srv.remove_scene(sid)
-                if len(per_res) == 4:
-                    break
-            srv.close()
-            acc["served_fp16_vs_fp64_per_resolution"] = per_res
-            acc["served_fp16_min_psnr_vs_fp64_db"] = min(v["psnr_db"] for v in per_res.values())
-        cpu["accuracy"] = acc
+        # accuracy gate (the oracle as reference): the benchmarked frame vs the FP64 render
+        r = gr.Renderer(0)
+        r.upload(scene)
+        img, _ = r.render(cams[0], W, H, stats=False)
+        r.close()
+        r64 = orc.render64(scene["position"], scene["scale_rot"], shal, scene["color"], N, 3, cams[0].packed(), W, H)
+        cpu["accuracy"] = {
+            "frame_psnr_vs_fp64_db": round(orc.psnr(img, r64["image"]), 2),
+            "frame_max_abs_diff_lsb": int(np.abs(img[..., :3].astype(int) - r64["image"][..., :3].astype(int)).max())}
     line = {
         "metric": "1080p frames/s at 6M Gaussians",
         "value": round(fps, 3),
@@ -373,17 +478,19 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{N} Gaussians (64-cluster generator, SH degree 3, FP32), {W}x{H}, "
                                f"{args.poses} seeded poses r=8 fov 50", "gaussians": N, "width": W, "height": H},
+        "with_stats_frames_per_s": round(stats_fps, 2),
         "stage_ms": {k: round(v, 4) for k, v in mean.items()},
         "stage_gbs": {k.replace("_ms", ""): round(v, 1) for k, v in gbs.items()},
+        "kernels": ktab,
         "tile_list_entries": int(K),
+        "visible_gaussians": int(V),
         "host_output_frames_per_s": round(host_fps, 2),
         "stage_path_frames_per_s": round(stage_fps, 2),
         "roofline": roof,
         "cpu_baseline": cpu,
-        "serving": serving,
+        "serving": serving or None,
         "setup_s": {"generate": round(gen_s, 2), "upload": round(upload_s, 2)},
     }
-    r.close()
     print(json.dumps(line), flush=True)
 
 

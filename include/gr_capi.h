@@ -329,16 +329,38 @@ typedef struct gr_frame_view {
 } gr_frame_view;
 GR_API gr_status gr_renderer_frame_view(gr_renderer* r, gr_frame_view* out);
 
+/* Timing of the renderer's frames (no counterpart in the reference, whose
+ * CudaTimer synchronises after every stage, core/utilities.h:173-182).  HIP
+ * events cost a few microseconds of idle GPU each, so a frame records only
+ * what is asked for:
+ *   stats != NULL in gr_renderer_render   the per-stage events of RenderStats
+ *   GR_TIMING_SORT_PASSES                 two events around the tile-sort passes
+ *   GR_TIMING_KERNELS                     an event after every kernel (or small group of
+ *                                         kernels) of the frame, and the visible count
+ * A frame with stats == NULL at GR_TIMING_NONE (the default) records none. */
+#define GR_TIMING_NONE 0
+#define GR_TIMING_SORT_PASSES 1
+#define GR_TIMING_KERNELS 2
+GR_API gr_status gr_renderer_set_timing(gr_renderer* r, int32_t level);
+
 /* Per-kernel timing of the last frame, from HIP events on the renderer's
- * stream (no counterpart in the reference; for roofline reporting).  The
- * tile-bit sort's onesweep passes run back to back between two events:
- * tile_sort_pass_ms is their total, tile_sort_passes their number and
- * tile_list_size the pairs each pass moves. */
+ * stream (for roofline reporting).  The tile-bit sort's onesweep passes run
+ * back to back between two events: tile_sort_pass_ms is their total (-1 when
+ * the frame did not time them), tile_sort_passes their number and
+ * tile_list_size the pairs each pass moves.  At GR_TIMING_KERNELS,
+ * kernel_ms[i] is the time of the work named kernel_name[i] (in frame order:
+ * "sh", "project", "compact", "record_count", "depth_sort", "offsets",
+ * "emit", "tile_sort_scan", "tile_sort_passes", "ranges", "raster") and
+ * visible the number of Gaussians with a non-empty tile rect (else -1). */
+#define GR_KERNEL_TIMES_MAX 24
 typedef struct gr_kernel_times {
     double tile_sort_pass_ms;
     int32_t tile_sort_passes;
-    int32_t pad;
+    int32_t kernels;
     int64_t tile_list_size;
+    int64_t visible;
+    double kernel_ms[GR_KERNEL_TIMES_MAX];
+    char kernel_name[GR_KERNEL_TIMES_MAX][24];
 } gr_kernel_times;
 GR_API gr_status gr_renderer_kernel_times(gr_renderer* r, gr_kernel_times* out);
 
@@ -417,5 +439,10 @@ GR_API gr_status gr_server_poll(gr_server* server, gr_completion* out, int32_t m
                                 int32_t wait_ms);
 GR_API gr_status gr_server_get_stats(gr_server* server, gr_server_stats* out);
 GR_API double gr_server_clock_ms(gr_server* server);
+/* Message of the last request that completed with an error status (empty if
+ * none).  Requests fail on the server's scheduler thread, so their messages
+ * never reach the submitting thread's gr_last_error(); this copies the latest
+ * one (truncated to buf_bytes - 1 characters, NUL-terminated) into buf. */
+GR_API gr_status gr_server_last_error(gr_server* server, char* buf, int32_t buf_bytes);
 
 #endif /* GR_CAPI_H_ */

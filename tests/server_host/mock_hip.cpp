@@ -65,6 +65,10 @@ void set_last_error(const std::string& msg) {
     std::lock_guard<std::mutex> lk(g_err_mu);
     g_last_error = msg;
 }
+std::string get_last_error() {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_last_error;
+}
 }  // namespace gr_detail
 
 extern "C" {
@@ -75,7 +79,23 @@ hipError_t hipGetDevice(int* d) {
     return hipSuccess;
 }
 const char* hipGetErrorString(hipError_t) { return "mock error"; }
+// Allocation failure injection: after mock_fail_malloc_after(k), the next k device
+// allocations succeed and the ones after them fail with hipErrorOutOfMemory (k < 0: none fail).
+static std::mutex g_fail_mu;
+static int g_malloc_budget = -1;
+void mock_fail_malloc_after(int k) {
+    std::lock_guard<std::mutex> lk(g_fail_mu);
+    g_malloc_budget = k;
+}
 hipError_t hipMalloc(void** p, size_t n) {
+    {
+        std::lock_guard<std::mutex> lk(g_fail_mu);
+        if (g_malloc_budget == 0) {
+            *p = nullptr;
+            return hipErrorOutOfMemory;
+        }
+        if (g_malloc_budget > 0) --g_malloc_budget;
+    }
     *p = std::malloc(n);
     std::memset(*p, 0xcd, n);
     return hipSuccess;

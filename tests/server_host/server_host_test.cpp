@@ -3,6 +3,8 @@
 //
 //   server_host_test mix <graphs> <budget_bytes>   alone + fused rounds over two scenes
 //   server_host_test add_budget                    a scene larger than the budget is refused
+//   server_host_test add_exact                     a scene within 1.1x of the budget is admitted exactly
+//   server_host_test add_fail                      an allocation failing inside add_scene leaks nothing
 //   server_host_test threads <graphs> [budget]     concurrent submitters, a scene removed mid-run
 //
 // Each request's camera position.x carries a tag in [1, 250]; the mocked
@@ -20,6 +22,8 @@
 #include <vector>
 
 #include "gr_capi.h"
+
+extern "C" void mock_fail_malloc_after(int k);   // mock_hip.cpp
 
 namespace {
 
@@ -163,6 +167,52 @@ int run_add_budget() {
     return 0;
 }
 
+// A scene whose exact size fits the budget with less than the workspaces' 25 % headroom to
+// spare is admitted and accounted at its exact size.
+int run_add_exact() {
+    Model m(80000, 3, 0);
+    gr_server_config cfg{-1, 2, 4, 0, 0};
+    gr_server* s = nullptr;
+    if (gr_server_create(&s, &cfg) != GR_OK) return 2;
+    int32_t id = -1;
+    if (gr_server_add_scene(s, &m.host, &id) != GR_OK) return 3;
+    gr_server_stats st{};
+    gr_server_get_stats(s, &st);
+    const long long exact = st.resident_bytes;
+    gr_server_destroy(s);
+    const long long budget = exact + exact / 10;   // 1.1x the scene
+    gr_server_config c2{-1, 2, 4, 0, budget};
+    if (gr_server_create(&s, &c2) != GR_OK) return 2;
+    const gr_status add = gr_server_add_scene(s, &m.host, &id);
+    gr_server_get_stats(s, &st);
+    printf("RESULT add_status=%d exact=%lld resident=%lld peak=%lld budget=%lld\n", (int)add, exact,
+           (long long)st.resident_bytes, (long long)st.peak_bytes, budget);
+    gr_server_destroy(s);
+    return 0;
+}
+
+// An allocation that fails partway through gr_server_add_scene frees the buffers allocated
+// before it: nothing stays counted as resident (and LeakSanitizer sees no leak at exit).
+int run_add_fail() {
+    Model m(80000, 3, 0);
+    gr_server_config cfg{-1, 2, 4, 0, 0};
+    gr_server* s = nullptr;
+    if (gr_server_create(&s, &cfg) != GR_OK) return 2;
+    int32_t id = -1;
+    mock_fail_malloc_after(2);   // position and scale/rotation succeed, the SH buffer fails
+    const gr_status add = gr_server_add_scene(s, &m.host, &id);
+    mock_fail_malloc_after(-1);
+    gr_server_stats st{};
+    gr_server_get_stats(s, &st);
+    const long long after_fail = st.resident_bytes;
+    const gr_status again = gr_server_add_scene(s, &m.host, &id);   // the server still admits it
+    gr_server_get_stats(s, &st);
+    printf("RESULT add_status=%d resident_after_fail=%lld retry_status=%d resident=%lld\n", (int)add,
+           after_fail, (int)again, (long long)st.resident_bytes);
+    gr_server_destroy(s);
+    return 0;
+}
+
 int run_threads(int graphs, long long budget) {
     gr_server_config cfg{-1, 3, 16, graphs, budget};
     gr_server* s = nullptr;
@@ -210,6 +260,8 @@ int main(int argc, char** argv) {
     const std::string mode = argv[1];
     if (mode == "mix") return run_mix(argc > 2 ? atoi(argv[2]) : 0, argc > 3 ? atoll(argv[3]) : 0);
     if (mode == "add_budget") return run_add_budget();
+    if (mode == "add_exact") return run_add_exact();
+    if (mode == "add_fail") return run_add_fail();
     if (mode == "threads") return run_threads(argc > 2 ? atoi(argv[2]) : 0, argc > 3 ? atoll(argv[3]) : 0);
     return 1;
 }

@@ -35,9 +35,9 @@ def c3(orc):
     return scene, shal
 
 
-def camera(scene, W, H):
+def camera(scene, W, H, pose=0):
     c = scene["centroid"]
-    return gr.camera_make(gr.pose_eye(3, 0, c, 8.0), c, float(np.deg2rad(50.0)), W, H, 0.01, 100.0)
+    return gr.camera_make(gr.pose_eye(3, pose, c, 8.0), c, float(np.deg2rad(50.0)), W, H, 0.01, 100.0)
 
 
 def fp64(orc, scene, shal, size):
@@ -111,3 +111,31 @@ def test_c3_precision(torch_cuda, orc, c3, size, precision):
         f"decoded inputs, vs FP64: max|d| {dmax} LSB, PSNR {p:.2f} dB, pairs only {name} {only_p}, "
         f"only FP64 {only64}, order differences {order}")
     assert p >= 50.0
+
+
+def test_c3_fp32_all_poses_1080p(torch_cuda, orc, c3):
+    """The other seven seeded poses of the 1080p workload (pose 0 is test_c3_fp32): each frame's
+    tile list and image bit-exact against the FP32 oracle, and >= 50 dB against the FP64 render."""
+    scene, shal = c3
+    W, H = SIZES["1080p"]
+    r = gr.Renderer()
+    r.upload(scene)
+    try:
+        for pose in range(1, 8):
+            say(f"c3 fp32 1080p pose {pose}")
+            cam = camera(scene, W, H, pose)
+            img, st = r.render(cam, W, H)
+            keys, vals, _ = frame_list(r)
+            ref = orc.render(scene["position"], scene["scale_rot"], shal, None, N, 3, cam.packed(), W, H)
+            assert st["tile_list_size"] == ref["k"], f"pose {pose}"
+            assert np.array_equal(keys, ref["keys"]) and np.array_equal(vals, ref["values"]), f"pose {pose}"
+            assert np.array_equal(img, ref["image"]), f"pose {pose}"
+            r64 = orc.render64(scene["position"], scene["scale_rot"], shal, scene["color"], N, 3, cam.packed(), W, H)
+            p = orc.psnr(img, r64["image"])
+            dmax = int(np.abs(img[..., :3].astype(int) - r64["image"][..., :3].astype(int)).max())
+            say(f"[c3] fp32 1080p pose {pose}: K {ref['k']}, tile list and image == FP32 oracle, vs FP64: "
+                f"max|d| {dmax} LSB, PSNR {p:.2f} dB")
+            assert p >= 50.0, f"pose {pose}"
+    finally:
+        r.close()
+

@@ -397,3 +397,71 @@ def test_sort_scratch_reused_across_sizes(torch_cuda):
         temp.free()
         ctx.close()
     assert grown == sorted(grown), f"the scratch shrank: {grown}"
+
+
+def test_raster_rejects_missing_buffers(torch_cuda):
+    """gr_rasterize_tile refuses a tile list without values (and a missing per-Gaussian input)
+    with GR_ERR_INVALID_ARGUMENT before anything is launched; the context keeps working."""
+    say("raster rejects NULL buffers")
+    scene = gr.generate_scene(1, 2000, 3, 5)
+    cam = gr.camera_make([0.0, 0.0, 3.0], [0.0, 0.0, 0.0], float(np.deg2rad(60.0)), 320, 240, 0.01, 100.0)
+    p = StagePipeline(torch_cuda, scene, cam, 320, 240)
+    try:
+        p.run_sh()
+        p.run_project()
+        p.run_build()
+        p.run_sort()
+        p.run_ranges()
+        p.ctx.set_tile_list_args(gr.TileListArgs(p.keys[p.sel].data_ptr(), None, p.k, max(p.k, 1)))
+        with pytest.raises(gr.GrError) as e:
+            p.ctx.raster()
+        assert e.value.status == 1
+        ga = gr.GlobalArgs.from_buffer_copy(p.ga)
+        ga.conic = None
+        p.ctx.set_global_args(ga)
+        p.ctx.set_tile_list_args(gr.TileListArgs(p.keys[p.sel].data_ptr(), p.vals[p.sel].data_ptr(), p.k,
+                                                 max(p.k, 1)))
+        with pytest.raises(gr.GrError) as e:
+            p.ctx.raster()
+        assert e.value.status == 1
+        p.ctx.set_global_args(p.ga)
+        img = p.run_raster()
+        assert img[..., 3].max() == 255
+    finally:
+        p.close()
+
+
+
+def test_timing_levels_render_the_same_frame(torch_cuda):
+    """A frame without stats or events, one timing the tile-sort passes, one with per-kernel
+    marks and one with per-stage stats render the same image and tile list; the timed ones
+    report their intervals."""
+    say("timing levels")
+    s = gr.generate_scene(2, 200_000, 3, 71)
+    c = s["centroid"]
+    cam = gr.camera_make(gr.pose_eye(71, 0, c, 8.0), c, FOV50, 1280, 720, 0.01, 100.0)
+    r = gr.Renderer()
+    r.upload(s)
+    base, st = r.render(cam, 1280, 720)
+    keys0, vals0, _ = frame_list(r)
+    assert st["total_ms"] > 0 and st["tile_list_size"] == len(keys0)
+    for level in ("none", "sort_passes", "kernels"):
+        r.set_timing(level)
+        img, st = r.render(cam, 1280, 720, stats=False)
+        assert st is None
+        assert np.array_equal(img, base), f"timing level {level} changed the image"
+        kt = r.kernel_times()
+        keys, vals, _ = frame_list(r)
+        assert np.array_equal(keys, keys0) and np.array_equal(vals, vals0)
+        if level == "none":
+            assert kt["tile_sort_pass_ms"] == -1.0 and not kt["kernel_ms"]
+        else:
+            assert kt["tile_sort_pass_ms"] > 0
+        if level == "kernels":
+            names = list(kt["kernel_ms"])
+            for want in ("sh", "project", "record_count", "emit", "tile_sort_passes", "raster"):
+                assert want in names, names
+            assert all(v >= 0 for v in kt["kernel_ms"].values())
+            assert 0 < kt["visible"] <= s["count"]
+            say(f"kernel ms: {kt['kernel_ms']}")
+    r.close()

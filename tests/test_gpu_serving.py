@@ -109,11 +109,15 @@ def test_budget_admission(torch_cuda):
     s, views = scene_and_views(2, 200_000, 41, 4, [(1920, 1080)])
     scene_bytes = 200_000 * (16 + 16 + 192)
     # too small for any batch: completes with GR_ERR_BUDGET, server keeps working
-    srv = gr.Server(slots=2, max_views=4, use_graphs=False, budget_bytes=scene_bytes + (1 << 20))
+    tight = scene_bytes + (1 << 20)
+    srv = gr.Server(slots=2, max_views=4, use_graphs=False, budget_bytes=tight)
     sid = srv.add_scene(s)
     rid = srv.submit(sid, *views[0])
     d, _ = srv.wait_all([rid])[rid]
     assert d["status"] == 5
+    assert "budget" in srv.last_error()
+    st = srv.stats()
+    assert st["resident_bytes"] <= tight and st["peak_bytes"] <= tight, st
     srv.close()
     # room for about one batch at a time: everything completes, matching the renderer
     ref = renderer_images(s, views)

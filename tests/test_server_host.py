@@ -7,7 +7,9 @@ whose graphs replay captured closures.  The scenarios check:
 
 * every completed request receives its own image (routing through fused
   batches, phase-B batch splits and graph replays);
-* the HBM budget: a scene larger than the budget is refused (GR_ERR_BUDGET),
+* the HBM budget: a scene larger than the budget is refused (GR_ERR_BUDGET), a
+  scene is admitted and accounted at its exact size, an allocation failure
+  inside gr_server_add_scene frees what it had allocated,
   a tight budget still completes every request with peak <= budget, a budget
   too small for one view fails those requests with GR_ERR_BUDGET and nothing
   else;
@@ -68,6 +70,20 @@ def test_mix_routes_images(harness, graphs):
 
 def test_scene_over_budget_refused(harness):
     assert run(harness, "add_budget")["add_status"] == 5
+
+
+def test_scene_admitted_at_exact_size(harness):
+    """Budget 1.1x the scene: admitted, and resident/peak bytes are the scene's exact size."""
+    r = run(harness, "add_exact")
+    assert r["add_status"] == 0
+    assert r["resident"] == r["exact"] and r["peak"] <= r["budget"]
+
+
+def test_failed_scene_upload_frees_its_buffers(harness):
+    r = run(harness, "add_fail")
+    assert r["add_status"] == 3            # GR_ERR_OUT_OF_MEMORY
+    assert r["resident_after_fail"] == 0
+    assert r["retry_status"] == 0 and r["resident"] > 0
 
 
 @pytest.mark.parametrize("graphs", [0, 1], ids=["eager", "graphs"])
