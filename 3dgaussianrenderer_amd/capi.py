@@ -142,6 +142,8 @@ SIGNATURES = [
     ("gr_build_tile_list", C.c_int, [_P, _P, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
     ("gr_sort_tile_list", C.c_int, [_P, _P, C.c_int32, _PP, C.POINTER(C.c_size_t), C.POINTER(DoubleBufferU64),
                                     C.POINTER(DoubleBufferI32)]),
+    ("gr_sort_tile_list_static", C.c_int, [_P, _P, C.c_int32, C.POINTER(DoubleBufferU64), C.POINTER(DoubleBufferI32),
+                                           C.POINTER(C.c_int32)]),
     ("gr_device_free", C.c_int, [_P]),
     ("gr_radix_sort_pairs", C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _PP, C.POINTER(C.c_size_t),
                                       C.POINTER(DoubleBufferU64), C.POINTER(DoubleBufferI32)]),
@@ -370,6 +372,13 @@ class Context:
     def sort(self, size: int, keys: DoubleBufferU64, values: DoubleBufferI32, temp: "SortTemp", timer=None):
         check(lib().gr_sort_tile_list(self.h, timer, size, C.byref(temp.ptr), C.byref(temp.nbytes), C.byref(keys),
                                       C.byref(values)))
+
+    def sort_static(self, size: int, keys: DoubleBufferU64, values: DoubleBufferI32, timer=None) -> int:
+        """sortTileList on the library's static scratch (allocates nothing); returns the most
+        pairs that scratch holds."""
+        mx = C.c_int32()
+        check(lib().gr_sort_tile_list_static(self.h, timer, size, C.byref(keys), C.byref(values), C.byref(mx)))
+        return mx.value
 
     def radix_sort(self, count: int, begin_bit: int, end_bit: int, keys: DoubleBufferU64, values: DoubleBufferI32,
                    temp: "SortTemp"):

@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -332,6 +333,81 @@ gr_status gr_sort_tile_list(gr_context* ctx, gr_timer* timer, int32_t size, void
     GR_CHECK(timer_start(ctx, timer));
     GR_CHECK(gr_radix_sort_pairs(ctx, size, 0, end_bit, temp, temp_bytes, keys, values));
     GR_CHECK(timer_stop(ctx, timer));
+    return GR_OK;
+}
+
+}  // extern "C"
+
+// ---- sortTileList without allocation ------------------------------------
+// The scratch of gr_sort_tile_list_static: digit histograms, partition tickets and the two
+// look-back status regions of the onesweep sort, in one file-scope device array (every GPU the
+// library runs on holds its own copy until the process ends).  Host code never passes the
+// array's name to a kernel -- on the host it is only a stand-in -- but the device address that
+// hipGetSymbolAddress returns for the current device, looked up once per device.
+constexpr size_t kStaticSortScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticSortScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first call per device).
+gr_status static_scratch(void** out) {
+    int dev = 0;
+    GR_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(GR_ERR_INVALID_ARGUMENT, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        GR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(GR_ERR_HIP, "hipGetSymbolAddress returned NULL for the sort scratch");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GR_OK;
+}
+
+// Largest pair count whose sort scratch fits the static array (the layout grows with the count).
+int32_t static_sort_max_pairs() {
+    int64_t lo = 1, hi = (int64_t)1 << 30;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) / 2;
+        if (radix_sort_temp_bytes((int32_t)std::min<int64_t>(mid, (1 << 30) - 1), 0, 64) <= kStaticSortScratchBytes)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return (int32_t)std::min<int64_t>(lo, (1 << 30) - 1);
+}
+}  // namespace
+
+extern "C" {
+
+gr_status gr_sort_tile_list_static(gr_context* ctx, gr_timer* timer, int32_t size, gr_double_buffer_u64* keys,
+                                   gr_double_buffer_i32* values, int32_t* max_pairs) {
+    static const int32_t kMax = static_sort_max_pairs();
+    if (max_pairs) *max_pairs = kMax;
+    if (!ctx || !ctx->has_args) return fail(GR_ERR_INVALID_ARGUMENT, "global args not set");
+    if (!keys || !values) return fail(GR_ERR_INVALID_ARGUMENT, "gr_sort_tile_list_static: NULL argument");
+    if (size < 0) return fail(GR_ERR_INVALID_ARGUMENT, "negative tile-list size %d", size);
+    if (size > kMax)
+        return fail(GR_ERR_CAPACITY,
+                    "tile list of %d pairs exceeds the %d pairs the static sort scratch (%zu bytes) holds",
+                    size, kMax, kStaticSortScratchBytes);
+    if (values->selector != keys->selector) return fail(GR_ERR_INVALID_ARGUMENT, "key/value selectors differ");
+    void* temp = nullptr;
+    GR_CHECK(static_scratch(&temp));
+    const FrameGeom g = ctx_geom(ctx);
+    const int end_bit = 32 + tile_bits_for(g.tiles_x * g.tiles_y);
+    uint64_t* kb[2] = {keys->buffers[0], keys->buffers[1]};
+    int32_t* vb[2] = {values->buffers[0], values->buffers[1]};
+    int32_t sel = keys->selector;
+    GR_CHECK(timer_start(ctx, timer));
+    GR_HIP(radix_sort_pairs(kb, vb, &sel, size, nullptr, 0, end_bit, temp, kStaticSortScratchBytes, ctx->stream));
+    GR_CHECK(timer_stop(ctx, timer));
+    keys->selector = sel;
+    values->selector = sel;
     return GR_OK;
 }
 

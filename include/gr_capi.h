@@ -197,6 +197,20 @@ GR_API gr_status gr_build_tile_list(gr_context* ctx, gr_timer* timer, int32_t nu
 GR_API gr_status gr_sort_tile_list(gr_context* ctx, gr_timer* timer, int32_t tile_list_size,
                                    void** device_temp_storage, size_t* temp_storage_bytes,
                                    gr_double_buffer_u64* keys, gr_double_buffer_i32* values);
+/* No counterpart in the reference: sortTileList for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same sort as gr_sort_tile_list, with
+ * its scratch in a fixed 16 MiB device array of the library (one per GPU,
+ * held until the process ends) instead of caller temp storage: no hipMalloc,
+ * hipFree or other allocation, and no memset (its counters are cleared by a
+ * fill kernel).  At most *max_pairs pairs (about 54M; written on every call,
+ * also on failure, when max_pairs is not NULL); a larger list is refused with
+ * GR_ERR_CAPACITY before anything is launched.  Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU (on different streams)
+ * race; calls one after another on a stream do not. */
+GR_API gr_status gr_sort_tile_list_static(gr_context* ctx, gr_timer* timer, int32_t tile_list_size,
+                                          gr_double_buffer_u64* keys, gr_double_buffer_i32* values,
+                                          int32_t* max_pairs);
 /* Frees device memory that the library allocated for the caller (the sort
  * temp storage; the reference frees it with cudaFree, app/main.cpp:474-477). */
 GR_API gr_status gr_device_free(void* device_ptr);
