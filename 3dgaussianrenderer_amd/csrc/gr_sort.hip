@@ -96,7 +96,7 @@ static SortLayout sort_layout(int32_t count, int tile) {
     size_t off = 0;
     l.hist_off = off;
     off += sizeof(uint32_t) * kHistReplicas * kMaxPasses * kRadix;
-    l.ticket_off = off;
+    l.ticket_off = off;   // kMaxPasses tickets, then kMaxPasses trivial-pass flags at +128
     off += 256;
     off = (off + 255) & ~(size_t)255;
     const size_t region = (size_t)max_parts * kRadix * sizeof(uint32_t);
@@ -192,8 +192,10 @@ __global__ __launch_bounds__(kHistThreads) void sort_hist_kernel(const KeyT* __r
     }
 }
 
-// exclusive scan of each pass's 256 bins; one workgroup of 256 lanes per pass
-__global__ __launch_bounds__(256) void sort_hist_scan_kernel(uint32_t* hist) {
+// exclusive scan of each pass's 256 bins; one workgroup of 256 lanes per pass.  A pass whose
+// keys all share one digit is flagged trivial (trivial[pass] = 1): its stable permutation is the
+// identity, so the pass copies its partitions through unchanged.
+__global__ __launch_bounds__(256) void sort_hist_scan_kernel(uint32_t* hist, uint32_t* trivial) {
     __shared__ uint32_t s_w[4];
     uint32_t* h = hist + blockIdx.x * kRadix;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -210,6 +212,9 @@ __global__ __launch_bounds__(256) void sort_hist_scan_kernel(uint32_t* hist) {
     uint32_t off = 0;
     for (int k = 0; k < w; ++k) off += s_w[k];
     h[t] = off + incl - v;
+    const uint32_t total = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    const int one_digit = __syncthreads_or(total > 0u && v == total);
+    if (t == 0) trivial[blockIdx.x] = one_digit ? 1u : 0u;
 }
 
 __device__ __forceinline__ uint32_t ld_status(const uint32_t* p) {
@@ -265,7 +270,8 @@ template <typename KeyT, bool PAIRS, class S>
 __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::kWavesPerEU))) void onesweep_kernel(
     const KeyT* __restrict__ kin, const int32_t* __restrict__ vin, KeyT* __restrict__ kout,
     int32_t* __restrict__ vout, int32_t n, const uint32_t* count_dev, int strict, int shift, int bits,
-    const uint32_t* __restrict__ digit_off, uint32_t* status, uint32_t* status_next, uint32_t* ticket) {
+    const uint32_t* __restrict__ digit_off, uint32_t* status, uint32_t* status_next, uint32_t* ticket,
+    const uint32_t* trivial) {
     constexpr int kSortThreads = S::kThreads;
     constexpr int kSortWaves = S::kWaves;
     constexpr int kSortItems = S::kItems;
@@ -294,6 +300,15 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::
     if (status_next && t < kRadix) status_next[(size_t)part * kRadix + t] = 0u;
 
     const int64_t base = (int64_t)part * kSortTile;
+    if (*trivial) {
+        // every key has the same digit: the pass is the identity, the partition is copied through
+        const int64_t end = min((int64_t)n, base + kSortTile);
+        for (int64_t i = base + t; i < end; i += kSortThreads) {
+            kout[i] = kin[i];
+            if constexpr (PAIRS) vout[i] = vin[i];
+        }
+        return;
+    }
     const int64_t wbase = base + (int64_t)w * (64 * kSortItems) + lane;
     const uint32_t dmask = (1u << bits) - 1u;
     KeyT key[kSortItems];
@@ -417,9 +432,10 @@ static hipError_t sort_passes_t(KeyT* keys[2], int32_t* values[2], int32_t* sel,
     const int passes = sort_passes(begin_bit, end_bit);
     uint32_t* hist = reinterpret_cast<uint32_t*>(tb + l.hist_off);
     uint32_t* tickets = reinterpret_cast<uint32_t*>(tb + l.ticket_off);
+    uint32_t* trivial = reinterpret_cast<uint32_t*>(tb + l.ticket_off + 128);
     uint32_t* status[2] = {reinterpret_cast<uint32_t*>(tb + l.status_off[0]),
                            reinterpret_cast<uint32_t*>(tb + l.status_off[1])};
-    hipLaunchKernelGGL(sort_hist_scan_kernel, dim3(passes), dim3(256), 0, s, hist);
+    hipLaunchKernelGGL(sort_hist_scan_kernel, dim3(passes), dim3(256), 0, s, hist, trivial);
     if (ev_begin) {
         const hipError_t e = hipEventRecord(ev_begin, s);
         if (e != hipSuccess) return e;
@@ -432,12 +448,12 @@ static hipError_t sort_passes_t(KeyT* keys[2], int32_t* values[2], int32_t* sel,
             hipLaunchKernelGGL((onesweep_kernel<KeyT, true, S>), dim3(l.partitions), dim3(S::kThreads), 0, s,
                                keys[cur], values[cur], keys[cur ^ 1], values[cur ^ 1], count, count_dev, strict,
                                shift, bits, hist + p * kRadix, status[p & 1],
-                               (p + 1 < passes) ? status[(p + 1) & 1] : nullptr, tickets + p);
+                               (p + 1 < passes) ? status[(p + 1) & 1] : nullptr, tickets + p, trivial + p);
         else
             hipLaunchKernelGGL((onesweep_kernel<KeyT, false, S>), dim3(l.partitions), dim3(S::kThreads), 0, s,
                                keys[cur], nullptr, keys[cur ^ 1], nullptr, count, count_dev, strict, shift, bits,
                                hist + p * kRadix, status[p & 1], (p + 1 < passes) ? status[(p + 1) & 1] : nullptr,
-                               tickets + p);
+                               tickets + p, trivial + p);
         *sel = cur ^ 1;
     }
     if (ev_end) {

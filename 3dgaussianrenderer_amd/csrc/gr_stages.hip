@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gr_kernels.hpp"
+#include "gr_overlap.hpp"
 
 namespace gr {
 
@@ -152,47 +153,6 @@ hipError_t launch_sh(const ShArgs& a, int degree, int precision, hipStream_t s) 
     }
 }
 
-// Footprint: getAABBRect core/gaussian_render.cu:274-298 and the tile rect of
-// :379-389 (used by the projection's visibility ballot and by the build).
-struct Footprint {
-    int x0, y0, nx, ny;
-};
-
-__device__ __forceinline__ Footprint footprint(float cx, float cy, float csx, float csy, float ex,
-                                               float ey, const FrameGeom& g) {
-    const float rx = csx * ex, ry = csy * ex;        // principal axis
-    const float ux = csy * ey, uy = (-csx) * ey;     // minor axis
-    float mnx = 1e12f, mny = 1e12f, mxx = -1e12f, mxy = -1e12f;
-#pragma unroll
-    for (int j = 0; j != 4; ++j) {
-        const int bottom = j & 1, top = j >> 1;
-        const float r = (float)(bottom ^ top) * 2.0f - 1.0f;
-        const float u = (float)top * 2.0f - 1.0f;
-        const float vx = rx * r + ux * u;
-        const float vy = ry * r + uy * u;
-        mnx = gmin(mnx, vx);
-        mny = gmin(mny, vy);
-        mxx = gmax(mxx, vx);
-        mxy = gmax(mxy, vy);
-    }
-    mnx = mnx + cx;
-    mny = mny + cy;
-    mxx = mxx + cx;
-    mxy = mxy + cy;
-    const float tfx = ((mnx + 1.0f) * 0.5f) * g.tiles_per_axis_x;
-    const float tfy = ((mny + 1.0f) * 0.5f) * g.tiles_per_axis_y;
-    const float tfz = ((mxx + 1.0f) * 0.5f) * g.tiles_per_axis_x;
-    const float tfw = ((mxy + 1.0f) * 0.5f) * g.tiles_per_axis_y;
-    Footprint f;
-    f.x0 = iclamp(f2i_sat(floorf(tfx)), 0, g.tiles_x);
-    f.y0 = iclamp(f2i_sat(floorf(tfy)), 0, g.tiles_y);
-    const int x1 = iclamp(f2i_sat(ceilf(tfz)), 0, g.tiles_x);
-    const int y1 = iclamp(f2i_sat(ceilf(tfw)), 0, g.tiles_y);
-    f.nx = x1 - f.x0;
-    f.ny = y1 - f.y0;
-    return f;
-}
-
 // ---------------------------------------------------------------------------
 // EWA projection: core/gaussian_render.cu:135-215 (quaternion decode :45-48).
 // The eigenvector (cos, sin) is written directly instead of going through
@@ -325,69 +285,6 @@ hipError_t launch_project(const ProjectArgs& a, hipStream_t s) {
 // count and candidate mask and its emit pass writes each record's entries at
 // an offset assigned in depth order.
 // ---------------------------------------------------------------------------
-// IEEE quotient a / b from rb = RN(1/b): q0 = RN(a*rb), e = a - b*q0 (exact
-// through the FMA), q = RN(q0 + e*rb) is the correctly rounded quotient when
-// nothing underflows or overflows (Markstein 1990, "Computation of elementary
-// functions on the IBM RISC System/6000 processor", Theorem 1).  The caller
-// guarantees 2^-40 <= |b| <= 2^40 (else fast == false); tiny or zero
-// numerators take the exact paths below.  Same bits as a / b, a third of the
-// instructions of the full division sequence.
-__device__ __forceinline__ float div_by(float a, float b, float rb, bool fast) {
-    if (fast && fabsf(a) >= 0x1p-40f) {
-        const float q0 = a * rb;
-        const float e = fmaf(-b, q0, a);
-        return fmaf(e, rb, q0);
-    }
-    if (fast && a == 0.0f) return a * rb;
-    return a / b;
-}
-
-struct EllipseRec {
-    float cx, cy, csx, csy;     // centre, (cos, sin)
-    float ex, ey, rex, rey;     // extents and their correctly rounded reciprocals
-};
-
-__device__ __forceinline__ bool div_fast_ok(float b) {
-    const float ab = fabsf(b);
-    return ab >= 0x1p-40f && ab <= 0x1p40f;
-}
-
-// convertToEllipseCoordinates (core/gaussian_render.cu:234-239)
-__device__ __forceinline__ v2 to_ellipse(float px, float py, const EllipseRec& e, bool fx, bool fy) {
-    const float dx = px - e.cx, dy = py - e.cy;
-    return v2{div_by(dx * e.csx + dy * e.csy, e.ex, e.rex, fx), div_by(dy * e.csx - dx * e.csy, e.ey, e.rey, fy)};
-}
-
-// intersectsUnitCircle (core/gaussian_render.cu:241-248)
-__device__ __forceinline__ bool hits_unit_circle(v2 v0, v2 v1) {
-    const float ddx = v1.x - v0.x, ddy = v1.y - v0.y;
-    const float lsq = ddx * ddx + ddy * ddy;
-    const float t = saturate(((-v0.x) * ddx + (-v0.y) * ddy) / lsq);
-    const float px = v0.x + t * (v1.x - v0.x);
-    const float py = v0.y + t * (v1.y - v0.y);
-    return (px * px + py * py) < 1.0f;
-}
-
-// ellipseRectOverlap (core/gaussian_render.cu:250-272) for tile (gx, gy); the
-// reference ORs all six tests, here the first true one returns.
-__device__ __forceinline__ bool tile_overlap(int gx, int gy, const EllipseRec& e, const FrameGeom& g) {
-    const float mnx = (float)gx * g.tile_clip_x - 1.0f;
-    const float mny = (float)gy * g.tile_clip_y - 1.0f;
-    const float mxx = mnx + g.tile_clip_x;
-    const float mxy = mny + g.tile_clip_y;
-    if (e.cx > mnx && e.cx < mxx && e.cy > mny && e.cy < mxy) return true;
-    const bool fx = div_fast_ok(e.ex), fy = div_fast_ok(e.ey);
-    const float ctx = mnx + (mxx - mnx) * 0.5f;
-    const float cty = mny + (mxy - mny) * 0.5f;
-    if (len2sq(to_ellipse(ctx, cty, e, fx, fy)) < 1.0f) return true;
-    const v2 p0 = to_ellipse(mnx, mny, e, fx, fy);
-    const v2 p1 = to_ellipse(mxx, mny, e, fx, fy);
-    const v2 p2 = to_ellipse(mxx, mxy, e, fx, fy);
-    const v2 p3 = to_ellipse(mnx, mxy, e, fx, fy);
-    return hits_unit_circle(p0, p1) || hits_unit_circle(p1, p2) || hits_unit_circle(p2, p3) ||
-           hits_unit_circle(p3, p0);
-}
-
 __device__ __forceinline__ uint32_t depth_word(float z) {
     return f2u_sat(gclamp((z + 1.0f) * 0.5f, 0.0f, 1.0f) * 4294967296.0f);
 }
@@ -416,14 +313,22 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
     return v;
 }
 
-// Record index o (0..63) of candidate/entry c: the last o with excl[o] <= c
-// (excl non-decreasing, excl[0] = 0).
-__device__ __forceinline__ int owner_of(const int* excl, int c) {
-    int o = 0;
-#pragma unroll
-    for (int step = 32; step >= 1; step >>= 1)
-        if (excl[o + step] <= c) o += step;
-    return o;
+// Owner (wave-local record 0..63) of item base + lane of a step, when record `lane` holds items
+// [excl, excl + cnt) (excl non-decreasing over the records, base a multiple of 64): every record
+// with items in the step marks the position of its first one with the step's tag, a ballot
+// collects the marks and each lane takes the nearest mark at or below it -- one LDS write, one
+// read and a cross-lane read instead of a binary search of six dependent LDS reads.  s_mark is
+// the wave's 64 slots, set to ~0 (no valid tag) before the first step.  Lanes past the step's
+// last item get an arbitrary owner.
+__device__ __forceinline__ int step_owner(uint32_t* s_mark, int base, int excl, int cnt, int lane) {
+    const uint32_t tag = (uint32_t)base;   // low 6 bits free: base is a multiple of 64
+    if (cnt > 0 && excl < base + 64 && excl + cnt > base) s_mark[max(excl, base) - base] = tag | (uint32_t)lane;
+    wave_sync();
+    const uint32_t v = s_mark[lane];
+    const uint64_t starts = __ballot((v & ~63u) == tag);
+    const uint64_t le = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const int slot = le ? 63 - __clzll(le) : 0;
+    return __shfl((int)(v & 63u), slot, 64);
 }
 
 // Position of the k-th set bit of m (k < popcount(m)), by halving.
@@ -460,6 +365,8 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
     __shared__ int s_excl[W][64];
     __shared__ int4 s_rect[W][64];          // x0, y0, nx, depth word
     __shared__ EllipseRec s_ell[W][64];
+    __shared__ FastEllipse s_fast[W][64];
+    __shared__ uint32_t s_mark[W][64];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int batch = blockIdx.x * W + w;
@@ -479,12 +386,16 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
         cnt = (f.nx > 0 && f.ny > 0) ? f.nx * f.ny : 0;
         if (cnt > 0) {
             s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, EMIT ? (int)depth_word(a.clip_z[vo]) : 0);
-            s_ell[w][lane] = EllipseRec{c.x, c.y, e.x, e.y, e.z, e.w, 1.0f / e.z, 1.0f / e.w};
+            const EllipseRec er{c.x, c.y, e.x, e.y, e.z, e.w, 1.0f / e.z, 1.0f / e.w};
+            s_ell[w][lane] = er;
+            s_fast[w][lane] = fast_ellipse(er, f, geom.tile_clip_x, geom.tile_clip_y);
         }
     }
     const int incl = wave_incl_scan(cnt, lane);
     const int total = __shfl(incl, 63, 64);
-    s_excl[w][lane] = incl - cnt;
+    const int excl = incl - cnt;
+    s_excl[w][lane] = excl;
+    s_mark[w][lane] = ~0u;
     wave_sync();
     const size_t bslot = (size_t)view * a.batches + batch;
     const uint32_t out_base = EMIT ? a.batch_counts[bslot] : 0u;
@@ -492,14 +403,13 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
     for (int base = 0; base < total; base += 64) {
         const int cc = base + lane;
         bool pass = false;
-        int o = 0;
+        const int o = step_owner(s_mark[w], base, excl, cnt, lane);
         uint32_t tile = 0;
         if (cc < total) {
-            o = owner_of(s_excl[w], cc);
             const int4 rc = s_rect[w][o];
             int q, r;
             divmod_small(cc - s_excl[w][o], rc.z, q, r);
-            pass = tile_overlap(rc.x + r, rc.y + q, s_ell[w][o], geom);
+            pass = tile_overlap_fast(rc.x + r, rc.y + q, s_ell[w][o], s_fast[w][o], geom);
             tile = (uint32_t)(vd.tile_base + (rc.y + q) * geom.tiles_x + rc.x + r);
         }
         const uint64_t m = __ballot(pass);
@@ -511,6 +421,7 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
             }
         }
         running += (uint32_t)__popcll(m);
+        wave_sync();
     }
     if (!EMIT && lane == 0) a.batch_counts[bslot] = running;
 }
@@ -521,17 +432,18 @@ __global__ __launch_bounds__(kBuildThreads) void build_kernel(BuildArgs a) {
 // exact entry count and a 64-bit mask whose bit l says whether candidate l
 // (row-major in the footprint) passed the exact test -- the whole answer for
 // footprints of at most 64 tiles, so the emit pass needs no tests for them.
-// (Queueing the candidates the two centre tests leave open and running the
-// edge tests on full waves of them measured slower: most candidates need the
-// edge tests anyway.)
+// Each record lane gathers its own count and mask from each step's ballot.
+// (Walking the records straight from the projection's index-order arrays,
+// without compacting them first, measured slower: 62 % of their lanes are
+// visible, so the waves are more and emptier.)
 __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a) {
     constexpr int W = kBuildThreads / 64;
     __shared__ int s_excl[W][64];
     __shared__ int4 s_rect[W][64];       // x0, y0, nx, candidates
     __shared__ EllipseRec s_ell[W][64];
-    __shared__ float2 s_clip[W][64];     // tile size of the record's view, clip units
-    __shared__ uint32_t s_cnt[W][64];
-    __shared__ uint64_t s_mask[W][64];
+    __shared__ float4 s_fast[W][64];     // kv, kh, lo of the bounded test; tile width (clip units)
+    __shared__ float s_tcy[W][64];       // tile height (clip units) of the record's view
+    __shared__ uint32_t s_mark[W][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int limit = (int)min(*a.record_count, (uint32_t)a.nviews * (uint32_t)a.count);
     const int first = (blockIdx.x * W + w) * kBuildBatch;
@@ -546,47 +458,52 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
         const Footprint f = footprint(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, g);
         cnt = (f.nx > 0 && f.ny > 0) ? f.nx * f.ny : 0;
         s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, cnt);
-        s_ell[w][lane] = EllipseRec{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, 1.0f / r1.x, 1.0f / r1.y};
-        s_clip[w][lane] = make_float2(g.tile_clip_x, g.tile_clip_y);
+        const EllipseRec er{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, 1.0f / r1.x, 1.0f / r1.y};
+        s_ell[w][lane] = er;
+        const FastEllipse fe = fast_ellipse(er, f, g.tile_clip_x, g.tile_clip_y);
+        s_fast[w][lane] = make_float4(fe.kv, fe.kh, fe.lo, g.tile_clip_x);
+        s_tcy[w][lane] = g.tile_clip_y;
     }
-    s_cnt[w][lane] = 0u;
-    s_mask[w][lane] = 0ull;
+    s_mark[w][lane] = ~0u;
     const int incl = wave_incl_scan(cnt, lane);
     const int total = __shfl(incl, 63, 64);
-    s_excl[w][lane] = incl - cnt;
+    const int excl = incl - cnt;
+    s_excl[w][lane] = excl;
     wave_sync();
+    // this lane's record: its entries and the mask of its passing candidates 0..63, gathered
+    // from each step's ballot
+    uint32_t my_cnt = 0u;
+    uint64_t my_mask = 0ull;
     for (int base = 0; base < total; base += 64) {
         const int cc = base + lane;
         bool pass = false;
-        int o = 0, l = 0;
+        const int o = step_owner(s_mark[w], base, excl, cnt, lane);
         if (cc < total) {
-            o = owner_of(s_excl[w], cc);
-            l = cc - s_excl[w][o];
+            const int l = cc - s_excl[w][o];
             const int4 rc = s_rect[w][o];
             int q, r;
             divmod_small(l, rc.z, q, r);
+            const float4 sf = s_fast[w][o];
+            const EllipseRec& er = s_ell[w][o];
             FrameGeom og;
-            og.tile_clip_x = s_clip[w][o].x;
-            og.tile_clip_y = s_clip[w][o].y;
-            pass = tile_overlap(rc.x + r, rc.y + q, s_ell[w][o], og);
+            og.tile_clip_x = sf.w;
+            og.tile_clip_y = s_tcy[w][o];
+            pass = tile_overlap_fast(rc.x + r, rc.y + q, er, fast_ellipse_from(er, sf.x, sf.y, sf.z), og);
         }
         const uint64_t m = __ballot(pass);
-        // the lane holding a record's first candidate of this step adds the record's part
-        if (cc < total && (l == 0 || lane == 0)) {
-            const int len = min(s_excl[w][o] + s_rect[w][o].w, base + 64) - cc;
-            const uint64_t bits = (m >> lane) & low_bits(len);
-            s_cnt[w][o] += (uint32_t)__popcll(bits);
-            if (l < 64) s_mask[w][o] |= bits << l;
+        if (cnt > 0 && excl < base + 64 && excl + cnt > base) {
+            const int st = max(excl, base);
+            const uint64_t bits = (m >> (st - base)) & low_bits(min(excl + cnt, base + 64) - st);
+            my_cnt += (uint32_t)__popcll(bits);
+            if (st - excl < 64) my_mask |= bits << (st - excl);
         }
-        wave_sync();
     }
     if (pos < limit) {
-        const uint32_t c = s_cnt[w][lane];
-        a.record_counts[pos] = c;
-        a.record_masks[pos] = s_mask[w][lane];
+        a.record_counts[pos] = my_cnt;
+        a.record_masks[pos] = my_mask;
         // the depth sort's value: record index | count (saturated) in the bits above it
         const uint32_t esc = 0xffffffffu >> a.index_bits;
-        a.depth_words[pos] = (uint32_t)pos | (min(c, esc) << a.index_bits);
+        a.depth_words[pos] = (uint32_t)pos | (min(my_cnt, esc) << a.index_bits);
     }
 }
 
@@ -610,6 +527,8 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
     __shared__ float2 s_clip[W][64];
     __shared__ uint32_t s_done[W][64];
     __shared__ int s_cand[W][64];        // candidates of a large footprint
+    __shared__ float4 s_fast[W][64];     // kv, kh, lo of the bounded test (large footprints)
+    __shared__ uint32_t s_mark[W][64];
     __shared__ uint32_t s_hist[3][256];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool hist = a.tile_hist != nullptr;
@@ -665,19 +584,24 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
             } else {
                 big = cand;
                 s_cand[w][lane] = cand;
-                s_ell[w][lane] = EllipseRec{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, 1.0f / r1.x, 1.0f / r1.y};
+                const EllipseRec er{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, 1.0f / r1.x, 1.0f / r1.y};
+                s_ell[w][lane] = er;
+                const FastEllipse fe = fast_ellipse(er, f, vd.geom.tile_clip_x, vd.geom.tile_clip_y);
+                s_fast[w][lane] = make_float4(fe.kv, fe.kh, fe.lo, 0.0f);
                 s_clip[w][lane] = make_float2(vd.geom.tile_clip_x, vd.geom.tile_clip_y);
             }
         }
         // entries of the small footprints
         int incl = wave_incl_scan(ecnt, lane);
         int total = __shfl(incl, 63, 64);
-        s_excl[w][lane] = incl - ecnt;
+        int excl = incl - ecnt;
+        s_excl[w][lane] = excl;
+        s_mark[w][lane] = ~0u;
         wave_sync();
         for (int base = 0; base < total; base += 64) {
             const int e = base + lane;
+            const int o = step_owner(s_mark[w], base, excl, ecnt, lane);
             if (e < total) {
-                const int o = owner_of(s_excl[w], e);
                 const int k = e - s_excl[w][o];
                 const int l = select_bit(s_mask[w][o], k);
                 const int4 rc = s_rect[w][o];
@@ -685,21 +609,24 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
                 divmod_small(l, rc.z, q, r);
                 put(s_off[w][o] + (uint32_t)k, (uint32_t)(s_tb[w][o] + (rc.y + q) * rc.w + rc.x + r), o);
             }
+            wave_sync();
         }
         // candidates of the large footprints
         wave_sync();
         incl = wave_incl_scan(big, lane);
         total = __shfl(incl, 63, 64);
-        s_excl[w][lane] = incl - big;
+        excl = incl - big;
+        s_excl[w][lane] = excl;
         s_done[w][lane] = 0u;
+        s_mark[w][lane] = ~0u;
         wave_sync();
         for (int base = 0; base < total; base += 64) {
             const int cc = base + lane;
             bool pass = false;
-            int o = 0, l = 0;
+            int l = 0;
+            const int o = step_owner(s_mark[w], base, excl, big, lane);
             uint32_t tile = 0u;
             if (cc < total) {
-                o = owner_of(s_excl[w], cc);
                 l = cc - s_excl[w][o];
                 const int4 rc = s_rect[w][o];
                 int q, r;
@@ -707,7 +634,9 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
                 FrameGeom og;
                 og.tile_clip_x = s_clip[w][o].x;
                 og.tile_clip_y = s_clip[w][o].y;
-                pass = tile_overlap(rc.x + r, rc.y + q, s_ell[w][o], og);
+                const float4 sf = s_fast[w][o];
+                const EllipseRec& er = s_ell[w][o];
+                pass = tile_overlap_fast(rc.x + r, rc.y + q, er, fast_ellipse_from(er, sf.x, sf.y, sf.z), og);
                 tile = (uint32_t)(s_tb[w][o] + (rc.y + q) * rc.w + rc.x + r);
             }
             const uint64_t m = __ballot(pass);

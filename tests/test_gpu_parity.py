@@ -158,6 +158,24 @@ def test_radix_sort_stable(torch_cuda, n, begin, end):
     assert np.array_equal(sk, keys[order])
 
 
+@pytest.mark.parametrize("n", [1000, 70_001, 2_000_000])
+def test_radix_sort_constant_digits(torch_cuda, n):
+    """Keys whose second and fourth bytes are the same for every key (as the top byte of the
+    depth words of one view is): those passes are the identity and run as copies; the result
+    is numpy's stable sort on the 32 key bits."""
+    say(f"radix sort with constant digits n={n}")
+    rng = np.random.default_rng(n)
+    b0 = rng.integers(0, 8, n, dtype=np.uint64)            # few values: stability is exercised
+    b2 = rng.integers(0, 256, n, dtype=np.uint64)
+    keys = b0 | (np.uint64(0x5A) << np.uint64(8)) | (b2 << np.uint64(16)) | (np.uint64(0xFF) << np.uint64(24))
+    keys |= rng.integers(0, 1 << 20, n, dtype=np.uint64) << np.uint64(40)   # above the sorted bits
+    vals = np.arange(n, dtype=np.int32)
+    sk, sv = gpu_radix_sort(torch_cuda, keys, vals, 0, 32)
+    order = np.argsort(keys & np.uint64(0xFFFFFFFF), kind="stable")
+    assert np.array_equal(sv, vals[order])
+    assert np.array_equal(sk, keys[order])
+
+
 def test_empty_and_culled(torch_cuda, orc):
     say("empty scene / all culled")
     W, H = 256, 128
