@@ -3,11 +3,13 @@
 // core/renderer.cpp:94-139 / app/main.cpp:326-433.
 //
 // Frame (one HIP stream; the only host sync is at the end, after the image):
-//   SH (degree > 0) -> project (+ per-64 visibility ballot)
-//   -> compaction of the visible Gaussians into build records and depth
-//      words -> build count: exact entries and the mask of passing
-//      candidates per record, and the depth sort's value (record index with
-//      the count packed above it)
+//   front end, one pass per 256 Gaussians: SH colour (degree > 0), projection,
+//      visibility (non-empty tile rect) and compaction of the block's visible
+//      Gaussians into build records in the block's own staging slice
+//   -> scan of the block counts (the record numbering) -> staging slot of each
+//      record -> build count: exact entries, the mask of passing candidates and
+//      the depth word per record, and the depth sort's value (record index
+//      with the count packed above it)
 //   -> 32-bit onesweep sort of the (depth word, record | count) pairs -> the
 //      counts, now in depth order, are scanned (K) and scattered back as
 //      per-record output offsets
@@ -105,7 +107,7 @@ struct gr_renderer {
     int64_t last_visible = -1;
     int32_t last_passes = 0;
     int32_t n = 0, degree = 0, sh_count = 3, precision = GR_PRECISION_FP32;
-    DevBuf<float4> position, scale_rot, color, conic, ellipse;
+    DevBuf<float4> position, scale_rot, color, conic;
     DevBuf<uint2> packed;        // precision form: scale + opacity, 16-bit
     DevBuf<uint16_t> sh16;       // precision form: aligned SH, 16-bit
     DevBuf<float2> clip_xy;
@@ -115,15 +117,15 @@ struct gr_renderer {
     DevBuf<uint64_t> full_keys;     // the list in the reference's form, made on inspection
     DevBuf<int32_t> full_values;
     DevBuf<uchar4> out;
-    DevBuf<uint64_t> vis_bits;      // per 64 Gaussians: non-empty tile rect ballot
-    DevBuf<uint32_t> vis_counts;    // popcounts -> exclusive offsets, total, scan scratch
     DevBuf<uint32_t> batch_counts;  // total K, scan scratch
     DevBuf<uint32_t> rec_counts;    // entries per build record
     DevBuf<uint32_t> rec_offsets;   // output offset per build record
     DevBuf<uint64_t> rec_masks;     // passing candidates 0..63 per build record
     DevBuf<uint32_t> dkeys[2];      // depth words of the visible Gaussians
     DevBuf<int32_t> dorder[2];      // their build-record indices
-    DevBuf<float4> records;         // build records of the visible Gaussians (2 x float4 each)
+    DevBuf<float4> records;         // staged build records of the visible Gaussians (2 x float4 each)
+    DevBuf<uint32_t> blocks;        // front-end block counts (then offsets), the record total, scan scratch
+    DevBuf<uint32_t> slots;         // staging slot of each record
     void* sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
     int32_t sel = 0;
@@ -159,13 +161,13 @@ gr_status gr_renderer_destroy(gr_renderer* r) {
     (void)hipSetDevice(r->device);
     if (r->ctx) (void)gr_context_synchronize(r->ctx);
     r->position.release(); r->scale_rot.release(); r->color.release(); r->conic.release();
-    r->ellipse.release(); r->clip_xy.release(); r->clip_z.release(); r->sh.release();
+    r->clip_xy.release(); r->clip_z.release(); r->sh.release();
     r->packed.release(); r->sh16.release();
     r->tile_range.release(); r->entries[0].release(); r->entries[1].release(); r->full_keys.release();
     r->full_values.release(); r->out.release();
-    r->vis_bits.release(); r->vis_counts.release(); r->batch_counts.release();
+    r->batch_counts.release();
     r->dkeys[0].release(); r->dkeys[1].release(); r->dorder[0].release(); r->dorder[1].release();
-    r->records.release(); r->rec_counts.release(); r->rec_offsets.release(); r->rec_masks.release();
+    r->records.release(); r->blocks.release(); r->slots.release(); r->rec_counts.release(); r->rec_offsets.release(); r->rec_masks.release();
     if (r->sort_temp) (void)hipFree(r->sort_temp);
     for (auto& e : r->ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : r->pass_ev) if (e) (void)hipEventDestroy(e);
@@ -198,7 +200,6 @@ gr_status gr_renderer_upload_model(gr_renderer* r, const gr_model_host* m) {
     R_CHECK(r->scale_rot.ensure(nn));
     R_CHECK(r->color.ensure(nn));
     R_CHECK(r->conic.ensure(nn));
-    R_CHECK(r->ellipse.ensure(nn));
     R_CHECK(r->clip_xy.ensure(nn));
     R_CHECK(r->clip_z.ensure(nn));
     if (m->precision != GR_PRECISION_FP32) {
@@ -269,8 +270,6 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     const int n = r->n;
     const int batches = (n + kBuildBatch - 1) / kBuildBatch;
     const size_t nn = (size_t)std::max(n, 1);
-    R_CHECK(r->vis_bits.ensure((size_t)std::max(batches, 1)));
-    R_CHECK(r->vis_counts.ensure((size_t)batches + 2 + scan_scratch_words(batches)));
     R_CHECK(r->batch_counts.ensure(2 + scan_scratch_words((int32_t)nn)));
     R_CHECK(r->rec_counts.ensure(nn));
     R_CHECK(r->rec_offsets.ensure(nn));
@@ -280,7 +279,10 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         R_CHECK(r->dkeys[k].ensure(nn));
         R_CHECK(r->dorder[k].ensure(nn));
     }
-    R_CHECK(r->records.ensure(2 * nn));
+    const int nb = front_blocks(n);   // 0 for an empty scene
+    R_CHECK(r->records.ensure((size_t)2 * kFrontThreads * std::max(nb, 1)));
+    R_CHECK(r->blocks.ensure((size_t)nb + 1 + scan_scratch_words(nb)));
+    R_CHECK(r->slots.ensure(nn));
     // one scratch serves the depth sort and the tile-bit sort; sized before anything is enqueued
     const int32_t capacity = (int32_t)std::min<size_t>(std::min(r->entries[0].cap, r->entries[1].cap), (1u << 30) - 1);
     R_CHECK(ensure_temp(r->sort_temp, r->sort_temp_bytes,
@@ -292,7 +294,6 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     vd.tile_base = 0;
     vd.tiles = tiles;
     vd.out = outp;
-    uint32_t* vis_total = r->vis_counts.p + batches;
     uint32_t* k_total = r->batch_counts.p;
 
     // Timing.  Per-stage events run only when stats are requested; per-kernel marks only at
@@ -309,63 +310,43 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         r->mark_name[r->marks] = name;
         return hipEventRecord(r->mark_ev[r->marks++], stream);
     };
+    uint32_t* vis_total = r->blocks.p + nb;   // visible Gaussians (records)
     R_HIP(stage(0));
     R_HIP(mark("begin"));
-    if (r->degree > 0 && n > 0) {
-        ShArgs a{};
-        a.position = r->position.p;
-        a.sh = r->precision != GR_PRECISION_FP32 ? (const void*)r->sh16.p : (const void*)r->sh.p;
-        a.packed = r->precision != GR_PRECISION_FP32 ? r->packed.p : nullptr;
-        a.color = r->color.p;
-        a.count = n;
-        a.sh_count = r->sh_count;
-        a.nviews = 1;
-        a.v0 = vd;
-        R_HIP(launch_sh(a, r->degree, r->precision, stream));
-    }
-    R_HIP(stage(1));
-    R_HIP(mark("sh"));
     {
-        ProjectArgs a{};
+        // SH colour + projection + compaction of the visible Gaussians into build records
+        FrontArgs a{};
         a.position = r->position.p;
         a.scale_rotation = r->scale_rot.p;
         a.packed = r->packed.p;
+        a.sh = r->precision != GR_PRECISION_FP32 ? (const void*)r->sh16.p : (const void*)r->sh.p;
+        a.degree = n > 0 ? r->degree : 0;
+        a.sh_count = r->sh_count;
         a.precision = r->precision;
+        a.color = r->color.p;
         a.clip_xy = r->clip_xy.p;
         a.clip_z = r->clip_z.p;
-        a.ellipse = r->ellipse.p;
+        a.ellipse = nullptr;   // one view: the ellipses live in the records
         a.conic = r->conic.p;
-        a.vis_bits = r->vis_bits.p;
-        a.vis_counts = r->vis_counts.p;
-        a.batches = batches;
+        a.records = r->records.p;
+        a.block_counts = r->blocks.p;
         a.count = n;
         a.nviews = 1;
         a.v0 = vd;
-        R_HIP(launch_project(a, stream));
+        R_HIP(launch_front(a, stream));
     }
-    R_HIP(stage(2));
-    R_HIP(mark("project"));
-    // visible Gaussians -> build records + (depth word, record) pairs
-    R_HIP(launch_build_scan(r->vis_counts.p, batches, vis_total, vis_total + 1, stream));
-    {
-        CompactArgs a{};
-        a.vis_bits = r->vis_bits.p;
-        a.vis_offsets = r->vis_counts.p;
-        a.clip_xy = r->clip_xy.p;
-        a.clip_z = r->clip_z.p;
-        a.ellipse = r->ellipse.p;
-        a.records = r->records.p;
-        a.depth_keys = r->dkeys[0].p;
-        a.count = n;
-        a.batches = batches;
-        a.nviews = 1;
-        R_HIP(launch_compact_depth(a, stream));
-    }
-    R_HIP(mark("compact"));
-    // exact entry count and candidate mask of every record (index order)
+    R_HIP(stage(1));
+    R_HIP(mark("front"));
+    // record numbering: block offsets + the total, then each record's staging slot
+    R_HIP(launch_build_scan(r->blocks.p, nb, vis_total, r->blocks.p + nb + 1, stream));
+    R_HIP(launch_record_slots(r->blocks.p, vis_total, nb, kFrontThreads, r->slots.p, stream));
+    R_HIP(mark("numbering"));
+    // exact entry count, candidate mask and depth word of every record
     BuildArgs b{};
     b.records = r->records.p;
-    b.record_count = vis_total;
+    b.record_slots = r->slots.p;
+    b.record_total = vis_total;
+    b.depth_keys = r->dkeys[0].p;
     b.record_counts = r->rec_counts.p;
     b.record_masks = r->rec_masks.p;
     b.depth_words = reinterpret_cast<uint32_t*>(r->dorder[0].p);
@@ -377,6 +358,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     b.nviews = 1;
     b.v0 = vd;
     R_HIP(launch_build_count(b, stream));
+    R_HIP(stage(2));
     R_HIP(stage(3));
     R_HIP(mark("record_count"));
     // depth order of the records
@@ -477,14 +459,14 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     r->last_w = width;
     r->last_h = height;
     if (stats) {
-        // events: 0 SH 1 project 2 compaction + count 3 depth sort 4 offsets + emit + the tile
-        // sort's histogram scan 5 tile-bit sort passes 6 ranges + raster 7
+        // events: 0 front end (SH + projection + compaction) 1 count 2,3 depth sort 4 offsets +
+        // emit + the tile sort's histogram scan 5 tile-bit sort passes 6 ranges + raster 7
         float t[8];
         for (int k = 0; k < 7; ++k) R_HIP(hipEventElapsedTime(&t[k], r->ev[k], r->ev[k + 1]));
         R_HIP(hipEventElapsedTime(&t[7], r->ev[0], r->ev[7]));
-        stats->sh_eval_ms = t[0];
-        stats->project_ms = t[1];
-        stats->build_tile_ms = t[2] + t[4];   // compaction, count, offsets, emit (+ histogram scan)
+        stats->sh_eval_ms = t[0];   // the fused front end: SH, projection and compaction
+        stats->project_ms = 0.0;    // folded into the front end
+        stats->build_tile_ms = t[1] + t[4];   // numbering, count, offsets, emit (+ histogram scan)
         stats->sort_ms = t[3] + t[5];         // depth sort + tile-bit sort
         stats->raster_ms = t[6];              // ranges + raster
         stats->total_ms = t[7];
@@ -531,7 +513,7 @@ gr_status gr_renderer_frame_view(gr_renderer* r, gr_frame_view* v) {
     v->color = reinterpret_cast<const float*>(r->color.p);
     v->position_clip_xy = reinterpret_cast<const float*>(r->clip_xy.p);
     v->position_clip_z = r->clip_z.p;
-    v->screen_ellipse = reinterpret_cast<const float*>(r->ellipse.p);
+    v->screen_ellipse = nullptr;   // the frame keeps the ellipses in its build records only
     v->conic = reinterpret_cast<const float*>(r->conic.p);
     // the list is held as (index << 32 | tile) entries in (tile, depth word, index) order; its
     // reference-form keys and values are made here, once per frame inspected

@@ -6,10 +6,12 @@
 // thread owns every HIP call of the render path.  Each slot is a stream plus
 // a grow-only workspace; a batch (up to max_views requests of one scene, any
 // mix of resolutions) runs on one slot in two phases:
-//   A  camera block H2D -> SH (all views) -> projection (all views, with the
-//      visibility ballot) -> compaction of every view's visible Gaussians
-//      into build records -> build count per record (entries and mask of
-//      passing candidates) -> one 32-bit sort of all views' records by depth
+//   A  camera block H2D -> front end, one pass over the scene for all views:
+//      SH colour, projection, visibility and compaction of every view's
+//      visible Gaussians into build records (each block of the scene into its own staging
+//      slice) -> record numbering (scan of the block counts, staging slot per record) ->
+//      build count per record (entries, mask of passing candidates, depth word) -> one
+//      32-bit sort of all views' records by depth
 //      word -> output offsets from the counts in depth order -> K readback
 //   B  emit (entries at their depth-order offsets, tile-digit histograms)
 //      -> onesweep sort of the tile bits (device-side K) -> ranges -> raster
@@ -152,9 +154,10 @@ enum class SlotState { Idle, PhaseA, WaitMemory, PhaseB };
 
 struct Workspace {
     Dev color, xy, z, ell, con;      // [V][N]
-    Dev vis_bits, vis_counts;         // [V][batches] ballots; popcounts -> offsets, total, scan scratch
+    Dev blocks;                       // front-end block counts (then offsets), the record count, scan scratch
     Dev dkeys0, dkeys1, dorder0, dorder1;   // [<= V*N] depth words and record indices of visible Gaussians
-    Dev records;                      // [<= V*N][2] float4 build records
+    Dev records;                      // [blocks][256 * V][2] float4 staged build records
+    Dev slots;                        // [<= V*N] staging slot of each record
     Dev dsort_temp;                   // depth sort scratch
     Dev counts;                       // K (device) + scan scratch
     Dev rec_counts, rec_offsets;      // [V*N] entries and output offset per record
@@ -168,8 +171,8 @@ struct Workspace {
     Pinned img_host[2];               // image staging, alternating between batches
     uint64_t generation = 0;          // bumps when a pointer a captured graph holds changes
     std::vector<Dev*> devs() {
-        return {&color, &xy, &z, &ell, &con, &vis_bits, &vis_counts, &dkeys0, &dkeys1, &dorder0, &dorder1,
-                &records, &rec_counts, &rec_offsets, &rec_masks, &dsort_temp, &counts, &list0, &list1, &sort_temp, &ranges, &images, &views};
+        return {&color, &xy, &z, &ell, &con, &blocks, &dkeys0, &dkeys1, &dorder0, &dorder1,
+                &records, &slots, &rec_counts, &rec_offsets, &rec_masks, &dsort_temp, &counts, &list0, &list1, &sort_temp, &ranges, &images, &views};
     }
 };
 
@@ -288,8 +291,11 @@ constexpr int32_t kBatchTiles = 1 << 16;   // tiles per batch beyond which views
 constexpr int64_t kMaxRecords = (1 << 30) - 1;   // views x Gaussians of a batch (depth sort's count)
 
 // Views of an n-Gaussian scene one batch may hold: each view adds n depth-sort records.
+// At most kFrontMaxViews views per batch (the front end's one pass over the scene).
 int max_views_for(const gr_server* s, int32_t n) {
-    return (int)std::max<int64_t>(1, std::min<int64_t>(s->cfg.max_views_per_batch, kMaxRecords / std::max(n, 1)));
+    const int64_t cap = std::min<int64_t>(std::min<int64_t>(s->cfg.max_views_per_batch, kFrontMaxViews),
+                                          kMaxRecords / std::max(n, 1));
+    return (int)std::max<int64_t>(1, cap);
 }
 
 Scene* scene_ptr(gr_server* s, int32_t id) {
@@ -354,8 +360,6 @@ void copier(gr_server* s) {
     }
 }
 
-int64_t per_view_bytes(const Scene& sc) { return (int64_t)sc.n * (16 + 8 + 4 + 16 + 16); }
-
 // Phase-A buffers of a slot and the bytes a batch needs in each.
 struct PhaseANeed {
     Dev* dev;
@@ -365,20 +369,19 @@ struct PhaseANeed {
 std::vector<PhaseANeed> phase_a_needs(Slot& sl, const Scene& sc, size_t nviews, int64_t img_bytes,
                                       int32_t tiles_total) {
     Workspace& w = sl.ws;
-    const int64_t V = (int64_t)nviews, N = sc.n;
-    const int64_t batches = (N + kBuildBatch - 1) / kBuildBatch;
+    const int64_t V = (int64_t)nviews, N = sc.n, nb = front_blocks(sc.n);
     return {{&w.color, V * N * 16},
             {&w.xy, V * N * 8},
             {&w.z, V * N * 4},
             {&w.ell, V * N * 16},
             {&w.con, V * N * 16},
-            {&w.vis_bits, V * batches * 8},
-            {&w.vis_counts, 4 * (V * batches + 2 + scan_scratch_words((int32_t)(V * batches)))},
+            {&w.blocks, 4 * (nb + 1 + scan_scratch_words((int32_t)nb))},
             {&w.dkeys0, V * N * 4},
             {&w.dkeys1, V * N * 4},
             {&w.dorder0, V * N * 4},
             {&w.dorder1, V * N * 4},
-            {&w.records, V * N * 32},
+            {&w.records, V * std::max<int64_t>(nb, 1) * kFrontThreads * 32},
+            {&w.slots, V * N * 4},
             {&w.dsort_temp, (int64_t)radix_sort_temp_bytes((int32_t)std::max<int64_t>(V * N, 1), 0, 32)},
             {&w.counts, 4 * (2 + scan_scratch_words((int32_t)(V * N)))},
             {&w.rec_counts, V * N * 4},
@@ -465,11 +468,13 @@ BuildArgs record_build_args(Slot& sl, const Scene& sc) {
     Workspace& w = sl.ws;
     const int V = (int)sl.batch.size();
     const int N = sc.n;
-    const int nbv = V * ((N + kBuildBatch - 1) / kBuildBatch);
+
     BuildArgs b{};
     b.entries = w.list0.as<uint64_t>();
     b.records = w.records.as<float4>();
-    b.record_count = w.vis_counts.as<uint32_t>() + nbv;
+    b.record_slots = w.slots.as<uint32_t>();
+    b.record_total = w.blocks.as<uint32_t>() + front_blocks(N);
+    b.depth_keys = w.dkeys0.as<uint32_t>();
     b.record_counts = w.rec_counts.as<uint32_t>();
     b.record_offsets = w.rec_offsets.as<uint32_t>();
     b.record_masks = w.rec_masks.as<uint64_t>();
@@ -489,72 +494,50 @@ gr_status enqueue_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     Workspace& w = sl.ws;
     const int V = (int)sl.batch.size();
     const int N = sc.n;
-    const int batches = (N + kBuildBatch - 1) / kBuildBatch;
     ViewDesc* vdev = w.views.as<ViewDesc>();
     const ViewDesc* vhost = w.views_host.as<ViewDesc>();
     S_HIP(hipMemcpyAsync(vdev, vhost, V * sizeof(ViewDesc), hipMemcpyHostToDevice, sl.stream));
-    if (sc.degree > 0) {
-        ShArgs a{};
-        a.position = sc.position.as<float4>();
-        a.sh = sc.sh.p;
-        a.packed = sc.packed.as<uint2>();
-        a.color = w.color.as<float4>();
-        a.count = N;
-        a.sh_count = sc.sh_count;
-        a.nviews = V;
-        a.v0 = vhost[0];
-        a.views = vdev;
-        S_HIP(launch_sh(a, sc.degree, sc.precision, sl.stream));
-    } else {
+    if (sc.degree == 0) {
         for (int v = 0; v < V; ++v)
             S_HIP(hipMemcpyAsync(w.color.as<float4>() + (size_t)v * N, sc.color0.p, (size_t)N * 16,
                                  hipMemcpyDeviceToDevice, sl.stream));
     }
-    const int nbv = V * batches;
-    uint32_t* vis = w.vis_counts.as<uint32_t>();
-    ProjectArgs p{};
-    p.position = sc.position.as<float4>();
-    p.scale_rotation = sc.scale_rot.as<float4>();
-    p.packed = sc.packed.as<uint2>();
-    p.precision = sc.precision;
-    p.clip_xy = w.xy.as<float2>();
-    p.clip_z = w.z.as<float>();
-    p.ellipse = w.ell.as<float4>();
-    p.conic = w.con.as<float4>();
-    p.vis_bits = w.vis_bits.as<uint64_t>();
-    p.vis_counts = vis;
-    p.batches = batches;
-    p.count = N;
-    p.nviews = V;
-    p.v0 = vhost[0];
-    p.views = vdev;
-    S_HIP(launch_project(p, sl.stream));
-    // depth order of all views' visible Gaussians
-    S_HIP(launch_build_scan(vis, nbv, vis + nbv, vis + nbv + 2, sl.stream));
-    CompactArgs c{};
-    c.vis_bits = w.vis_bits.as<uint64_t>();
-    c.vis_offsets = vis;
-    c.clip_xy = w.xy.as<float2>();
-    c.clip_z = w.z.as<float>();
-    c.ellipse = w.ell.as<float4>();
-    c.records = w.records.as<float4>();
-    c.depth_keys = w.dkeys0.as<uint32_t>();
-    c.count = N;
-    c.batches = batches;
-    c.nviews = V;
-    S_HIP(launch_compact_depth(c, sl.stream));
+    const int nb = front_blocks(N);
+    uint32_t* vis = w.blocks.as<uint32_t>() + nb;   // records of the batch
+    FrontArgs f{};
+    f.position = sc.position.as<float4>();
+    f.scale_rotation = sc.scale_rot.as<float4>();
+    f.packed = sc.packed.as<uint2>();
+    f.sh = sc.sh.p;
+    f.degree = sc.degree;
+    f.sh_count = sc.sh_count;
+    f.precision = sc.precision;
+    f.color = w.color.as<float4>();
+    f.clip_xy = w.xy.as<float2>();
+    f.clip_z = w.z.as<float>();
+    f.ellipse = w.ell.as<float4>();
+    f.conic = w.con.as<float4>();
+    f.records = w.records.as<float4>();
+    f.block_counts = w.blocks.as<uint32_t>();
+    f.count = N;
+    f.nviews = V;
+    f.v0 = vhost[0];
+    f.views = vdev;
+    S_HIP(launch_front(f, sl.stream));
+    S_HIP(launch_build_scan(w.blocks.as<uint32_t>(), nb, vis, w.blocks.as<uint32_t>() + nb + 1, sl.stream));
+    S_HIP(launch_record_slots(w.blocks.as<uint32_t>(), vis, nb, kFrontThreads * V, w.slots.as<uint32_t>(),
+                              sl.stream));
     BuildArgs b = record_build_args(sl, sc);
     S_HIP(launch_build_count(b, sl.stream));
     uint32_t* dk[2] = {w.dkeys0.as<uint32_t>(), w.dkeys1.as<uint32_t>()};
     int32_t* dv[2] = {w.dorder0.as<int32_t>(), w.dorder1.as<int32_t>()};
     int32_t dsel = 0;
-    S_HIP(radix_sort_pairs_u32(dk, dv, &dsel, V * N, vis + nbv, 0, 32, w.dsort_temp.p, w.dsort_temp.bytes,
-                               sl.stream));
+    S_HIP(radix_sort_pairs_u32(dk, dv, &dsel, V * N, vis, 0, 32, w.dsort_temp.p, w.dsort_temp.bytes, sl.stream));
     uint32_t* counts = w.counts.as<uint32_t>();
     const int vn = V * N;
     // output offsets: the counts carried by the sorted words, scanned in depth order (K) and
     // scattered back to the records
-    S_HIP(launch_build_scan_packed(reinterpret_cast<const uint32_t*>(dv[dsel]), vn, vis + nbv,
+    S_HIP(launch_build_scan_packed(reinterpret_cast<const uint32_t*>(dv[dsel]), vn, vis,
                                    w.rec_counts.as<uint32_t>(), b.index_bits, counts, counts + 2,
                                    w.rec_offsets.as<uint32_t>(), sl.stream));
     S_HIP(hipMemcpyAsync(w.k_host.p, counts, 4, hipMemcpyDeviceToHost, sl.stream));

@@ -57,77 +57,92 @@ template <> __device__ __forceinline__ float h16<uint16_t>(uint32_t bits) {
     return __uint_as_float(bits << 16);
 }
 
+// Colour of one Gaussian seen from camera position cp: the reference's band sums
+// (core/gaussian_render.cu:62-92, band 3 core/cuda_renderer.cu:119-157), clamped, with the
+// given opacity in w.
+template <int DEG>
+__device__ __forceinline__ float4 sh_color(const float (&c)[3 * (DEG + 1) * (DEG + 1)], const float4& p, v3 cp,
+                                           float opacity) {
+    const v3 dir = normalize3(v3{cp.x - p.x, cp.y - p.y, cp.z - p.z});
+    const float x = dir.x, y = dir.y, z = dir.z;
+    float res[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) res[ch] = 0.0f + 0.282094792f * c[ch];
+    if (DEG > 0) {
+        const float k1 = 0.488602512f * y, k2 = 0.488602512f * z, k3 = 0.488602512f * x;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const float t = (k1 * c[3 + ch] + k2 * c[6 + ch]) + k3 * c[9 + ch];
+            res[ch] = res[ch] + t;
+        }
+        if (DEG > 1) {
+            const float xx = x * x, yy = y * y;
+            const float k4 = (1.09254843f * x) * y;
+            const float k5 = (1.09254843f * y) * z;
+            const float k6 = ((-0.946174696f * xx) - 0.946174696f * yy) + 0.630783131f;
+            const float k7 = (1.09254843f * x) * z;
+            const float k8 = (0.546274215f * (x - y)) * (x + y);
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                const float t = (((k4 * c[12 + ch] + k5 * c[15 + ch]) + k6 * c[18 + ch]) + k7 * c[21 + ch]) +
+                                k8 * c[24 + ch];
+                res[ch] = res[ch] + t;
+            }
+            if (DEG > 2) {
+                const float k9 = (0.295021795f * y) * (6.0f * xx - 2.0f * yy);
+                const float k10 = ((2.89061144f * x) * y) * z;
+                const float k11 = (3.6563664f * y) * ((-0.625f * xx - 0.625f * yy) + 0.5f);
+                const float k12 = (0.373176333f * z) * ((-5.0f * xx - 5.0f * yy) + 2.0f);
+                const float k13 = (0.457045799f * x) * ((-5.0f * xx - 5.0f * yy) + 4.0f);
+                const float k14 = ((1.44530572f * z) * (x - y)) * (x + y);
+                const float k15 = (0.59004359f * x) * (xx - 3.0f * yy);
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) {
+                    const float t = (((((k9 * c[27 + ch] + k10 * c[30 + ch]) + k11 * c[33 + ch]) +
+                                       k12 * c[36 + ch]) + k13 * c[39 + ch]) + k14 * c[42 + ch]) +
+                                    k15 * c[45 + ch];
+                    res[ch] = res[ch] + t;
+                }
+            }
+        }
+    }
+    float4 out;
+    out.x = gclamp(res[0] + 0.5f, 0.0f, 1.0f);
+    out.y = gclamp(res[1] + 0.5f, 0.0f, 1.0f);
+    out.z = gclamp(res[2] + 0.5f, 0.0f, 1.0f);
+    out.w = opacity;
+    return out;
+}
+
+// The coefficients of Gaussian i of SH group `grp` (lane k of the group) in registers, and its
+// opacity: one 4-byte (2-byte) load per coefficient and lane, each wave-coalesced (staging the
+// group's contiguous block through LDS with 16-byte loads measured slower: 289 vs 227 us at 6M
+// Gaussians).
+template <int DEG, typename T>
+__device__ __forceinline__ void sh_load(const void* shp, int sh_count, int grp, int k,
+                                        float (&c)[3 * (DEG + 1) * (DEG + 1)]) {
+    constexpr int NC = 3 * (DEG + 1) * (DEG + 1);
+    const T* sh = static_cast<const T*>(shp) + (size_t)grp * kShGroup * sh_count + k;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) c[j] = sh_ld<T>(sh + (size_t)j * kShGroup);
+}
+
 template <int DEG, typename T>
 __global__ __launch_bounds__(256) void sh_kernel(ShArgs a) {
     constexpr int NC = 3 * (DEG + 1) * (DEG + 1);
     const int i = blockIdx.x * kShGroup + threadIdx.x;
     if (i >= a.count) return;
     const float4 p = a.position[i];
-    // one 4-byte (2-byte) load per coefficient and lane, each wave-coalesced; staging the
-    // group's contiguous block through LDS with 16-byte loads measured slower (289 vs 227 us
-    // at 6M Gaussians)
-    const T* sh = static_cast<const T*>(a.sh) + (size_t)blockIdx.x * kShGroup * a.sh_count + threadIdx.x;
     float c[NC];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) c[k] = sh_ld<T>(sh + (size_t)k * kShGroup);
+    sh_load<DEG, T>(a.sh, a.sh_count, blockIdx.x, threadIdx.x, c);
     float opacity;
     if constexpr (sizeof(T) == 2) {
         opacity = h16<T>(a.packed[i].y >> 16);   // opacity of the precision form
     } else {
         opacity = p.w;
     }
-    for (int v = 0; v < a.nviews; ++v) {
-        const v3 cp = view_of(v, a.v0, a.views).cam.position;
-        const v3 dir = normalize3(v3{cp.x - p.x, cp.y - p.y, cp.z - p.z});
-        const float x = dir.x, y = dir.y, z = dir.z;
-        float res[3];
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch) res[ch] = 0.0f + 0.282094792f * c[ch];
-        if (DEG > 0) {
-            const float k1 = 0.488602512f * y, k2 = 0.488602512f * z, k3 = 0.488602512f * x;
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                const float t = (k1 * c[3 + ch] + k2 * c[6 + ch]) + k3 * c[9 + ch];
-                res[ch] = res[ch] + t;
-            }
-            if (DEG > 1) {
-                const float xx = x * x, yy = y * y;
-                const float k4 = (1.09254843f * x) * y;
-                const float k5 = (1.09254843f * y) * z;
-                const float k6 = ((-0.946174696f * xx) - 0.946174696f * yy) + 0.630783131f;
-                const float k7 = (1.09254843f * x) * z;
-                const float k8 = (0.546274215f * (x - y)) * (x + y);
-#pragma unroll
-                for (int ch = 0; ch < 3; ++ch) {
-                    const float t = (((k4 * c[12 + ch] + k5 * c[15 + ch]) + k6 * c[18 + ch]) + k7 * c[21 + ch]) +
-                                    k8 * c[24 + ch];
-                    res[ch] = res[ch] + t;
-                }
-                if (DEG > 2) {
-                    const float k9 = (0.295021795f * y) * (6.0f * xx - 2.0f * yy);
-                    const float k10 = ((2.89061144f * x) * y) * z;
-                    const float k11 = (3.6563664f * y) * ((-0.625f * xx - 0.625f * yy) + 0.5f);
-                    const float k12 = (0.373176333f * z) * ((-5.0f * xx - 5.0f * yy) + 2.0f);
-                    const float k13 = (0.457045799f * x) * ((-5.0f * xx - 5.0f * yy) + 4.0f);
-                    const float k14 = ((1.44530572f * z) * (x - y)) * (x + y);
-                    const float k15 = (0.59004359f * x) * (xx - 3.0f * yy);
-#pragma unroll
-                    for (int ch = 0; ch < 3; ++ch) {
-                        const float t = (((((k9 * c[27 + ch] + k10 * c[30 + ch]) + k11 * c[33 + ch]) +
-                                           k12 * c[36 + ch]) + k13 * c[39 + ch]) + k14 * c[42 + ch]) +
-                                        k15 * c[45 + ch];
-                        res[ch] = res[ch] + t;
-                    }
-                }
-            }
-        }
-        float4 out;
-        out.x = gclamp(res[0] + 0.5f, 0.0f, 1.0f);
-        out.y = gclamp(res[1] + 0.5f, 0.0f, 1.0f);
-        out.z = gclamp(res[2] + 0.5f, 0.0f, 1.0f);
-        out.w = opacity;
-        a.color[(size_t)v * a.count + i] = out;
-    }
+    for (int v = 0; v < a.nviews; ++v)
+        a.color[(size_t)v * a.count + i] = sh_color<DEG>(c, p, view_of(v, a.v0, a.views).cam.position, opacity);
 }
 
 template <typename T>
@@ -159,38 +174,41 @@ hipError_t launch_sh(const ShArgs& a, int degree, int precision, hipStream_t s) 
 // atan2 -> cos/sin (core/gaussian_render.cu:200,212); the two differ by about
 // one ulp and the direct form is exactly reproducible on the host.
 // ---------------------------------------------------------------------------
+// World covariance of one Gaussian (core/gaussian_render.cu:140-154): scale and the packed
+// quaternion bits from the FP32 form (scale_rotation) or the precision form (16-bit scale in
+// `packed`, quaternion bits in position.w).
 template <typename T>
-__global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.count) return;
-    const float4 pd = a.position[i];
+__device__ __forceinline__ m3 world_sigma(const float4& pd, const float4* scale_rotation, const uint2* packed, int i) {
     float4 sr;
     if constexpr (sizeof(T) == 2) {
-        // precision form: scale from 16-bit halves, quaternion bits in position.w
-        const uint2 pk = a.packed[i];
+        const uint2 pk = packed[i];
         sr = make_float4(h16<T>(pk.x & 0xffffu), h16<T>(pk.x >> 16), h16<T>(pk.y & 0xffffu), pd.w);
     } else {
-        sr = a.scale_rotation[i];
+        sr = scale_rotation[i];
     }
-
     m3 S;
     S.c[0] = v3{sr.x, 0.0f, 0.0f};
     S.c[1] = v3{0.0f, sr.y, 0.0f};
     S.c[2] = v3{0.0f, 0.0f, sr.z};
-
     const uint32_t q8 = __float_as_uint(sr.w);
     const float rx = ((float)((q8 >> 24u) & 0xffu) / 255.0f) * 2.0f - 1.0f;
     const float ry = ((float)((q8 >> 16u) & 0xffu) / 255.0f) * 2.0f - 1.0f;
     const float rz = ((float)((q8 >> 8u) & 0xffu) / 255.0f) * 2.0f - 1.0f;
     const float rw = ((float)(q8 & 0xffu) / 255.0f) * 2.0f - 1.0f;
     const m3 RS = mul33(quat_to_m3(rw, rx, ry, rz), S);
-    const m3 sigma = mul33(RS, transpose3(RS));
+    return mul33(RS, transpose3(RS));
+}
 
-    for (int vi = 0; vi < a.nviews; ++vi) {
-    const ViewDesc& vd = view_of(vi, a.v0, a.views);
+struct ProjOut {
+    float cx, cy, cz;       // clip position (-128 when culled)
+    float4 ellipse;         // cos, sin, extent x, extent y
+    float4 conic;           // (a, b, c, 0) of the inverse 2D covariance
+};
+
+// One Gaussian in one view: EWA projection, core/gaussian_render.cu:156-213.
+__device__ __forceinline__ ProjOut project_view(const float4& pd, const m3& sigma, const ViewDesc& vd) {
     const CameraArgs& cam = vd.cam;
     const FrameGeom& geom = vd.geom;
-    const size_t o = (size_t)vi * a.count + i;
     const v4 vp4 = mul44v(cam.view, v4{pd.x, pd.y, pd.z, 1.0f});
     const v3 v = v3{vp4.x, vp4.y, vp4.z};
     const v2 fc = cam.fov_cotangent;
@@ -243,20 +261,36 @@ __global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
     extx = extx * vis;
     exty = exty * vis;
 
-    a.clip_xy[o] = make_float2(clip.x, clip.y);
-    a.clip_z[o] = clip.z;
-    a.ellipse[o] = make_float4(e0.x, e0.y, extx, exty);
-    a.conic[o] = make_float4(conx, cony, conz, 0.0f);
-    if (a.vis_bits) {
-        // lanes of a wave are Gaussians 64b .. 64b+63 (block of 256, i contiguous)
-        const Footprint f = footprint(clip.x, clip.y, e0.x, e0.y, extx, exty, geom);
-        const uint64_t m = __ballot(f.nx > 0 && f.ny > 0);
-        if ((threadIdx.x & 63) == 0) {
-            const size_t b = (size_t)vi * a.batches + (i >> 6);
-            a.vis_bits[b] = m;
-            a.vis_counts[b] = (uint32_t)__popcll(m);
-        }
-    }
+    ProjOut o;
+    o.cx = clip.x;
+    o.cy = clip.y;
+    o.cz = clip.z;
+    o.ellipse = make_float4(e0.x, e0.y, extx, exty);
+    o.conic = make_float4(conx, cony, conz, 0.0f);
+    return o;
+}
+
+// Whether a projected Gaussian has a non-empty tile rect (it then gets build candidates).
+__device__ __forceinline__ bool has_tiles(const ProjOut& o, const FrameGeom& g) {
+    const Footprint f = footprint(o.cx, o.cy, o.ellipse.x, o.ellipse.y, o.ellipse.z, o.ellipse.w, g);
+    return f.nx > 0 && f.ny > 0;
+}
+
+// One lane per Gaussian, every view of the batch.
+template <typename T>
+__global__ __launch_bounds__(256) void project_kernel(ProjectArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.count) return;
+    const float4 pd = a.position[i];
+    const m3 sigma = world_sigma<T>(pd, a.scale_rotation, a.packed, i);
+    for (int vi = 0; vi < a.nviews; ++vi) {
+        const ViewDesc& vd = view_of(vi, a.v0, a.views);
+        const size_t o = (size_t)vi * a.count + i;
+        const ProjOut po = project_view(pd, sigma, vd);
+        a.clip_xy[o] = make_float2(po.cx, po.cy);
+        a.clip_z[o] = po.cz;
+        a.ellipse[o] = po.ellipse;
+        a.conic[o] = po.conic;
     }
 }
 
@@ -445,14 +479,16 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
     __shared__ float s_tcy[W][64];       // tile height (clip units) of the record's view
     __shared__ uint32_t s_mark[W][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int limit = (int)min(*a.record_count, (uint32_t)a.nviews * (uint32_t)a.count);
+    const int limit = (int)min(*a.record_total, (uint32_t)a.nviews * (uint32_t)a.count);
     const int first = (blockIdx.x * W + w) * kBuildBatch;
     if (first >= limit) return;   // wave-uniform; no block barriers below
     const int pos = first + lane;
     int cnt = 0;
     if (pos < limit) {
-        const float4 r0 = a.records[2 * (size_t)pos];
-        const float4 r1 = a.records[2 * (size_t)pos + 1];
+        const size_t slot = a.record_slots[pos];
+        const float4 r0 = a.records[2 * slot];
+        const float4 r1 = a.records[2 * slot + 1];
+        a.depth_keys[pos] = __float_as_uint(r1.z);
         const int view = __float_as_int(r1.w) / a.count;
         const FrameGeom& g = view_of(view, a.v0, a.views).geom;
         const Footprint f = footprint(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, g);
@@ -535,7 +571,7 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
     if (hist)
         for (int i = threadIdx.x; i < 3 * 256; i += kBuildThreads) (&s_hist[0][0])[i] = 0u;
     __syncthreads();
-    const int limit = (int)min(*a.record_count, (uint32_t)a.nviews * (uint32_t)a.count);
+    const int limit = (int)min(*a.record_total, (uint32_t)a.nviews * (uint32_t)a.count);
     const uint32_t cap = (uint32_t)a.capacity;
     auto put = [&](uint32_t p, uint32_t tile, int o) {
         if (p < cap) {
@@ -553,8 +589,9 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
     auto fetch = [&](int f) {
         const int q = f + lane;
         if (f < limit && q < limit) {
-            n_r0 = a.records[2 * (size_t)q];
-            n_r1 = a.records[2 * (size_t)q + 1];
+            const size_t slot = a.record_slots[q];
+            n_r0 = a.records[2 * slot];
+            n_r1 = a.records[2 * slot + 1];
             n_mask = a.record_masks[q];
             n_off = a.record_offsets[q];
             n_cnt = a.record_counts[q];
@@ -673,7 +710,8 @@ static dim3 build_grid(const BuildArgs& a) {
 hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
     if (a.records) {
-        if (!a.record_counts || !a.record_masks || !a.depth_words || a.index_bits < 1 || a.index_bits > 31 ||
+        if (!a.record_slots || !a.record_total || !a.record_counts || !a.record_masks || !a.depth_words ||
+            !a.depth_keys || a.index_bits < 1 || a.index_bits > 31 ||
             ((int64_t)1 << a.index_bits) < (int64_t)a.nviews * a.count)
             return hipErrorInvalidValue;
         hipLaunchKernelGGL(record_count_kernel, build_grid(a), dim3(kBuildThreads), 0, s, a);
@@ -686,7 +724,8 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
     if (a.records) {
-        if (!a.record_counts || !a.record_masks || !a.record_offsets || !a.entries || a.tile_passes > 3)
+        if (!a.record_slots || !a.record_total || !a.record_counts || !a.record_masks || !a.record_offsets ||
+            !a.entries || a.tile_passes > 3)
             return hipErrorInvalidValue;
         constexpr int W = kBuildThreads / 64;
         const unsigned blocks = (unsigned)std::max(1, std::min((a.batches + W - 1) / W, kEmitBlocks));
@@ -702,36 +741,137 @@ static unsigned grid_for(int64_t n) {
     return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
 }
 
-// Compaction of the visible Gaussians (non-empty tile rect) of every view into
-// build records and depth words, in (view, index) order (the count pass adds
-// the depth sort's values):
-// one wave per 64-Gaussian batch, position = scanned batch offset + rank in
-// the ballot.
-__global__ __launch_bounds__(256) void compact_depth_kernel(CompactArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int v = blockIdx.y;
-    if (b >= a.batches) return;
-    const size_t slot = (size_t)v * a.batches + b;
-    const uint64_t bits = a.vis_bits[slot];
-    if (!((bits >> lane) & 1ull)) return;
-    const int i = b * kBuildBatch + lane;
-    const size_t o = (size_t)v * a.count + i;
-    const uint32_t p = a.vis_offsets[slot] + (uint32_t)__popcll(bits & lanemask_lt(lane));
-    const float2 c = a.clip_xy[o];
-    const float4 e = a.ellipse[o];
-    const uint32_t dw = depth_word(a.clip_z[o]);
-    a.records[2 * (size_t)p] = make_float4(c.x, c.y, e.x, e.y);
-    a.records[2 * (size_t)p + 1] = make_float4(e.z, e.w, __uint_as_float(dw), __int_as_float(v * a.count + i));
-    a.depth_keys[p] = dw;
+// ---------------------------------------------------------------------------
+// Front end: SH colour + projection + compaction into build records, one pass (FrontArgs).
+// ---------------------------------------------------------------------------
+template <int DEG, typename T>
+__global__ __launch_bounds__(kFrontThreads) void front_kernel(FrontArgs a) {
+    constexpr int NC = 3 * (DEG + 1) * (DEG + 1);
+    constexpr int WV = kFrontThreads / 64;
+    __shared__ uint64_t s_bits[kFrontMaxViews][WV];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int blk = blockIdx.x;
+    const int i = blk * kFrontThreads + t;
+    const bool in = i < a.count;
+    float4 pd = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (in) pd = a.position[i];
+    // colours of every view (the coefficients are dead before the projections)
+    if (in && a.degree > 0) {
+        float c[NC];
+        sh_load<DEG, T>(a.sh, a.sh_count, blk, t, c);
+        float opacity;
+        if constexpr (sizeof(T) == 2) {
+            opacity = h16<T>(a.packed[i].y >> 16);
+        } else {
+            opacity = pd.w;
+        }
+        for (int v = 0; v < a.nviews; ++v)
+            a.color[(size_t)v * a.count + i] = sh_color<DEG>(c, pd, view_of(v, a.v0, a.views).cam.position, opacity);
+    }
+    // projections; the record of view 0 stays in registers
+    float4 ra0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb0 = ra0;
+    {
+        m3 sigma{};
+        if (in) sigma = world_sigma<T>(pd, a.scale_rotation, a.packed, i);
+        for (int v = 0; v < a.nviews; ++v) {
+            const ViewDesc& vd = view_of(v, a.v0, a.views);
+            bool vis = false;
+            if (in) {
+                const size_t o = (size_t)v * a.count + i;
+                const ProjOut po = project_view(pd, sigma, vd);
+                a.clip_xy[o] = make_float2(po.cx, po.cy);
+                a.clip_z[o] = po.cz;
+                if (a.ellipse) a.ellipse[o] = po.ellipse;
+                a.conic[o] = po.conic;
+                vis = has_tiles(po, vd.geom);
+                if (v == 0) {
+                    ra0 = make_float4(po.cx, po.cy, po.ellipse.x, po.ellipse.y);
+                    rb0 = make_float4(po.ellipse.z, po.ellipse.w, __uint_as_float(depth_word(po.cz)),
+                                      __int_as_float(i));
+                }
+            }
+            const uint64_t m = __ballot(vis);
+            if (lane == 0) s_bits[v][w] = m;
+        }
+    }
+    __syncthreads();
+    // the block's records, (view, index) order, into its slice of the staging array
+    float4* slice = a.records + 2 * (size_t)blk * kFrontThreads * a.nviews;
+    uint32_t base = 0u;
+    for (int v = 0; v < a.nviews; ++v) {
+        uint32_t before = 0u, vtot = 0u;
+#pragma unroll
+        for (int k = 0; k < WV; ++k) {
+            const uint32_t c = (uint32_t)__popcll(s_bits[v][k]);
+            if (k < w) before += c;
+            vtot += c;
+        }
+        const uint64_t m = s_bits[v][w];
+        if ((m >> lane) & 1ull) {
+            const uint32_t r = base + before + (uint32_t)__popcll(m & lanemask_lt(lane));
+            float4 ra = ra0, rb = rb0;
+            if (v > 0) {
+                // written by this lane in the loop above
+                const size_t o = (size_t)v * a.count + i;
+                const float2 xy = a.clip_xy[o];
+                const float4 e = a.ellipse[o];
+                ra = make_float4(xy.x, xy.y, e.x, e.y);
+                rb = make_float4(e.z, e.w, __uint_as_float(depth_word(a.clip_z[o])), __int_as_float(v * a.count + i));
+            }
+            slice[2 * (size_t)r] = ra;
+            slice[2 * (size_t)r + 1] = rb;
+        }
+        base += vtot;
+    }
+    if (t == 0) a.block_counts[blk] = base;
 }
 
-hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t s) {
-    if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
-    hipLaunchKernelGGL(compact_depth_kernel, dim3((a.batches + 3) / 4, a.nviews), dim3(256), 0, s, a);
+template <typename T>
+static hipError_t launch_front_t(const FrontArgs& a, unsigned blocks, hipStream_t s) {
+    switch (a.degree) {
+        case 0: hipLaunchKernelGGL((front_kernel<0, T>), dim3(blocks), dim3(kFrontThreads), 0, s, a); break;
+        case 1: hipLaunchKernelGGL((front_kernel<1, T>), dim3(blocks), dim3(kFrontThreads), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((front_kernel<2, T>), dim3(blocks), dim3(kFrontThreads), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((front_kernel<3, T>), dim3(blocks), dim3(kFrontThreads), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
+hipError_t launch_front(const FrontArgs& a, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    if (a.nviews < 1 || a.nviews > kFrontMaxViews || !a.records || !a.block_counts ||
+        (a.nviews > 1 && !a.ellipse) || (a.degree > 0 && (!a.sh || !a.color)))
+        return hipErrorInvalidValue;
+    const unsigned blocks = (unsigned)front_blocks(a.count);
+    switch (a.precision) {
+        case 0: return launch_front_t<float>(a, blocks, s);
+        case 1: return launch_front_t<__half>(a, blocks, s);
+        case 2: return launch_front_t<uint16_t>(a, blocks, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// One wave per front-end block: its records' global numbers -> their staging slots.
+__global__ __launch_bounds__(256) void record_slots_kernel(const uint32_t* offsets, const uint32_t* total,
+                                                           int32_t blocks, uint32_t region, uint32_t* slots) {
+    const int b = (int)((blockIdx.x * 256u + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+    if (b >= blocks) return;
+    const uint32_t o = offsets[b];
+    const uint32_t e = b + 1 < blocks ? offsets[b + 1] : *total;
+    const uint32_t slot = (uint32_t)b * region;
+    for (uint32_t r = (uint32_t)lane; o + r < e; r += 64u) slots[o + r] = slot + r;
+}
+
+hipError_t launch_record_slots(const uint32_t* offsets, const uint32_t* total, int32_t blocks, int32_t region,
+                               uint32_t* slots, hipStream_t s) {
+    if (blocks <= 0) return hipSuccess;
+    if (!offsets || !total || !slots || region <= 0 || (int64_t)blocks * region > (int64_t)UINT32_MAX)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(record_slots_kernel, dim3((unsigned)((blocks + 3) / 4)), dim3(256), 0, s, offsets, total,
+                       blocks, (uint32_t)region, slots);
+    return hipGetLastError();
+}
 
 // Exclusive scan of the per-batch counts, in place, in three launches
 // (reduce per 2048-element block, scan of the block sums, local scan + block
@@ -920,28 +1060,45 @@ hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t s) 
     return hipGetLastError();
 }
 
-__device__ __forceinline__ uint32_t range_tile(const RangeArgs& a, int i) {
-    return (uint32_t)(a.keys[i] >> a.tile_shift);
-}
+// Each lane takes 4 consecutive entries (two 16-byte loads) and the entry before them.
+constexpr int kRangeItems = 4;
 
 __global__ __launch_bounds__(256) void ranges_kernel(RangeArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (a.size_dev && *a.size_dev > (uint32_t)a.size) return;   // the list did not fit: nothing to range
-    const int size = a.size_dev ? (int)*a.size_dev : a.size;
-    if (size <= 0) return;
+    const int64_t size = a.size_dev ? (int64_t)*a.size_dev : (int64_t)a.size;
+    const int64_t i0 = kRangeItems * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (i0 >= size) return;
     // tile ids at or past `tiles` never occur in a well-formed list; they are ignored, not written
     const uint32_t nt = (uint32_t)a.tiles;
-    if (i == 0) {
-        const uint32_t first = range_tile(a, 0);
-        if (first < nt) a.tile_range[(size_t)first * 2] = 0;
-        const uint32_t last = range_tile(a, size - 1);
-        if (last < nt) a.tile_range[(size_t)last * 2 + 1] = size;
-    } else if (i < size) {
-        const uint32_t prev = range_tile(a, i - 1);
-        const uint32_t cur = range_tile(a, i);
-        if (cur != prev) {
-            if (prev < nt) a.tile_range[(size_t)prev * 2 + 1] = i;
-            if (cur < nt) a.tile_range[(size_t)cur * 2] = i;
+    const int sh = a.tile_shift;
+    uint64_t e[kRangeItems];
+    // 16-byte loads when the list is 16-byte aligned (the library's lists are; a caller's list of
+    // the stage API need only be 8-byte aligned)
+    if (i0 + kRangeItems <= size && ((uintptr_t)a.keys & 15u) == 0) {
+        const ulonglong2* p = reinterpret_cast<const ulonglong2*>(a.keys + i0);
+        const ulonglong2 u = p[0], v = p[1];
+        e[0] = u.x;
+        e[1] = u.y;
+        e[2] = v.x;
+        e[3] = v.y;
+    } else {
+#pragma unroll
+        for (int j = 0; j < kRangeItems; ++j) e[j] = i0 + j < size ? a.keys[i0 + j] : 0ull;
+    }
+    uint32_t prev = i0 > 0 ? (uint32_t)(a.keys[i0 - 1] >> sh) : 0u;
+#pragma unroll
+    for (int j = 0; j < kRangeItems; ++j) {
+        const int64_t i = i0 + j;
+        if (i < size) {
+            const uint32_t cur = (uint32_t)(e[j] >> sh);
+            if (i == 0) {
+                if (cur < nt) a.tile_range[(size_t)cur * 2] = 0;
+            } else if (cur != prev) {
+                if (prev < nt) a.tile_range[(size_t)prev * 2 + 1] = (int32_t)i;
+                if (cur < nt) a.tile_range[(size_t)cur * 2] = (int32_t)i;
+            }
+            if (i == size - 1 && cur < nt) a.tile_range[(size_t)cur * 2 + 1] = (int32_t)size;
+            prev = cur;
         }
     }
 }
@@ -949,7 +1106,8 @@ __global__ __launch_bounds__(256) void ranges_kernel(RangeArgs a) {
 hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
     if (a.size <= 0) return hipSuccess;
     if (!a.keys || (a.tile_shift != 0 && a.tile_shift != 32)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(ranges_kernel, dim3((a.size + 255) / 256), dim3(256), 0, s, a);
+    const int64_t lanes = ((int64_t)a.size + kRangeItems - 1) / kRangeItems;
+    hipLaunchKernelGGL(ranges_kernel, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -83,14 +83,41 @@ def percentile(x, q):
     return float(np.percentile(np.asarray(x, np.float64), q)) if len(x) else float("nan")
 
 
-def run_load(server: capi.Server, scene_ids, requests, rate: float | None = None, arrivals=None, seed: int = 3):
+class BufferPool:
+    """Pre-faulted host buffers for served images, handed out per request and returned when the
+    request completes (a client that reuses its buffers, as the sequential baseline does)."""
+
+    def __init__(self, count: int, max_bytes: int):
+        self._free = [np.ones(max_bytes, np.uint8) for _ in range(count)]
+        self._cv = threading.Condition()
+
+    def get(self, shape):
+        with self._cv:
+            while not self._free:
+                self._cv.wait()
+            buf = self._free.pop()
+        return buf[: int(np.prod(shape))].reshape(shape)
+
+    def put(self, view):
+        base = view.base if view.base is not None else view
+        while base.base is not None:
+            base = base.base
+        with self._cv:
+            self._free.append(base)
+            self._cv.notify()
+
+
+def run_load(server: capi.Server, scene_ids, requests, rate: float | None = None, arrivals=None, seed: int = 3,
+             pool: BufferPool | None = None):
     """Submits `requests` with Poisson arrivals at `rate` req/s (None: all at once, or the
-    explicit `arrivals` offsets in seconds) and waits for every completion."""
+    explicit `arrivals` offsets in seconds) and waits for every completion.  With a pool, each
+    request's image goes into a pre-faulted pooled buffer (submission waits for a free one, so
+    at most the pool's size is in flight); without one, into a fresh buffer per request."""
     rng = np.random.default_rng(seed)
     n = len(requests)
     if arrivals is None:
         arrivals = np.cumsum(rng.exponential(1.0 / rate, n)) if rate else np.zeros(n)
-    outs = [np.empty((H, W, 4), np.uint8) for (_, _, W, H) in requests]
+    outs = [None] * n if pool is not None else [np.empty((H, W, 4), np.uint8) for (_, _, W, H) in requests]
     ids = []
     done = {}
     stop = threading.Event()
@@ -99,6 +126,8 @@ def run_load(server: capi.Server, scene_ids, requests, rate: float | None = None
         while not stop.is_set() or len(done) < len(ids):
             for d, _ in server.poll(512, 20):
                 done[d["request_id"]] = d
+                if pool is not None:
+                    pool.put(outs[d["user_tag"]])
             if stop.is_set() and len(done) >= len(ids):
                 break
 
@@ -109,6 +138,8 @@ def run_load(server: capi.Server, scene_ids, requests, rate: float | None = None
         delay = t0 + float(arrivals[k]) - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
+        if pool is not None:
+            outs[k] = pool.get((H, W, 4))
         ids.append(server.submit(scene_ids[i], cam, W, H, outs[k], tag=k))
     stop.set()
     deadline = time.perf_counter() + 600
@@ -129,7 +160,7 @@ def run_load(server: capi.Server, scene_ids, requests, rate: float | None = None
         "p50_ms": percentile(lat, 50),
         "p99_ms": percentile(lat, 99),
         "mean_batch_views": float(np.mean([c["batch_views"] for c in comps])),
-        "images": outs,
+        "images": outs if pool is None else None,
     }
 
 

@@ -125,7 +125,9 @@ typedef struct gr_double_buffer_i32 {
     int32_t selector;
 } gr_double_buffer_i32;
 
-/* core/types.h:26-33 (RenderStats), milliseconds; total_ms is end to end. */
+/* core/types.h:26-33 (RenderStats), milliseconds; total_ms is end to end.
+ * The renderer computes SH colour, projection and the compaction of the
+ * visible Gaussians in one pass: sh_eval_ms is that pass and project_ms is 0. */
 typedef struct gr_render_stats {
     double sh_eval_ms;
     double project_ms;
@@ -322,6 +324,8 @@ GR_API gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* camera
                                     int32_t height, uint8_t* out_rgba8, int32_t out_is_device,
                                     gr_render_stats* stats);
 /* Device pointers of the last frame's intermediate buffers (for tests).
+ * screen_ellipse is NULL: the renderer's front end keeps each visible
+ * Gaussian's ellipse only in its build record.
  * sorted_keys/sorted_values is the frame's sorted tile list in the form of
  * the reference's sortTileList output (core/gaussian_render.cu:591-632):
  * (tile << 32 | depth word) keys.  The renderer keeps the list as tile ids
@@ -363,8 +367,10 @@ GR_API gr_status gr_renderer_set_timing(gr_renderer* r, int32_t level);
  * the frame did not time them), tile_sort_passes their number and
  * tile_list_size the pairs each pass moves.  At GR_TIMING_KERNELS,
  * kernel_ms[i] is the time of the work named kernel_name[i] (in frame order:
- * "sh", "project", "compact", "record_count", "depth_sort", "offsets",
- * "emit", "tile_sort_scan", "tile_sort_passes", "ranges", "raster") and
+ * "front" (SH colour, projection and block-local compaction in one pass),
+ * "numbering" (record numbers and staging slots), "record_count",
+ * "depth_sort", "offsets", "emit", "tile_sort_scan", "tile_sort_passes",
+ * "ranges", "raster") and
  * visible the number of Gaussians with a non-empty tile rect (else -1). */
 #define GR_KERNEL_TIMES_MAX 24
 typedef struct gr_kernel_times {
@@ -401,7 +407,7 @@ typedef struct gr_server gr_server;
 typedef struct gr_server_config {
     int32_t device;               /* -1: current device */
     int32_t slots;                /* batches in flight (>= 1) */
-    int32_t max_views_per_batch;  /* >= 1 */
+    int32_t max_views_per_batch;  /* >= 1; a batch holds at most 64 views */
     int32_t use_graphs;           /* capture + replay repeated stage sequences with hipGraph */
     int64_t hbm_budget_bytes;     /* resident scenes + workspaces; 0 = no limit */
 } gr_server_config;

@@ -226,34 +226,50 @@ hipError_t launch_project(const ProjectArgs& a, hipStream_t s) {
         touch(a.clip_z, n * 4);
         touch(a.ellipse, n * 16);
         touch(a.conic, n * 16);
-        if (a.vis_bits) {   // every Gaussian visible
-            for (int v = 0; v < a.nviews; ++v)
-                for (int b = 0; b < a.batches; ++b) {
-                    const int here = std::min(64, a.count - 64 * b);
-                    a.vis_bits[(size_t)v * a.batches + b] = here == 64 ? ~0ull : ((1ull << here) - 1ull);
-                    a.vis_counts[(size_t)v * a.batches + b] = (uint32_t)here;
-                }
+    });
+    return hipSuccess;
+}
+
+// Front end: every Gaussian of every view visible; each block's records in (view, index) order
+// in its own staging slice.
+hipError_t launch_front(const FrontArgs& a, hipStream_t s) {
+    if (a.nviews < 1 || a.nviews > kFrontMaxViews || !a.records || !a.block_counts ||
+        (a.nviews > 1 && !a.ellipse) || (a.degree > 0 && (!a.sh || !a.color)))
+        return hipErrorInvalidValue;
+    run_or_record(s, [=] {
+        const size_t n = (size_t)a.count, vn = (size_t)a.nviews * n;
+        rd(a.position, n * 16);
+        if (a.precision) rd(a.packed, n * 8); else rd(a.scale_rotation, n * 16);
+        if (a.degree > 0) {
+            rd(a.sh, ((n + 255) / 256) * 256 * (size_t)a.sh_count * (a.precision ? 2 : 4));
+            touch(a.color, vn * 16);
+        }
+        touch(a.clip_xy, vn * 8);
+        touch(a.clip_z, vn * 4);
+        if (a.ellipse) touch(a.ellipse, vn * 16);
+        touch(a.conic, vn * 16);
+        const int nb = front_blocks(a.count);
+        for (int blk = 0; blk < nb; ++blk) {
+            const uint32_t here = (uint32_t)std::min(kFrontThreads, a.count - kFrontThreads * blk);
+            const uint32_t c = here * (uint32_t)a.nviews;
+            touch(a.records + 2 * (size_t)blk * kFrontThreads * a.nviews, (size_t)c * 32);
+            a.block_counts[blk] = c;
         }
     });
     return hipSuccess;
 }
 
-hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t s) {
+hipError_t launch_record_slots(const uint32_t* offsets, const uint32_t* total, int32_t blocks, int32_t region,
+                               uint32_t* slots, hipStream_t s) {
+    if (blocks <= 0) return hipSuccess;
+    if (!offsets || !total || !slots || region <= 0 || (int64_t)blocks * region > (int64_t)UINT32_MAX)
+        return hipErrorInvalidValue;
     run_or_record(s, [=] {
-        for (int v = 0; v < a.nviews; ++v)
-            for (int b = 0; b < a.batches; ++b) {
-                const size_t slot = (size_t)v * a.batches + b;
-                uint32_t p = a.vis_offsets[slot];
-                for (int l = 0; l < 64; ++l)
-                    if ((a.vis_bits[slot] >> l) & 1ull) {
-                        rd(&a.clip_z[(size_t)v * a.count + 64 * b + l], 4);
-                        rd(&a.clip_xy[(size_t)v * a.count + 64 * b + l], 8);
-                        rd(&a.ellipse[(size_t)v * a.count + 64 * b + l], 16);
-                        touch(&a.records[2 * (size_t)p], 32);
-                        a.depth_keys[p] = 7u;
-                        ++p;
-                    }
-            }
+        for (int b = 0; b < blocks; ++b) {
+            const uint32_t o = offsets[b], e = b + 1 < blocks ? offsets[b + 1] : *total;
+            if (e < o || e - o > (uint32_t)region) std::abort();   // a block holds at most `region` records
+            for (uint32_t r = 0; o + r < e; ++r) slots[o + r] = (uint32_t)b * (uint32_t)region + r;
+        }
     });
     return hipSuccess;
 }
@@ -262,11 +278,12 @@ hipError_t launch_compact_depth(const CompactArgs& a, hipStream_t s) {
 hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
     run_or_record(s, [=] {
         if (a.records) {
-            const uint32_t nvis = std::min<uint32_t>(*a.record_count, (uint32_t)(a.nviews * a.count));
-            rd(a.records, (size_t)nvis * 32);
+            const uint32_t nvis = std::min<uint32_t>(*a.record_total, (uint32_t)(a.nviews * a.count));
             if (a.index_bits < 1 || a.index_bits > 31 || ((int64_t)1 << a.index_bits) < (int64_t)a.nviews * a.count)
                 std::abort();
             for (uint32_t q = 0; q < nvis; ++q) {
+                rd(&a.records[2 * (size_t)a.record_slots[q]], 32);
+                a.depth_keys[q] = 7u;
                 a.record_counts[q] = 3u;
                 a.record_masks[q] = 0xbull;   // 4 candidates, 3 of them pass
                 a.depth_words[q] = q | (3u << a.index_bits);
@@ -330,9 +347,10 @@ hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     run_or_record(s, [=] {
         uint32_t K;
         if (a.records) {
-            const uint32_t nvis = std::min<uint32_t>(*a.record_count, (uint32_t)(a.nviews * a.count));
+            const uint32_t nvis = std::min<uint32_t>(*a.record_total, (uint32_t)(a.nviews * a.count));
             K = 0;
             for (uint32_t q = 0; q < nvis; ++q) {
+                rd(&a.records[2 * (size_t)a.record_slots[q]], 32);
                 if (a.record_offsets[q] > 3u * nvis) std::abort();   // offsets lie inside the list
                 if (__builtin_popcountll(a.record_masks[q]) != (int)a.record_counts[q]) std::abort();
                 K += a.record_counts[q];

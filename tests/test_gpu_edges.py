@@ -459,7 +459,7 @@ def test_timing_levels_render_the_same_frame(torch_cuda):
             assert kt["tile_sort_pass_ms"] > 0
         if level == "kernels":
             names = list(kt["kernel_ms"])
-            for want in ("sh", "project", "record_count", "emit", "tile_sort_passes", "raster"):
+            for want in ("front", "numbering", "record_count", "depth_sort", "emit", "tile_sort_passes", "raster"):
                 assert want in names, names
             assert all(v >= 0 for v in kt["kernel_ms"].values())
             assert 0 < kt["visible"] <= s["count"]
