@@ -145,8 +145,9 @@ struct BuildArgs {
     uint32_t* depth_words;       // count pass: record index | entry count << index_bits (depth sort values)
     int32_t index_bits;          // packed_index_bits(nviews * count)
     const uint32_t* record_offsets; // emit pass: output offset per record
-    uint32_t* tile_hist;         // record emit: 256-bin histograms of the tile's 8-bit digits (NULL: none)
+    uint32_t* tile_hist;         // record emit: 256-bin histograms of the tile's digits (NULL: none)
     int32_t tile_passes;         // digits counted into tile_hist (<= 3)
+    int32_t tile_digit_bits;     // their width: sort_digit_bits(0, tile bits)
     int32_t count;
     int32_t capacity;
     int32_t batches;             // ceil(count / 64) per view (index-order walk)
@@ -218,6 +219,10 @@ hipError_t launch_raster(const RasterArgs& a, hipStream_t stream);
 
 // radix sort (gr_sort.hip)
 size_t radix_sort_temp_bytes(int32_t count, int32_t begin_bit, int32_t end_bit);
+// Digit width of a sort over bits [begin_bit, end_bit): ceil((end - begin) / 8) passes of at most
+// 8 bits, as even as possible (13 tile bits: 7 + 6); pass p sorts bits
+// [begin + p * w, min(begin + (p + 1) * w, end)).
+int sort_digit_bits(int32_t begin_bit, int32_t end_bit);
 // Sorts (keys[sel], values[sel]) on bits [begin_bit, end_bit); on return *sel
 // names the buffer pair that holds the sorted data.  The number of pairs is
 // min(*count_dev, count) when count_dev is not NULL (a device-side count, so
@@ -232,8 +237,8 @@ constexpr int kSortHistCopies = 16;
 constexpr int kSortHistCopyStride = 8 * 256;
 // Two-step form of the keys-only 64-bit sort for a producer that accumulates
 // the digit histograms itself: radix_sort_prepare zeroes the histograms
-// (passes x 256 uint32 at *hist, pass p counting digit
-// (key >> (begin_bit + 8p)) & 255, in each of the kSortHistCopies copies), the partition tickets and the first
+// (passes x 256 uint32 at *hist, pass p counting digit (key >> (begin_bit + w p)) & (2^w - 1),
+// w = sort_digit_bits(begin_bit, end_bit), in each of the kSortHistCopies copies), the partition tickets and the first
 // look-back region of `temp` sized for `count`; after the producer has added
 // exactly the keys the sort will see, radix_sort_keys_prepared scans them and
 // runs the passes.  A device-side count above `count` then means the producer

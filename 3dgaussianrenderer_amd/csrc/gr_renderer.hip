@@ -379,6 +379,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     const int tile_bits = tile_bits_for(tiles);
     R_HIP(radix_sort_prepare(capacity, r->sort_temp, r->sort_temp_bytes, &b.tile_hist, stream));
     b.tile_passes = (tile_bits + 7) / 8;
+    b.tile_digit_bits = sort_digit_bits(0, tile_bits);
     R_HIP(launch_build_emit(b, stream));
     R_HIP(mark("emit"));
     // tile bits: the sort records its begin event after its histogram scan and its end event

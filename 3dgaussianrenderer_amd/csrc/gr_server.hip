@@ -554,6 +554,7 @@ gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
     BuildArgs b = record_build_args(sl, sc);
     S_HIP(radix_sort_prepare((int32_t)sl.capacity, w.sort_temp.p, w.sort_temp.bytes, &b.tile_hist, sl.stream));
     b.tile_passes = (sl.tile_bits + 7) / 8;
+    b.tile_digit_bits = sort_digit_bits(0, sl.tile_bits);
     S_HIP(launch_build_emit(b, sl.stream));
     uint64_t* kb[2] = {w.list0.as<uint64_t>(), w.list1.as<uint64_t>()};
     int32_t sel = 0;

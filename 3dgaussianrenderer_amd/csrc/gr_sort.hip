@@ -23,6 +23,9 @@
 // count, so one sort handles fewer than 2^30 pairs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <mutex>
+
 #include "gr_kernels.hpp"
 
 namespace gr {
@@ -145,6 +148,7 @@ __global__ __launch_bounds__(kHistThreads) void sort_hist_kernel(const KeyT* __r
                                                                  int end_bit, int passes, uint32_t* hist) {
     constexpr int kP = (int)sizeof(KeyT);   // at most one 8-bit pass per key byte
     __shared__ uint32_t s_h[kP][kRadix];
+    const int dw = (end_bit - begin_bit + passes - 1) / passes;   // the passes' digit width
     if (count_dev) n = (int32_t)min((uint32_t)n, *count_dev);
     __shared__ KeyT s_k[kHistChunk + kHistChunk / kHistSeg];
     for (int i = threadIdx.x; i < kP * kRadix; i += blockDim.x) (&s_h[0][0])[i] = 0u;
@@ -169,8 +173,8 @@ __global__ __launch_bounds__(kHistThreads) void sort_hist_kernel(const KeyT* __r
 #pragma unroll
             for (int p = 0; p < kP; ++p) {
                 if (p < passes) {
-                    const int shift = begin_bit + 8 * p;
-                    const int bits = min(8, end_bit - shift);
+                    const int shift = begin_bit + dw * p;
+                    const int bits = min(dw, end_bit - shift);
                     const uint32_t d = (uint32_t)(k >> shift) & ((1u << bits) - 1u);
                     if (d != cur[p]) {
                         if (cnt[p]) atomicAdd(&s_h[p][cur[p]], cnt[p]);
@@ -224,15 +228,18 @@ __device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Publishes partition `part`'s count of digit t and returns in `excl` the count of that digit
-// in all earlier partitions (decoupled look-back over kLookback predecessors per step).
-__device__ __forceinline__ void lookback(uint32_t* status, int part, int t, uint32_t total, uint32_t& excl) {
-    uint32_t* my = status + (size_t)part * kRadix + t;
-    if (part == 0) {
-        st_status(my, kFlagIncl | total);
-        return;
-    }
-    st_status(my, kFlagAgg | total);
+// Publishes partition `part`'s count of digit t: an aggregate, or for partition 0 already the
+// inclusive prefix.
+__device__ __forceinline__ void lb_publish(uint32_t* status, int part, int t, uint32_t total) {
+    st_status(status + (size_t)part * kRadix + t, (part == 0 ? kFlagIncl : kFlagAgg) | total);
+}
+
+// Returns the count of digit t in all partitions before `part` (decoupled look-back over
+// kLookback predecessors per step: aggregates are summed until an inclusive prefix is found) and
+// publishes part's inclusive prefix.  Every wait is on an earlier partition.
+__device__ __forceinline__ uint32_t lb_resolve(uint32_t* status, int part, int t, uint32_t total) {
+    if (part == 0) return 0u;
+    uint32_t excl = 0u;
     int j = part - 1;
     bool done = false;
     while (!done) {
@@ -252,17 +259,93 @@ __device__ __forceinline__ void lookback(uint32_t* status, int part, int t, uint
         }
         if (!done) j -= q;
     }
-    st_status(my, kFlagIncl | (excl + total));
+    st_status(status + (size_t)part * kRadix + t, kFlagIncl | (excl + total));
+    return excl;
 }
 
-// PAIRS = false: keys only (vin/vout unused), e.g. tile-list entries that carry
-// their payload in the key bits above the sorted ones.
-// LDS of a pass: the per-wave digit counters are dead once every pair's
-// position in the partition is known, so the reordered partition overlays them.
+// Stable rank of a wave's keys among the wave's earlier keys of the same digit, item row by item
+// row: every lane ORs its bit into its digit's 64-bit lane mask in LDS (OR commutes, so the mask
+// does not depend on the order the LDS applies the lanes), reads the mask back -- its peers --
+// and the row's first lane of a digit adds the row's count to the digit's counter and clears the
+// mask.  rank[i] = wave-local rank; counters and masks start at zero and the counters end as the
+// wave's digit counts.  (Against 64-bit ballots per digit bit: fewer VALU operations per key and
+// a shorter ranking, which shortens the look-back chain.)
+template <typename KeyT, int IT, int RW>
+__device__ __forceinline__ void wave_rank(const KeyT (&key)[IT], uint32_t (&rank)[IT], int64_t wb, int32_t n,
+                                          int shift, uint32_t dmask, uint32_t* wcnt, uint64_t* mask, int lane) {
+    const uint64_t lt = (1ull << lane) - 1ull, me = 1ull << lane;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const bool valid = wb + i * 64 < n;
+        const uint32_t d = (uint32_t)(key[i] >> shift) & dmask;
+        if (valid) __hip_atomic_fetch_or(&mask[d], me, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t peers =
+            valid ? __hip_atomic_load(&mask[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0ull;
+        const uint32_t below = (uint32_t)__popcll(peers & lt);
+        const uint32_t prev = wcnt[d];
+        rank[i] = prev + below;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0u) {
+            wcnt[d] = prev + (uint32_t)__popcll(peers);
+            __hip_atomic_store(&mask[d], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Digit counts of a ranked partition (threads < RW own one digit each; wcnt[k][d] becomes the
+// exclusive prefix over the waves), an exclusive scan of them over the digits into s_dstart (all
+// kRadix entries; digits without keys are 0), and the count published for the look-back.  Needs
+// a barrier before (the ranking) and has two inside; returns the digit's total and start.
+template <int WV, int RW>
+__device__ __forceinline__ void partition_digits(uint32_t (*wcnt)[RW], uint32_t* s_dstart, uint32_t* s_wsum,
+                                                 uint32_t* status, uint32_t* status_next, int part, int nd, int t,
+                                                 uint32_t& total, uint32_t& dstart) {
+    const int lane = t & 63, w = t >> 6;
+    total = 0u;
+    if (t < kRadix) {
+        if (t < RW) {
+#pragma unroll
+            for (int k = 0; k < WV; ++k) {
+                const uint32_t c = wcnt[k][t];
+                wcnt[k][t] = total;
+                total += c;
+            }
+        }
+        if (t < nd) lb_publish(status, part, t, total);
+        if (status_next) status_next[(size_t)part * kRadix + t] = 0u;
+        uint32_t incl = total;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += o;
+        }
+        if (lane == 63) s_wsum[w] = incl;
+        s_dstart[t] = incl - total;    // within-wave part, completed below
+    }
+    __syncthreads();
+    if (t < kRadix) {
+        uint32_t woff = 0;
+        for (int k = 0; k < w; ++k) woff += s_wsum[k];
+        dstart = s_dstart[t] + woff;
+        s_dstart[t] = dstart;
+    }
+    __syncthreads();
+}
+
+// Pairs passes (the depth sort, the stage-level pair sort): one partition per workgroup.  The
+// partition's counts are published right after the ranking, and its look-back is resolved after
+// the reorder in LDS, so the wait overlaps the local work.  The per-wave digit counters and lane
+// masks are dead once every pair's position in the partition is known, so the reordered
+// partition overlays them.
 template <typename KeyT, bool PAIRS, class S>
 struct SortLds {
     static constexpr size_t kPairBytes = (size_t)S::kTile * (sizeof(KeyT) + (PAIRS ? sizeof(int32_t) : 0));
-    static constexpr size_t kCountBytes = (size_t)S::kWaves * kRadix * sizeof(uint32_t);
+    static constexpr size_t kCountBytes = (size_t)S::kWaves * kRadix * (sizeof(uint32_t) + sizeof(uint64_t));
     static constexpr size_t kBytes = kPairBytes > kCountBytes ? kPairBytes : kCountBytes;
 };
 
@@ -281,13 +364,17 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::
     __shared__ uint32_t s_gbase[kRadix];
     __shared__ uint32_t s_wsum[kRadix / 64];
     __shared__ __align__(16) unsigned char s_raw[SortLds<KeyT, PAIRS, S>::kBytes];
-    auto s_wcnt = reinterpret_cast<uint32_t(*)[kRadix]>(s_raw);       // [waves][digit], ranking phase
-    KeyT* s_keys = reinterpret_cast<KeyT*>(s_raw);                      // reorder phase
+    auto s_wcnt = reinterpret_cast<uint32_t(*)[kRadix]>(s_raw);                        // ranking phase
+    auto s_mask = reinterpret_cast<uint64_t(*)[kRadix]>(s_raw + (size_t)kSortWaves * kRadix * 4);
+    KeyT* s_keys = reinterpret_cast<KeyT*>(s_raw);                                       // reorder phase
     int32_t* s_vals = reinterpret_cast<int32_t*>(s_raw + (size_t)kSortTile * sizeof(KeyT));
 
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     if (t == 0) s_part = atomicAdd(ticket, 1u);
-    for (int i = t; i < kSortWaves * kRadix; i += kSortThreads) (&s_wcnt[0][0])[i] = 0u;
+    for (int i = t; i < kSortWaves * kRadix; i += kSortThreads) {
+        (&s_wcnt[0][0])[i] = 0u;
+        (&s_mask[0][0])[i] = 0ull;
+    }
     __syncthreads();
     const int part = (int)s_part;
     if (count_dev) {
@@ -297,11 +384,11 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::
         n = (int32_t)min((uint32_t)n, *count_dev);
     }
     if ((int64_t)part * kSortTile >= n) return;   // beyond the device-side count: whole workgroup
-    if (status_next && t < kRadix) status_next[(size_t)part * kRadix + t] = 0u;
 
     const int64_t base = (int64_t)part * kSortTile;
     if (*trivial) {
         // every key has the same digit: the pass is the identity, the partition is copied through
+        if (status_next && t < kRadix) status_next[(size_t)part * kRadix + t] = 0u;
         const int64_t end = min((int64_t)n, base + kSortTile);
         for (int64_t i = base + t; i < end; i += kSortThreads) {
             kout[i] = kin[i];
@@ -311,6 +398,7 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::
     }
     const int64_t wbase = base + (int64_t)w * (64 * kSortItems) + lane;
     const uint32_t dmask = (1u << bits) - 1u;
+    const int nd = 1 << bits;
     KeyT key[kSortItems];
     int32_t val[kSortItems];
     uint32_t rank[kSortItems];
@@ -320,56 +408,10 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::
         key[i] = idx < n ? kin[idx] : (KeyT)0;
         if constexpr (PAIRS) val[i] = idx < n ? vin[idx] : 0;
     }
-    // per-wave stable ranking: peers = lanes with the same digit (8 ballots)
-    const uint64_t lt = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int i = 0; i < kSortItems; ++i) {
-        const bool valid = wbase + i * 64 < n;
-        const uint32_t d = (uint32_t)(key[i] >> shift) & dmask;
-        uint64_t peers = __ballot(valid);
-        for (int b = 0; b < bits; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bb = __ballot(bit);
-            peers &= bit ? bb : ~bb;
-        }
-        const uint32_t below = (uint32_t)__popcll(peers & lt);
-        const uint32_t prev = s_wcnt[w][d];
-        rank[i] = prev + below;
-        if (valid && below == 0u) s_wcnt[w][d] = prev + (uint32_t)__popcll(peers);
-    }
+    wave_rank<KeyT, kSortItems, kRadix>(key, rank, wbase, n, shift, dmask, s_wcnt[w], s_mask[w], lane);
     __syncthreads();
-
-    // threads 0..255 own one digit each: wave prefix, partition total, look-back
-    uint32_t total = 0, excl = 0;
-    if (t < kRadix) {
-#pragma unroll
-        for (int k = 0; k < kSortWaves; ++k) {
-            const uint32_t c = s_wcnt[k][t];
-            s_wcnt[k][t] = total;
-            total += c;
-        }
-        lookback(status, part, t, total, excl);
-    }
-    // exclusive scan of the partition totals over the 256 digits (waves 0..3)
-    if (t < kRadix) {
-        uint32_t incl = total;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += o;
-        }
-        if (lane == 63) s_wsum[w] = incl;
-        s_dstart[t] = incl - total;    // within-wave part, completed below
-    }
-    __syncthreads();
-    if (t < kRadix) {
-        uint32_t woff = 0;
-        for (int k = 0; k < w; ++k) woff += s_wsum[k];
-        const uint32_t dstart = s_dstart[t] + woff;
-        s_dstart[t] = dstart;
-        s_gbase[t] = digit_off[t] + excl - dstart;
-    }
-    __syncthreads();
+    uint32_t total = 0u, dstart = 0u;
+    partition_digits<kSortWaves, kRadix>(s_wcnt, s_dstart, s_wsum, status, status_next, part, nd, t, total, dstart);
 
     // positions in the partition, then (counters dead) the reorder over the same LDS
 #pragma unroll
@@ -385,6 +427,7 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::
             if constexpr (PAIRS) s_vals[rank[i]] = val[i];
         }
     }
+    if (t < nd) s_gbase[t] = digit_off[t] + lb_resolve(status, part, t, total) - dstart;
     __syncthreads();
     const int nvalid = (int)min((int64_t)kSortTile, (int64_t)n - base);
     for (int j = t; j < nvalid; j += kSortThreads) {
@@ -396,7 +439,165 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::
     }
 }
 
+// Keys-only passes (tile lists): persistent workgroups that hold two partitions each.  A
+// workgroup takes partition Y while X -- ranked, counts published, reordered in LDS -- still
+// waits for its look-back; the waves that own X's digits resolve X's look-back while the other
+// waves rank their rows of Y, then rank theirs; Y's counts are published, X is stored and Y is
+// reordered into the LDS X held.  Progress: every look-back waits on earlier partitions only, and
+// the owner of the earliest unresolved partition never waits on an unresolved one, so it
+// finishes it.  The digit tables are RW = 2^RB wide (7-bit digits fit more keys per partition
+// in the LDS that keeps two workgroups on a CU).
+template <int THREADS, int ITEMS, int RB>
+struct KeysShape {
+    static constexpr int kThreads = THREADS;
+    static constexpr int kWaves = THREADS / 64;
+    static constexpr int kItems = ITEMS;
+    static constexpr int kTile = THREADS * ITEMS;
+    static constexpr int kRadixBits = RB;
+    static_assert(THREADS % 64 == 0 && THREADS >= 256, "one lane per digit in the look-back");
+};
+#ifndef GR_KSORT_ITEMS7
+#define GR_KSORT_ITEMS7 15
+#endif
+#ifndef GR_KSORT_ITEMS8
+#define GR_KSORT_ITEMS8 13
+#endif
+using KeysShape7 = KeysShape<512, GR_KSORT_ITEMS7, 7>;
+using KeysShape8 = KeysShape<512, GR_KSORT_ITEMS8, 8>;
+static_assert(KeysShape7::kTile >= kMinTile && KeysShape8::kTile >= kMinTile,
+              "the look-back regions hold count / kMinTile partitions");
+
+template <class S>
+__global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(4))) void onesweep_keys_kernel(
+    const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout, int32_t n, const uint32_t* count_dev, int strict,
+    int shift, int bits, const uint32_t* __restrict__ digit_off, uint32_t* status, uint32_t* status_next,
+    uint32_t* ticket, const uint32_t* trivial) {
+    constexpr int T = S::kThreads, WV = S::kWaves, IT = S::kItems, TILE = S::kTile;
+    constexpr int RW = 1 << S::kRadixBits;
+    __shared__ uint32_t s_part;
+    __shared__ uint32_t s_dstart[kRadix];
+    __shared__ uint32_t s_gbase[kRadix];
+    __shared__ uint32_t s_wsum[kRadix / 64];
+    __shared__ uint32_t s_wcnt[WV][RW];
+    __shared__ uint64_t s_mask[WV][RW];
+    __shared__ uint64_t s_keys[TILE];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (count_dev) {
+        if (strict && *count_dev > (uint32_t)n) return;
+        n = (int32_t)min((uint32_t)n, *count_dev);
+    }
+    const int parts = (int)(((int64_t)n + TILE - 1) / TILE);
+    const uint32_t dmask = (1u << bits) - 1u;
+    const int nd = 1 << bits;
+    auto take = [&]() -> int {
+        if (t == 0) s_part = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const int p = (int)s_part;
+        __syncthreads();
+        return p;
+    };
+    int X = take();
+    if (X >= parts) return;
+    if (*trivial) {
+        // every key has the same digit: the pass is the identity, partitions are copied through
+        for (;;) {
+            const int64_t base = (int64_t)X * TILE, end = min((int64_t)n, base + TILE);
+            if (status_next && t < kRadix) status_next[(size_t)X * kRadix + t] = 0u;
+            for (int64_t i = base + t; i < end; i += T) kout[i] = kin[i];
+            X = take();
+            if (X >= parts) return;
+        }
+    }
+    uint64_t key[IT];
+    uint32_t rank[IT];
+    auto wbase = [&](int P) { return (int64_t)P * TILE + (int64_t)w * (64 * IT) + lane; };
+    auto load = [&](int P) {
+        const int64_t wb = wbase(P);
+#pragma unroll
+        for (int i = 0; i < IT; ++i) key[i] = wb + i * 64 < n ? kin[wb + i * 64] : 0ull;
+    };
+    auto zero_tables = [&]() {
+        for (int i = t; i < WV * RW; i += T) {
+            (&s_wcnt[0][0])[i] = 0u;
+            (&s_mask[0][0])[i] = 0ull;
+        }
+    };
+    // after wave_rank and a barrier: publishes P's counts, leaves the partition-local positions
+    // in rank[]
+    auto finish = [&](int P, uint32_t& total, uint32_t& dstart) {
+        partition_digits<WV, RW>(s_wcnt, s_dstart, s_wsum, status, status_next, P, nd, t, total, dstart);
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const uint32_t d = (uint32_t)(key[i] >> shift) & dmask;
+            rank[i] += s_dstart[d] + s_wcnt[w][d];
+        }
+    };
+    auto reorder = [&](int P) {
+        const int64_t wb = wbase(P);
+#pragma unroll
+        for (int i = 0; i < IT; ++i)
+            if (wb + i * 64 < n) s_keys[rank[i]] = key[i];
+    };
+    uint32_t totX = 0u, dsX = 0u;
+    load(X);
+    zero_tables();
+    __syncthreads();
+    wave_rank<uint64_t, IT, RW>(key, rank, wbase(X), n, shift, dmask, s_wcnt[w], s_mask[w], lane);
+    __syncthreads();
+    finish(X, totX, dsX);
+    reorder(X);
+    for (;;) {
+        const int Y = take();   // its barriers also order the reorder of X before the store
+        const bool haveY = Y < parts;
+        uint32_t totY = 0u, dsY = 0u;
+        if (haveY) {
+            load(Y);
+            zero_tables();
+        }
+        __syncthreads();
+        if (t < nd) s_gbase[t] = digit_off[t] + lb_resolve(status, X, t, totX) - dsX;
+        if (haveY) wave_rank<uint64_t, IT, RW>(key, rank, wbase(Y), n, shift, dmask, s_wcnt[w], s_mask[w], lane);
+        __syncthreads();
+        if (haveY) finish(Y, totY, dsY);
+        const int nvalid = (int)min((int64_t)TILE, (int64_t)n - (int64_t)X * TILE);
+        for (int j = t; j < nvalid; j += T) {
+            const uint64_t k = s_keys[j];
+            kout[s_gbase[(uint32_t)(k >> shift) & dmask] + (uint32_t)j] = k;
+        }
+        if (!haveY) return;
+        __syncthreads();
+        reorder(Y);
+        X = Y;
+        totX = totY;
+        dsX = dsY;
+    }
+}
+
+// Resident workgroups of the keys kernel on the current device (the persistent grid).
+template <class S>
+static int keys_grid() {
+    static std::mutex mu;
+    static int grid[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!grid[dev]) {
+        int cus = 0, per = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)onesweep_keys_kernel<S>, S::kThreads, 0) !=
+                hipSuccess || per <= 0)
+            per = 1;
+        grid[dev] = cus * per;
+    }
+    return grid[dev];
+}
+
 static int sort_passes(int32_t begin_bit, int32_t end_bit) { return (end_bit - begin_bit + 7) / 8; }
+
+int sort_digit_bits(int32_t begin_bit, int32_t end_bit) {
+    const int p = sort_passes(begin_bit, end_bit);
+    return p > 0 ? (end_bit - begin_bit + p - 1) / p : 0;
+}
 
 // Zeroes the histograms, the partition tickets and the first look-back region.
 static hipError_t sort_prepare(const SortLayout& l, char* tb, hipStream_t s) {
@@ -440,20 +641,33 @@ static hipError_t sort_passes_t(KeyT* keys[2], int32_t* values[2], int32_t* sel,
         const hipError_t e = hipEventRecord(ev_begin, s);
         if (e != hipSuccess) return e;
     }
+    const int dw = sort_digit_bits(begin_bit, end_bit);
     for (int p = 0; p < passes; ++p) {
-        const int shift = begin_bit + 8 * p;
-        const int bits = min(8, end_bit - shift);
+        const int shift = begin_bit + dw * p;
+        const int bits = min(dw, end_bit - shift);
         const int cur = *sel;
-        if (values[0])
+        uint32_t* st = status[p & 1];
+        uint32_t* st_next = (p + 1 < passes) ? status[(p + 1) & 1] : nullptr;
+        if (values[0]) {
             hipLaunchKernelGGL((onesweep_kernel<KeyT, true, S>), dim3(l.partitions), dim3(S::kThreads), 0, s,
                                keys[cur], values[cur], keys[cur ^ 1], values[cur ^ 1], count, count_dev, strict,
-                               shift, bits, hist + p * kRadix, status[p & 1],
-                               (p + 1 < passes) ? status[(p + 1) & 1] : nullptr, tickets + p, trivial + p);
-        else
-            hipLaunchKernelGGL((onesweep_kernel<KeyT, false, S>), dim3(l.partitions), dim3(S::kThreads), 0, s,
-                               keys[cur], nullptr, keys[cur ^ 1], nullptr, count, count_dev, strict, shift, bits,
-                               hist + p * kRadix, status[p & 1], (p + 1 < passes) ? status[(p + 1) & 1] : nullptr,
-                               tickets + p, trivial + p);
+                               shift, bits, hist + p * kRadix, st, st_next, tickets + p, trivial + p);
+        } else if constexpr (sizeof(KeyT) == 8) {
+            // keys only: the persistent kernel, with 7-bit digit tables where the digit fits them
+            if (bits <= 7) {
+                const int parts = (count + KeysShape7::kTile - 1) / KeysShape7::kTile;
+                hipLaunchKernelGGL(onesweep_keys_kernel<KeysShape7>, dim3(std::min(keys_grid<KeysShape7>(), parts)),
+                                   dim3(KeysShape7::kThreads), 0, s, keys[cur], keys[cur ^ 1], count, count_dev,
+                                   strict, shift, bits, hist + p * kRadix, st, st_next, tickets + p, trivial + p);
+            } else {
+                const int parts = (count + KeysShape8::kTile - 1) / KeysShape8::kTile;
+                hipLaunchKernelGGL(onesweep_keys_kernel<KeysShape8>, dim3(std::min(keys_grid<KeysShape8>(), parts)),
+                                   dim3(KeysShape8::kThreads), 0, s, keys[cur], keys[cur ^ 1], count, count_dev,
+                                   strict, shift, bits, hist + p * kRadix, st, st_next, tickets + p, trivial + p);
+            }
+        } else {
+            return hipErrorInvalidValue;   // keys-only sorts are 64-bit
+        }
         *sel = cur ^ 1;
     }
     if (ev_end) {

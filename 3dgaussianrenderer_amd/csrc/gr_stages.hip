@@ -568,6 +568,8 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
     __shared__ uint32_t s_hist[3][256];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool hist = a.tile_hist != nullptr;
+    const int db = a.tile_digit_bits;
+    const uint32_t dm = (1u << db) - 1u;
     if (hist)
         for (int i = threadIdx.x; i < 3 * 256; i += kBuildThreads) (&s_hist[0][0])[i] = 0u;
     __syncthreads();
@@ -577,7 +579,7 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
         if (p < cap) {
             a.entries[p] = ((uint64_t)(uint32_t)s_gi[w][o] << 32) | tile;
             if (hist)
-                for (int k = 0; k < a.tile_passes; ++k) atomicAdd(&s_hist[k][(tile >> (8 * k)) & 255u], 1u);
+                for (int k = 0; k < a.tile_passes; ++k) atomicAdd(&s_hist[k][(tile >> (db * k)) & dm], 1u);
         }
     };
     // the next batch's record data is loaded while the current batch is written out
@@ -725,7 +727,7 @@ hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
     if (a.records) {
         if (!a.record_slots || !a.record_total || !a.record_counts || !a.record_masks || !a.record_offsets ||
-            !a.entries || a.tile_passes > 3)
+            !a.entries || a.tile_passes > 3 || (a.tile_hist && (a.tile_digit_bits < 1 || a.tile_digit_bits > 8)))
             return hipErrorInvalidValue;
         constexpr int W = kBuildThreads / 64;
         const unsigned blocks = (unsigned)std::max(1, std::min((a.batches + W - 1) / W, kEmitBlocks));

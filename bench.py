@@ -18,7 +18,7 @@ object reads.  A second loop renders the same frames with per-stage stats
 (RenderStats) and a third with an event after every kernel (the per-kernel
 table); both are reported beside the headline, never as it.
 
-Roofline: the dominant stage is the tile-bit onesweep sort, two launches per
+Roofline: the dominant stage is the tile-bit onesweep sort (`onesweep_keys_kernel`), two launches per
 frame (one launch is shorter than SH, record_count or raster, but the pair
 takes the most time of any stage).  The renderer's tile list holds 8-byte
 entries (Gaussian index << 32 | tile id; the entries are in depth order before
@@ -411,7 +411,7 @@ def main():
     # tile_sort_pass_ms spans all passes of a frame (back to back between two events)
     pass_avg_ms = float(np.mean(pass_ms)) / passes if passes > 0 else float("nan")
     achieved = 16.0 * K / (pass_avg_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": "onesweep_kernel<unsigned long, false> (tile-bit sort pass)",
+    roof = {"bound": "hbm", "kernel": "onesweep_keys_kernel (tile-bit sort pass)",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.traffic_profile),
             "algorithmic_bytes_per_launch": int(16.0 * K), "avg_launch_us": round(pass_avg_ms * 1e3, 2),

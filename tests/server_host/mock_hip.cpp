@@ -423,6 +423,11 @@ hipError_t radix_sort_prepare(int32_t count, void* temp, size_t temp_bytes, uint
     return hipSuccess;
 }
 
+int sort_digit_bits(int32_t begin_bit, int32_t end_bit) {
+    const int p = (end_bit - begin_bit + 7) / 8;
+    return p > 0 ? (end_bit - begin_bit + p - 1) / p : 0;
+}
+
 size_t radix_sort_temp_bytes(int32_t count, int32_t, int32_t end_bit) {
     return (size_t)count * 4 + 4096 + (size_t)end_bit;
 }
