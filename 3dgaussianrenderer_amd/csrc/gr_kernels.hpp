@@ -112,20 +112,34 @@ hipError_t launch_record_slots(const uint32_t* offsets, const uint32_t* total, i
 // Depth-ordered tile lists (the renderer's path): the visible Gaussians of
 // every view are the front end's build records, numbered in (block, view,
 // index) order.  The count pass walks the records in that order, one wave per
-// 64 records, and writes each record's exact entry count and its (depth word,
-// record index) pair; the pairs are sorted stably by depth
-// word; a scan of the counts taken in depth order gives every record its
-// output offset; the emit pass walks the records in the same order again and
-// writes each record's entries at its offset.  Entries of one
-// tile therefore sit in (depth word, index) order -- the order the stable
+// 64 records (footprint sizes spread evenly over the waves), and writes each
+// record's emit record and its (depth word, record index | entry count) pair;
+// the pairs are sorted stably by depth word; a scan of the counts taken in
+// depth order gives every record its output offset; the emit pass walks the
+// records in index order again, reading their emit records and offsets
+// contiguously, and writes each record's entries at its offset.  Entries of
+// one tile therefore sit in (depth word, index) order -- the order the stable
 // sort of (tile << 32 | depth) keys emitted in index order produces -- and
-// only the tile bits remain to be sorted.  Both build passes keep the
-// balanced index-order walk: depth order would put the largest (nearest)
-// splats into the same few waves.
+// only the tile bits remain to be sorted.  (Walking the emit in depth order,
+// with contiguous offsets and stores but gathered emit records, measured
+// slower: the nearest, largest footprints then crowd into the first waves.)
 //
 // Build record (32 bytes):
 //   [0] centre x, centre y, cos, sin     [1] extent x, extent y,
 //   depth word (bits), view * count + index (bits)
+//
+// Emit record (16 bytes, uint4), written by the count pass per record:
+//   footprints of at most 64 candidate tiles:
+//     x, y  mask of the passing candidates (candidate l = row-major index in the
+//           footprint rect; bit l of x | y << 32)
+//     z     view * count + index
+//     w     local tile index of the rect's corner (y0 * tiles_x + x0, < 2^20)
+//           | (rect width - 1) << 20
+//   larger footprints (their candidates are tested again by the emit):
+//     x     staging slot of the build record, y exact entry count,
+//     z     view * count + index, w kEmitBig
+constexpr uint32_t kEmitBig = 1u << 31;
+constexpr int kEmitCornerBits = 20;
 
 struct BuildArgs {
     const float2* clip_xy;       // [nviews][count]
@@ -140,9 +154,10 @@ struct BuildArgs {
     const uint32_t* record_slots;// record walk: staging slot of record p (launch_record_slots)
     const uint32_t* record_total;// record walk: device-side number of records
     uint32_t* depth_keys;        // count pass: depth word of record p (depth sort keys)
-    uint32_t* record_counts;     // count pass: exact entries per record
-    uint64_t* record_masks;      // count pass: passing candidates 0..63 per record
-    uint32_t* depth_words;       // count pass: record index | entry count << index_bits (depth sort values)
+    uint32_t* depth_words;       // count pass: record p | entry count << index_bits (depth sort values;
+                                 // the count saturated at 2^(32 - index_bits) - 1, then the emit
+                                 // record holds it)
+    uint4* emit_records;         // count pass: emit record of record p
     int32_t index_bits;          // packed_index_bits(nviews * count)
     const uint32_t* record_offsets; // emit pass: output offset per record
     uint32_t* tile_hist;         // record emit: 256-bin histograms of the tile's digits (NULL: none)
@@ -203,10 +218,10 @@ hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_
 // Bits of the record index in a packed record word for n records (>= 1, <= 31).
 int packed_index_bits(int64_t n);
 // Exclusive scan of the counts held by packed record words (record index in
-// bits [0, index_bits), count above it, saturated counts looked up in
-// counts[index]): the first *valid words, in their order; the prefix of word j
-// goes to offsets[index of word j] and the total to *total.
-hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint32_t* valid, const uint32_t* counts,
+// bits [0, index_bits), count above it, a saturated count taken from the
+// record's emit record): the first *valid words, in their order; the prefix
+// of word j goes to offsets[index of word j] and the total to *total.
+hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint32_t* valid, const uint4* emit_records,
                                     int32_t index_bits, uint32_t* total, uint32_t* scratch, uint32_t* offsets,
                                     hipStream_t stream);
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t stream);
@@ -224,7 +239,8 @@ size_t radix_sort_temp_bytes(int32_t count, int32_t begin_bit, int32_t end_bit);
 // [begin + p * w, min(begin + (p + 1) * w, end)).
 int sort_digit_bits(int32_t begin_bit, int32_t end_bit);
 // Sorts (keys[sel], values[sel]) on bits [begin_bit, end_bit); on return *sel
-// names the buffer pair that holds the sorted data.  The number of pairs is
+// names the buffer pair that holds the sorted data.  With values[0] == NULL
+// the keys are sorted alone (the persistent keys-only passes).  The number of pairs is
 // min(*count_dev, count) when count_dev is not NULL (a device-side count, so
 // the sort can follow a build without a host round trip), else count; grids
 // are sized for `count`.

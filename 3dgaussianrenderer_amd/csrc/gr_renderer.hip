@@ -7,12 +7,11 @@
 //      visibility (non-empty tile rect) and compaction of the block's visible
 //      Gaussians into build records in the block's own staging slice
 //   -> scan of the block counts (the record numbering) -> staging slot of each
-//      record -> build count: exact entries, the mask of passing candidates and
-//      the depth word per record, and the depth sort's value (record index
-//      with the count packed above it)
-//   -> 32-bit onesweep sort of the (depth word, record | count) pairs -> the
-//      counts, now in depth order, are scanned (K) and scattered back as
-//      per-record output offsets
+//      record -> build count: per record its emit record (mask of passing
+//      candidates, rect) and the depth sort's pair (depth word, record index
+//      with the entry count packed above it)
+//   -> 32-bit onesweep sort of the pairs -> the counts, now in depth order,
+//      are scanned (K) and scattered back as per-record output offsets
 //   -> emit (records in index order): each record's entries, as 8-byte
 //      (Gaussian index << 32 | tile id) words, at its offset, + tile digit
 //      histograms
@@ -118,11 +117,10 @@ struct gr_renderer {
     DevBuf<int32_t> full_values;
     DevBuf<uchar4> out;
     DevBuf<uint32_t> batch_counts;  // total K, scan scratch
-    DevBuf<uint32_t> rec_counts;    // entries per build record
+    DevBuf<uint4> emit_recs;        // emit record per build record
     DevBuf<uint32_t> rec_offsets;   // output offset per build record
-    DevBuf<uint64_t> rec_masks;     // passing candidates 0..63 per build record
     DevBuf<uint32_t> dkeys[2];      // depth words of the visible Gaussians
-    DevBuf<int32_t> dorder[2];      // their build-record indices
+    DevBuf<int32_t> dorder[2];      // their packed record words (record index | entry count)
     DevBuf<float4> records;         // staged build records of the visible Gaussians (2 x float4 each)
     DevBuf<uint32_t> blocks;        // front-end block counts (then offsets), the record total, scan scratch
     DevBuf<uint32_t> slots;         // staging slot of each record
@@ -167,7 +165,8 @@ gr_status gr_renderer_destroy(gr_renderer* r) {
     r->full_values.release(); r->out.release();
     r->batch_counts.release();
     r->dkeys[0].release(); r->dkeys[1].release(); r->dorder[0].release(); r->dorder[1].release();
-    r->records.release(); r->blocks.release(); r->slots.release(); r->rec_counts.release(); r->rec_offsets.release(); r->rec_masks.release();
+    r->rec_offsets.release(); r->emit_recs.release();
+    r->records.release(); r->blocks.release(); r->slots.release();
     if (r->sort_temp) (void)hipFree(r->sort_temp);
     for (auto& e : r->ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : r->pass_ev) if (e) (void)hipEventDestroy(e);
@@ -271,9 +270,8 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     const int batches = (n + kBuildBatch - 1) / kBuildBatch;
     const size_t nn = (size_t)std::max(n, 1);
     R_CHECK(r->batch_counts.ensure(2 + scan_scratch_words((int32_t)nn)));
-    R_CHECK(r->rec_counts.ensure(nn));
+    R_CHECK(r->emit_recs.ensure(nn));
     R_CHECK(r->rec_offsets.ensure(nn));
-    R_CHECK(r->rec_masks.ensure(nn));
     for (int k = 0; k < 2; ++k) {
         R_CHECK(r->entries[k].ensure(1));
         R_CHECK(r->dkeys[k].ensure(nn));
@@ -341,15 +339,14 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     R_HIP(launch_build_scan(r->blocks.p, nb, vis_total, r->blocks.p + nb + 1, stream));
     R_HIP(launch_record_slots(r->blocks.p, vis_total, nb, kFrontThreads, r->slots.p, stream));
     R_HIP(mark("numbering"));
-    // exact entry count, candidate mask and depth word of every record
+    // emit record and depth pair of every record
     BuildArgs b{};
     b.records = r->records.p;
     b.record_slots = r->slots.p;
     b.record_total = vis_total;
     b.depth_keys = r->dkeys[0].p;
-    b.record_counts = r->rec_counts.p;
-    b.record_masks = r->rec_masks.p;
     b.depth_words = reinterpret_cast<uint32_t*>(r->dorder[0].p);
+    b.emit_records = r->emit_recs.p;
     b.index_bits = packed_index_bits(std::max(n, 1));
     b.entries = r->entries[0].p;
     b.count = n;
@@ -373,7 +370,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     // output offsets: the counts carried by the sorted words, scanned in depth order (total K)
     // and scattered back to the records
     R_HIP(launch_build_scan_packed(reinterpret_cast<const uint32_t*>(r->dorder[dsel].p), n, vis_total,
-                                   r->rec_counts.p, b.index_bits, k_total, k_total + 1, r->rec_offsets.p, stream));
+                                   r->emit_recs.p, b.index_bits, k_total, k_total + 1, r->rec_offsets.p, stream));
     R_HIP(mark("offsets"));
     b.record_offsets = r->rec_offsets.p;
     const int tile_bits = tile_bits_for(tiles);

@@ -155,13 +155,13 @@ enum class SlotState { Idle, PhaseA, WaitMemory, PhaseB };
 struct Workspace {
     Dev color, xy, z, ell, con;      // [V][N]
     Dev blocks;                       // front-end block counts (then offsets), the record count, scan scratch
-    Dev dkeys0, dkeys1, dorder0, dorder1;   // [<= V*N] depth words and record indices of visible Gaussians
+    Dev dkeys0, dkeys1, dorder0, dorder1;   // [<= V*N] depth words and packed record words of visible Gaussians
     Dev records;                      // [blocks][256 * V][2] float4 staged build records
     Dev slots;                        // [<= V*N] staging slot of each record
     Dev dsort_temp;                   // depth sort scratch
     Dev counts;                       // K (device) + scan scratch
-    Dev rec_counts, rec_offsets;      // [V*N] entries and output offset per record
-    Dev rec_masks;                    // [V*N] passing candidates 0..63 per record
+    Dev emit_recs;                    // [V*N] emit record per record
+    Dev rec_offsets;                  // [V*N] output offset per record
     Dev list0, list1;                 // tile lists: Gaussian index << 32 | global tile id
     Dev sort_temp;
     Dev ranges;                       // 2 * tiles
@@ -171,8 +171,9 @@ struct Workspace {
     Pinned img_host[2];               // image staging, alternating between batches
     uint64_t generation = 0;          // bumps when a pointer a captured graph holds changes
     std::vector<Dev*> devs() {
-        return {&color, &xy, &z, &ell, &con, &blocks, &dkeys0, &dkeys1, &dorder0, &dorder1,
-                &records, &slots, &rec_counts, &rec_offsets, &rec_masks, &dsort_temp, &counts, &list0, &list1, &sort_temp, &ranges, &images, &views};
+        return {&color, &xy, &z, &ell, &con, &blocks, &dkeys0, &dkeys1, &dorder0, &dorder1, &records, &slots,
+                &emit_recs, &rec_offsets,
+                &dsort_temp, &counts, &list0, &list1, &sort_temp, &ranges, &images, &views};
     }
 };
 
@@ -384,9 +385,8 @@ std::vector<PhaseANeed> phase_a_needs(Slot& sl, const Scene& sc, size_t nviews, 
             {&w.slots, V * N * 4},
             {&w.dsort_temp, (int64_t)radix_sort_temp_bytes((int32_t)std::max<int64_t>(V * N, 1), 0, 32)},
             {&w.counts, 4 * (2 + scan_scratch_words((int32_t)(V * N)))},
-            {&w.rec_counts, V * N * 4},
+            {&w.emit_recs, V * N * 16},
             {&w.rec_offsets, V * N * 4},
-            {&w.rec_masks, V * N * 8},
             {&w.images, img_bytes},
             {&w.ranges, 8 * (int64_t)tiles_total},
             {&w.views, V * (int64_t)sizeof(ViewDesc)}};
@@ -475,10 +475,9 @@ BuildArgs record_build_args(Slot& sl, const Scene& sc) {
     b.record_slots = w.slots.as<uint32_t>();
     b.record_total = w.blocks.as<uint32_t>() + front_blocks(N);
     b.depth_keys = w.dkeys0.as<uint32_t>();
-    b.record_counts = w.rec_counts.as<uint32_t>();
-    b.record_offsets = w.rec_offsets.as<uint32_t>();
-    b.record_masks = w.rec_masks.as<uint64_t>();
     b.depth_words = w.dorder0.as<uint32_t>();
+    b.emit_records = w.emit_recs.as<uint4>();
+    b.record_offsets = w.rec_offsets.as<uint32_t>();
     b.index_bits = packed_index_bits(std::max<int64_t>((int64_t)V * N, 1));
     b.count = N;
     b.capacity = (int32_t)sl.capacity;
@@ -537,9 +536,8 @@ gr_status enqueue_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     const int vn = V * N;
     // output offsets: the counts carried by the sorted words, scanned in depth order (K) and
     // scattered back to the records
-    S_HIP(launch_build_scan_packed(reinterpret_cast<const uint32_t*>(dv[dsel]), vn, vis,
-                                   w.rec_counts.as<uint32_t>(), b.index_bits, counts, counts + 2,
-                                   w.rec_offsets.as<uint32_t>(), sl.stream));
+    S_HIP(launch_build_scan_packed(reinterpret_cast<const uint32_t*>(dv[dsel]), vn, vis, w.emit_recs.as<uint4>(),
+                                   b.index_bits, counts, counts + 2, w.rec_offsets.as<uint32_t>(), sl.stream));
     S_HIP(hipMemcpyAsync(w.k_host.p, counts, 4, hipMemcpyDeviceToHost, sl.stream));
     return GR_OK;
 }

@@ -484,15 +484,19 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
     if (first >= limit) return;   // wave-uniform; no block barriers below
     const int pos = first + lane;
     int cnt = 0;
+    uint32_t slot = 0u, dword = 0u, pg = 0u, corner = 0u;
     if (pos < limit) {
-        const size_t slot = a.record_slots[pos];
-        const float4 r0 = a.records[2 * slot];
-        const float4 r1 = a.records[2 * slot + 1];
-        a.depth_keys[pos] = __float_as_uint(r1.z);
-        const int view = __float_as_int(r1.w) / a.count;
+        slot = a.record_slots[pos];
+        const float4 r0 = a.records[2 * (size_t)slot];
+        const float4 r1 = a.records[2 * (size_t)slot + 1];
+        dword = __float_as_uint(r1.z);
+        pg = __float_as_uint(r1.w);
+        const int view = (int)pg / a.count;
         const FrameGeom& g = view_of(view, a.v0, a.views).geom;
         const Footprint f = footprint(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, g);
         cnt = (f.nx > 0 && f.ny > 0) ? f.nx * f.ny : 0;
+        if (cnt > 0 && cnt <= 64)
+            corner = (uint32_t)(f.y0 * g.tiles_x + f.x0) | ((uint32_t)(f.nx - 1) << kEmitCornerBits);
         s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, cnt);
         const EllipseRec er{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, 1.0f / r1.x, 1.0f / r1.y};
         s_ell[w][lane] = er;
@@ -535,30 +539,33 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
         }
     }
     if (pos < limit) {
-        a.record_counts[pos] = my_cnt;
-        a.record_masks[pos] = my_mask;
-        // the depth sort's value: record index | count (saturated) in the bits above it
+        // the depth sort's pair: depth word, record index | count (saturated) in the bits above it
         const uint32_t esc = 0xffffffffu >> a.index_bits;
+        a.depth_keys[pos] = dword;
         a.depth_words[pos] = (uint32_t)pos | (min(my_cnt, esc) << a.index_bits);
+        a.emit_records[pos] = cnt <= 64 ? make_uint4((uint32_t)my_mask, (uint32_t)(my_mask >> 32), pg, corner)
+                                        : make_uint4(slot, my_cnt, pg, kEmitBig);
     }
 }
 
-// Record walk, emit pass: the same 64-record batches (index order), each
-// wave taking batches grid-stride (one tile-digit histogram flush per
-// workgroup).  Record o's entries go to record_offsets[o] + 0, 1, ... in
-// candidate order, as (Gaussian index << 32 | tile id) entries: the offsets
-// follow depth order, so the depth word of the key is carried by the position
-// and the stable tile sort that follows needs the tile bits alone.  (Walking the
-// records in depth order instead would make the stores contiguous but turn
-// the record loads into gathers, which costs more.)
+// Record walk, emit pass: the same 64-record batches (index order) as the
+// count pass, each wave taking batches grid-stride (one tile-digit histogram
+// flush per workgroup), the emit records and offsets read contiguously.
+// Record o's entries go to record_offsets[o] + 0, 1, ... in candidate order,
+// as (Gaussian index << 32 | tile id) entries: the offsets follow depth
+// order, so the depth word of the key is carried by the position and the
+// stable tile sort that follows needs the tile bits alone.  Footprints of at
+// most 64 candidates come with their passing-candidate mask (entry k is
+// candidate select_bit(mask, k): no tests, no footprint); larger ones fetch
+// their build record and run the overlap tests again.
 __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a) {
     constexpr int W = kBuildThreads / 64;
     __shared__ int s_excl[W][64];
-    __shared__ int4 s_rect[W][64];       // x0, y0, nx, tiles_x
+    __shared__ int4 s_rect[W][64];       // global tile of the rect corner, nx, tiles_x, -
     __shared__ uint64_t s_mask[W][64];
     __shared__ uint32_t s_off[W][64];
     __shared__ int s_gi[W][64];
-    __shared__ int s_tb[W][64];          // tile base of the record's view
+    __shared__ int2 s_xy[W][64];         // rect corner in view tiles (large footprints)
     __shared__ EllipseRec s_ell[W][64];  // large footprints only
     __shared__ float2 s_clip[W][64];
     __shared__ uint32_t s_done[W][64];
@@ -582,50 +589,49 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
                 for (int k = 0; k < a.tile_passes; ++k) atomicAdd(&s_hist[k][(tile >> (db * k)) & dm], 1u);
         }
     };
-    // the next batch's record data is loaded while the current batch is written out
+    // the next batch's emit records are fetched while the current batch is written out
     const int stride = gridDim.x * W * kBuildBatch;
     int first = (blockIdx.x * W + w) * kBuildBatch;
-    float4 n_r0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), n_r1 = n_r0;
-    uint64_t n_mask = 0ull;
-    uint32_t n_off = 0u, n_cnt = 0u;
+    uint4 n_er = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t n_off = 0u;
     auto fetch = [&](int f) {
         const int q = f + lane;
         if (f < limit && q < limit) {
-            const size_t slot = a.record_slots[q];
-            n_r0 = a.records[2 * slot];
-            n_r1 = a.records[2 * slot + 1];
-            n_mask = a.record_masks[q];
             n_off = a.record_offsets[q];
-            n_cnt = a.record_counts[q];
+            n_er = a.emit_records[q];
         }
     };
     fetch(first);
     for (; first < limit; first += stride) {
         const int pos = first + lane;
-        const float4 r0 = n_r0, r1 = n_r1;
-        const uint64_t mask = n_mask;
-        const uint32_t off = n_off, rcnt = n_cnt;
+        const uint4 er = n_er;
+        const uint32_t off = n_off;
         fetch(first + stride);
         int ecnt = 0, big = 0;
         if (pos < limit) {
-            const int pg = __float_as_int(r1.w);
+            const int pg = (int)er.z;
             const int view = pg / a.count;
             const ViewDesc& vd = view_of(view, a.v0, a.views);
-            const Footprint f = footprint(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, vd.geom);
-            const int cand = (f.nx > 0 && f.ny > 0) ? f.nx * f.ny : 0;
-            s_rect[w][lane] = make_int4(f.x0, f.y0, f.nx, vd.geom.tiles_x);
-            s_tb[w][lane] = vd.tile_base;
-            s_mask[w][lane] = mask;
             s_off[w][lane] = off;
             s_gi[w][lane] = pg - view * a.count;
-            if (cand <= 64) {
-                ecnt = (int)rcnt;
+            if (!(er.w & kEmitBig)) {
+                const uint64_t mask = (uint64_t)er.x | ((uint64_t)er.y << 32);
+                ecnt = __popcll(mask);
+                s_mask[w][lane] = mask;
+                const int corner = (int)(er.w & ((1u << kEmitCornerBits) - 1u));
+                s_rect[w][lane] = make_int4(vd.tile_base + corner, (int)(er.w >> kEmitCornerBits) + 1,
+                                            vd.geom.tiles_x, 0);
             } else {
-                big = cand;
-                s_cand[w][lane] = cand;
-                const EllipseRec er{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, 1.0f / r1.x, 1.0f / r1.y};
-                s_ell[w][lane] = er;
-                const FastEllipse fe = fast_ellipse(er, f, vd.geom.tile_clip_x, vd.geom.tile_clip_y);
+                const float4 r0 = a.records[2 * (size_t)er.x];
+                const float4 r1 = a.records[2 * (size_t)er.x + 1];
+                const Footprint f = footprint(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, vd.geom);
+                big = f.nx * f.ny;
+                s_cand[w][lane] = big;
+                s_rect[w][lane] = make_int4(vd.tile_base + f.y0 * vd.geom.tiles_x + f.x0, f.nx, vd.geom.tiles_x, 0);
+                s_xy[w][lane] = make_int2(f.x0, f.y0);
+                const EllipseRec el{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, 1.0f / r1.x, 1.0f / r1.y};
+                s_ell[w][lane] = el;
+                const FastEllipse fe = fast_ellipse(el, f, vd.geom.tile_clip_x, vd.geom.tile_clip_y);
                 s_fast[w][lane] = make_float4(fe.kv, fe.kh, fe.lo, 0.0f);
                 s_clip[w][lane] = make_float2(vd.geom.tile_clip_x, vd.geom.tile_clip_y);
             }
@@ -645,8 +651,8 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
                 const int l = select_bit(s_mask[w][o], k);
                 const int4 rc = s_rect[w][o];
                 int q, r;
-                divmod_small(l, rc.z, q, r);
-                put(s_off[w][o] + (uint32_t)k, (uint32_t)(s_tb[w][o] + (rc.y + q) * rc.w + rc.x + r), o);
+                divmod_small(l, rc.y, q, r);
+                put(s_off[w][o] + (uint32_t)k, (uint32_t)(rc.x + q * rc.z + r), o);
             }
             wave_sync();
         }
@@ -668,15 +674,16 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
             if (cc < total) {
                 l = cc - s_excl[w][o];
                 const int4 rc = s_rect[w][o];
+                const int2 xy = s_xy[w][o];
                 int q, r;
-                divmod_small(l, rc.z, q, r);
+                divmod_small(l, rc.y, q, r);
                 FrameGeom og;
                 og.tile_clip_x = s_clip[w][o].x;
                 og.tile_clip_y = s_clip[w][o].y;
                 const float4 sf = s_fast[w][o];
-                const EllipseRec& er = s_ell[w][o];
-                pass = tile_overlap_fast(rc.x + r, rc.y + q, er, fast_ellipse_from(er, sf.x, sf.y, sf.z), og);
-                tile = (uint32_t)(s_tb[w][o] + (rc.y + q) * rc.w + rc.x + r);
+                const EllipseRec& el = s_ell[w][o];
+                pass = tile_overlap_fast(xy.x + r, xy.y + q, el, fast_ellipse_from(el, sf.x, sf.y, sf.z), og);
+                tile = (uint32_t)(rc.x + q * rc.z + r);
             }
             const uint64_t m = __ballot(pass);
             if (pass) {
@@ -712,8 +719,8 @@ static dim3 build_grid(const BuildArgs& a) {
 hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
     if (a.records) {
-        if (!a.record_slots || !a.record_total || !a.record_counts || !a.record_masks || !a.depth_words ||
-            !a.depth_keys || a.index_bits < 1 || a.index_bits > 31 ||
+        if (!a.record_slots || !a.record_total || !a.depth_keys || !a.depth_words || !a.emit_records ||
+            a.index_bits < 1 || a.index_bits > 31 ||
             ((int64_t)1 << a.index_bits) < (int64_t)a.nviews * a.count)
             return hipErrorInvalidValue;
         hipLaunchKernelGGL(record_count_kernel, build_grid(a), dim3(kBuildThreads), 0, s, a);
@@ -726,8 +733,9 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
     if (a.records) {
-        if (!a.record_slots || !a.record_total || !a.record_counts || !a.record_masks || !a.record_offsets ||
-            !a.entries || a.tile_passes > 3 || (a.tile_hist && (a.tile_digit_bits < 1 || a.tile_digit_bits > 8)))
+        if (!a.record_total || !a.record_offsets || !a.emit_records || !a.records || !a.entries ||
+            a.tile_passes > 3 ||
+            (a.tile_hist && (a.tile_digit_bits < 1 || a.tile_digit_bits > 8)))
             return hipErrorInvalidValue;
         constexpr int W = kBuildThreads / 64;
         const unsigned blocks = (unsigned)std::max(1, std::min((a.batches + W - 1) / W, kEmitBlocks));
@@ -890,28 +898,41 @@ __device__ __forceinline__ uint32_t block_sum_256(uint32_t v, uint32_t* s_w) {
     return s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
-// Packed record words (the depth sort's values): record index in bits [0, ib),
-// entry count in bits [ib, 32), saturated at 2^(32-ib) - 1, in which case
-// counts[index] holds it (1 <= ib <= 31).
-__device__ __forceinline__ uint32_t packed_count(uint32_t v, int ib, const uint32_t* counts) {
+// Packed record words (the low words of the depth pairs): record index in
+// bits [0, ib), entry count in bits [ib, 32), saturated at 2^(32-ib) - 1, in
+// which case the record's emit record gives it (1 <= ib <= 31).
+__device__ __forceinline__ uint32_t packed_count(uint32_t v, int ib, const uint4* erec) {
     const uint32_t c = v >> ib;
-    return c == (0xffffffffu >> ib) ? counts[v & ((1u << ib) - 1u)] : c;
+    if (c != (0xffffffffu >> ib)) return c;
+    const uint4 e = erec[v & ((1u << ib) - 1u)];
+    return (e.w & kEmitBig) ? e.y : (uint32_t)__popcll((uint64_t)e.x | ((uint64_t)e.y << 32));
 }
 
-// PACKED: data holds packed record words, the first *valid of them meaningful,
-// and the values scanned are their counts.
-template <bool PACKED>
-__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* data, int32_t n, uint32_t* sums,
-                                                                   const uint32_t* valid, const uint32_t* counts,
-                                                                   int ib) {
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* data, int32_t n, uint32_t* sums) {
     __shared__ uint32_t s_w[4];
     const int64_t base = (int64_t)blockIdx.x * kScanTile;
-    const int64_t nv = PACKED ? (int64_t)min(*valid, (uint32_t)n) : (int64_t)n;
     uint32_t v = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const int64_t i = base + k * kScanThreads + threadIdx.x;
-        if (i < nv) v += PACKED ? packed_count(data[i], ib, counts) : data[i];
+        if (i < n) v += data[i];
+    }
+    const uint32_t tot = block_sum_256(v, s_w);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// The counts of the first *valid packed record words.
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_packed_kernel(const uint32_t* words, int32_t n,
+                                                                          uint32_t* sums, const uint32_t* valid,
+                                                                          const uint4* erec, int ib) {
+    __shared__ uint32_t s_w[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    const int64_t nv = (int64_t)min(*valid, (uint32_t)n);
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k * kScanThreads + threadIdx.x;
+        if (i < nv) v += packed_count(words[i], ib, erec);
     }
     const uint32_t tot = block_sum_256(v, s_w);
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
@@ -945,24 +966,10 @@ __global__ __launch_bounds__(1024) void scan_sums_kernel(uint32_t* sums, int32_t
     if (threadIdx.x == 0) *total = (uint32_t)(s_carry > 0xffffffffull ? 0xffffffffull : s_carry);
 }
 
-// PACKED: the exclusive prefix of packed word j (j < *valid) goes to
-// offsets[index of word j] instead of back to data[j].
-template <bool PACKED>
-__global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data, int32_t n, const uint32_t* sums,
-                                                                 const uint32_t* valid, const uint32_t* counts,
-                                                                 int ib, uint32_t* offsets) {
-    __shared__ uint32_t s_w[4];
+// Local scan of kScanItems values per lane plus the block's offset from the
+// scanned block sums; returns this lane's first exclusive prefix.
+__device__ __forceinline__ uint32_t block_excl_256(uint32_t sum, uint32_t block_off, uint32_t* s_w) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t my = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-    const int64_t nv = PACKED ? (int64_t)min(*valid, (uint32_t)n) : (int64_t)n;
-    uint32_t word[kScanItems], v[kScanItems];
-    uint32_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        word[k] = (my + k < nv) ? data[my + k] : 0u;
-        v[k] = (my + k < nv) ? (PACKED ? packed_count(word[k], ib, counts) : word[k]) : 0u;
-        sum += v[k];
-    }
     uint32_t incl = sum;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -971,15 +978,49 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data,
     }
     if (lane == 63) s_w[w] = incl;
     __syncthreads();
-    uint32_t run = sums[blockIdx.x] + incl - sum;
+    uint32_t run = block_off + incl - sum;
     for (int k = 0; k < w; ++k) run += s_w[k];
+    return run;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data, int32_t n, const uint32_t* sums) {
+    __shared__ uint32_t s_w[4];
+    const int64_t my = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        if constexpr (PACKED) {
-            if (my + k < nv) offsets[word[k] & ((1u << ib) - 1u)] = run;
-        } else {
-            if (my + k < n) data[my + k] = run;
-        }
+        v[k] = (my + k < n) ? data[my + k] : 0u;
+        sum += v[k];
+    }
+    uint32_t run = block_excl_256(sum, sums[blockIdx.x], s_w);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (my + k < n) data[my + k] = run;
+        run += v[k];
+    }
+}
+
+// The exclusive prefix of packed word j (j < *valid) goes to offsets[index of word j].
+__global__ __launch_bounds__(kScanThreads) void scan_down_packed_kernel(const uint32_t* words, int32_t n,
+                                                                        const uint32_t* sums, const uint32_t* valid,
+                                                                        const uint4* erec, int ib,
+                                                                        uint32_t* offsets) {
+    __shared__ uint32_t s_w[4];
+    const int64_t my = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    const int64_t nv = (int64_t)min(*valid, (uint32_t)n);
+    uint32_t word[kScanItems], v[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        word[k] = (my + k < nv) ? words[my + k] : 0u;
+        v[k] = (my + k < nv) ? packed_count(word[k], ib, erec) : 0u;
+        sum += v[k];
+    }
+    uint32_t run = block_excl_256(sum, sums[blockIdx.x], s_w);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (my + k < nv) offsets[word[k] & ((1u << ib) - 1u)] = run;
         run += v[k];
     }
 }
@@ -992,11 +1033,9 @@ hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_
         return hipGetLastError();
     }
     const int nb = (n + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(scan_reduce_kernel<false>, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch, nullptr,
-                       nullptr, 0);
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch);
     hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, scratch, nb, total);
-    hipLaunchKernelGGL(scan_down_kernel<false>, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch, nullptr,
-                       nullptr, 0, nullptr);
+    hipLaunchKernelGGL(scan_down_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch);
     return hipGetLastError();
 }
 
@@ -1006,7 +1045,7 @@ int packed_index_bits(int64_t n) {
     return b;
 }
 
-hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint32_t* valid, const uint32_t* counts,
+hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint32_t* valid, const uint4* erec,
                                     int32_t index_bits, uint32_t* total, uint32_t* scratch, uint32_t* offsets,
                                     hipStream_t s) {
     if (index_bits < 1 || index_bits > 31 || (n > 0 && ((int64_t)1 << index_bits) < n)) return hipErrorInvalidValue;
@@ -1014,12 +1053,12 @@ hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, total, 0u, (int64_t)1);
         return hipGetLastError();
     }
+    if (!words || !valid || !erec || !scratch || !offsets) return hipErrorInvalidValue;
     const int nb = (n + kScanTile - 1) / kScanTile;
-    uint32_t* data = const_cast<uint32_t*>(words);   // read only in the packed form
-    hipLaunchKernelGGL(scan_reduce_kernel<true>, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch, valid, counts,
+    hipLaunchKernelGGL(scan_reduce_packed_kernel, dim3(nb), dim3(kScanThreads), 0, s, words, n, scratch, valid, erec,
                        index_bits);
     hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, scratch, nb, total);
-    hipLaunchKernelGGL(scan_down_kernel<true>, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch, valid, counts,
+    hipLaunchKernelGGL(scan_down_packed_kernel, dim3(nb), dim3(kScanThreads), 0, s, words, n, scratch, valid, erec,
                        index_bits, offsets);
     return hipGetLastError();
 }

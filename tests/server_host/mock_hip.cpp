@@ -283,10 +283,10 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
                 std::abort();
             for (uint32_t q = 0; q < nvis; ++q) {
                 rd(&a.records[2 * (size_t)a.record_slots[q]], 32);
+                // depth word 7, 3 entries: 4 candidates in a 4-wide rect at the corner, 3 of them pass
                 a.depth_keys[q] = 7u;
-                a.record_counts[q] = 3u;
-                a.record_masks[q] = 0xbull;   // 4 candidates, 3 of them pass
                 a.depth_words[q] = q | (3u << a.index_bits);
+                a.emit_records[q] = make_uint4(0xbu, 0u, q, 3u << kEmitCornerBits);
             }
             return;
         }
@@ -305,7 +305,7 @@ int packed_index_bits(int64_t n) {
     return b;
 }
 
-hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint32_t* valid, const uint32_t* counts,
+hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint32_t* valid, const uint4* erec,
                                     int32_t index_bits, uint32_t* total, uint32_t* scratch, uint32_t* offsets,
                                     hipStream_t s) {
     if (index_bits < 1 || index_bits > 31 || (n > 0 && ((int64_t)1 << index_bits) < n)) return hipErrorInvalidValue;
@@ -316,8 +316,8 @@ hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint
         for (uint32_t i = 0; i < nv; ++i) {
             const uint32_t w = words[i];
             const uint32_t idx = w & ((1u << index_bits) - 1u);
-            if (idx >= nv) std::abort();   // a word names a record that does not exist
-            const uint32_t c = (w >> index_bits) == esc ? counts[idx] : (w >> index_bits);
+            if (idx >= nv) std::abort();   // a pair names a record that does not exist
+            const uint32_t c = (w >> index_bits) == esc ? (uint32_t)__builtin_popcount(erec[idx].x) : (w >> index_bits);
             offsets[idx] = acc;
             acc += c;
         }
@@ -350,10 +350,10 @@ hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
             const uint32_t nvis = std::min<uint32_t>(*a.record_total, (uint32_t)(a.nviews * a.count));
             K = 0;
             for (uint32_t q = 0; q < nvis; ++q) {
-                rd(&a.records[2 * (size_t)a.record_slots[q]], 32);
+                const uint4 e = a.emit_records[q];
                 if (a.record_offsets[q] > 3u * nvis) std::abort();   // offsets lie inside the list
-                if (__builtin_popcountll(a.record_masks[q]) != (int)a.record_counts[q]) std::abort();
-                K += a.record_counts[q];
+                if ((e.w & kEmitBig) || e.y != 0u || e.z != q) std::abort();
+                K += (uint32_t)__builtin_popcount(e.x);
             }
             if (a.tile_hist)
                 for (int k = 0; k < a.tile_passes; ++k) a.tile_hist[k * 256] += std::min<uint32_t>(K, (uint32_t)a.capacity);
@@ -455,13 +455,14 @@ hipError_t radix_sort_keys_prepared(uint64_t* keys[2], int32_t* sel, int32_t cou
     return hipSuccess;
 }
 
+
+// moves the data like an even (end-begin <= 32 bits: 4 passes) or odd number of passes would
 hipError_t radix_sort_pairs_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                                 const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
                                 size_t temp_bytes, hipStream_t s) {
     return mock_sort(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
 }
 
-// moves the data like an even (end-begin <= 32 bits: 4 passes) or odd number of passes would
 template <typename KeyT>
 hipError_t mock_sort(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
                      int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes, hipStream_t s) {
@@ -476,10 +477,10 @@ hipError_t mock_sort(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t co
     run_or_record(s, [=] {
         const uint32_t n = count_dev ? std::min<uint32_t>(*count_dev, (uint32_t)count) : (uint32_t)count;
         std::memcpy(k1, k0, (size_t)n * sizeof(KeyT));
-        std::memcpy(v1, v0, (size_t)n * 4);
+        if (v0) std::memcpy(v1, v0, (size_t)n * 4);
         if (passes % 2 == 0) {
             std::memcpy(k0, k1, (size_t)n * sizeof(KeyT));
-            std::memcpy(v0, v1, (size_t)n * 4);
+            if (v0) std::memcpy(v0, v1, (size_t)n * 4);
         }
         touch(temp, tb);
     });
