@@ -1232,8 +1232,8 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
             __syncthreads();
             for (int b0 = 0; b0 < nr; b0 += kRasterBatch) {
                 const int b1 = min(b0 + kRasterBatch, nr);
-                for (int j = b0; j < b1; ++j) {
-                    const RasterEntry& en = s_e[j];
+                // two entries per step from one LDS base address (immediate offsets)
+                auto blend = [&](const RasterEntry& en) {
                     const float4 ea = en.a;
                     const float dx = pcx - ea.x, dy = pcy - ea.y;
                     const float4 eb = en.b;
@@ -1247,7 +1247,14 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
                         C2 = fmaf(gb.y, wgt, C2);
                         T = T - wgt;
                     }
+                };
+                int j = b0;
+                for (; j + 1 < b1; j += 2) {
+                    const RasterEntry* pe = &s_e[j];
+                    blend(pe[0]);
+                    blend(pe[1]);
                 }
+                if (j < b1) blend(s_e[j]);
                 const bool alive = __ballot(T > kTransmittanceStop) != 0ull;
                 if (lane == 0) s_alive[parity][w] = alive ? 1 : 0;
                 __syncthreads();
