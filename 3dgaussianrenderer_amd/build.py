@@ -49,6 +49,8 @@ HIP_FLAGS = [
     f"-I{INCLUDE}",
 ]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", f"-I{INCLUDE}"]
+# Tuning macros for build-time sweeps (e.g. GR_HIP_DEFINES="-DGR_KSORT_ITEMS7=16"): only -D flags.
+HIP_DEFINES = [d for d in os.environ.get("GR_HIP_DEFINES", "").split() if d.startswith("-D")]
 
 
 def _run(cmd: list[str], verbose: bool) -> None:
@@ -69,7 +71,7 @@ def build_library(verbose: bool = False) -> Path:
     objs = []
     for src in HIP_SOURCES:
         obj = objdir / (Path(src).stem + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)], verbose)
+        _run([HIPCC, *HIP_FLAGS, *HIP_DEFINES, "-c", str(CSRC / src), "-o", str(obj)], verbose)
         objs.append(str(obj))
     for src in CXX_SOURCES:
         obj = objdir / (Path(src).stem + ".o")

@@ -87,13 +87,14 @@ class FrameView(C.Structure):
 
 
 KERNEL_TIMES_MAX = 24
-TIMING = {"none": 0, "sort_passes": 1, "kernels": 2}
+TIMING = {"none": 0, "sort_passes": 1, "kernels": 2, "front": 3}
 
 
 class KernelTimes(C.Structure):
     _fields_ = [("tile_sort_pass_ms", C.c_double), ("tile_sort_passes", C.c_int32), ("kernels", C.c_int32),
                 ("tile_list_size", C.c_int64), ("visible", C.c_int64),
-                ("kernel_ms", C.c_double * KERNEL_TIMES_MAX), ("kernel_name", (C.c_char * 24) * KERNEL_TIMES_MAX)]
+                ("kernel_ms", C.c_double * KERNEL_TIMES_MAX), ("kernel_name", (C.c_char * 24) * KERNEL_TIMES_MAX),
+                ("front_ms", C.c_double)]
 
 
 class ServerConfig(C.Structure):
@@ -442,7 +443,8 @@ class Renderer:
         return None, (st.as_dict() if stats else None)
 
     def set_timing(self, level: str):
-        """'none', 'sort_passes' (two events around the tile-sort passes) or 'kernels'."""
+        """'none', 'sort_passes' (two events around the tile-sort passes), 'kernels' (an event after
+        every kernel) or 'front' (two events around the front-end kernel)."""
         check(lib().gr_renderer_set_timing(self.h, TIMING[level]))
 
     def kernel_times(self) -> dict:
@@ -450,6 +452,7 @@ class Renderer:
         t = KernelTimes()
         check(lib().gr_renderer_kernel_times(self.h, C.byref(t)))
         return {"tile_sort_pass_ms": t.tile_sort_pass_ms, "tile_sort_passes": t.tile_sort_passes,
+                "front_ms": t.front_ms,
                 "tile_list_size": t.tile_list_size, "visible": t.visible,
                 "kernel_ms": {t.kernel_name[i].value.decode(): t.kernel_ms[i] for i in range(t.kernels)}}
 

@@ -95,7 +95,9 @@ struct gr_renderer {
     int device = 0;
     gr_context* ctx = nullptr;
     hipEvent_t ev[9] = {};          // per-stage events (only when stats are requested) + frame end
-    hipEvent_t pass_ev[2] = {};     // around the tile-sort passes (timing level >= 1, no stats)
+    hipEvent_t pass_ev[2] = {};     // around the tile-sort passes (GR_TIMING_SORT_PASSES / KERNELS, no stats)
+    hipEvent_t front_ev[2] = {};    // around the front-end kernel (GR_TIMING_FRONT)
+    bool front_timed = false;       // the last frame recorded front_ev
     hipEvent_t mark_ev[kMaxMarks] = {};   // per-kernel marks (timing level 2)
     int32_t timing = GR_TIMING_NONE;
     // per-kernel marks of the last frame: event handle and the name of the work before it
@@ -149,6 +151,7 @@ gr_status gr_renderer_create(gr_renderer** out, int32_t device, void* stream) {
     for (int k = 0; k < 8; ++k) R_HIP(hipEventCreateWithFlags(&r->ev[k], hipEventDisableSystemFence));
     R_HIP(hipEventCreate(&r->ev[8]));
     for (auto& e : r->pass_ev) R_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    for (auto& e : r->front_ev) R_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     for (auto& e : r->mark_ev) R_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     *out = r;
     return GR_OK;
@@ -170,6 +173,7 @@ gr_status gr_renderer_destroy(gr_renderer* r) {
     if (r->sort_temp) (void)hipFree(r->sort_temp);
     for (auto& e : r->ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : r->pass_ev) if (e) (void)hipEventDestroy(e);
+    for (auto& e : r->front_ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : r->mark_ev) if (e) (void)hipEventDestroy(e);
     gr_context_destroy(r->ctx);
     delete r;
@@ -299,7 +303,9 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     // by the stage events, which share them).  Each event leaves the GPU idle for a few
     // microseconds, so a frame asked for none of them records none.
     const bool stages = stats != nullptr;
-    const bool kmarks = r->timing >= GR_TIMING_KERNELS;
+    const bool kmarks = r->timing == GR_TIMING_KERNELS;
+    const bool front_ev = r->timing == GR_TIMING_FRONT;
+    r->front_timed = front_ev;
     r->marks = 0;
     auto stage = [&](int k) -> hipError_t { return stages ? hipEventRecord(r->ev[k], stream) : hipSuccess; };
     auto mark = [&](const char* name) -> hipError_t {
@@ -311,6 +317,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     uint32_t* vis_total = r->blocks.p + nb;   // visible Gaussians (records)
     R_HIP(stage(0));
     R_HIP(mark("begin"));
+    if (front_ev) R_HIP(hipEventRecord(r->front_ev[0], stream));
     {
         // SH colour + projection + compaction of the visible Gaussians into build records
         FrontArgs a{};
@@ -333,6 +340,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         a.v0 = vd;
         R_HIP(launch_front(a, stream));
     }
+    if (front_ev) R_HIP(hipEventRecord(r->front_ev[1], stream));
     R_HIP(stage(1));
     R_HIP(mark("front"));
     // record numbering: block offsets + the total, then each record's staging slot
@@ -387,7 +395,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         if (stages) {
             pb = r->ev[5];
             pe = r->ev[6];
-        } else if (r->timing >= GR_TIMING_SORT_PASSES) {
+        } else if (r->timing == GR_TIMING_SORT_PASSES || r->timing == GR_TIMING_KERNELS) {
             pb = r->pass_ev[0];
             pe = r->pass_ev[1];
         }
@@ -474,7 +482,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
 }
 
 gr_status gr_renderer_set_timing(gr_renderer* r, int32_t level) {
-    if (!r || level < GR_TIMING_NONE || level > GR_TIMING_KERNELS)
+    if (!r || level < GR_TIMING_NONE || level > GR_TIMING_FRONT)
         return rfail(GR_ERR_INVALID_ARGUMENT, "gr_renderer_set_timing: bad argument");
     r->timing = level;
     return GR_OK;
@@ -492,6 +500,12 @@ gr_status gr_renderer_kernel_times(gr_renderer* r, gr_kernel_times* out) {
         out->tile_sort_pass_ms = ms;
     } else {
         out->tile_sort_pass_ms = -1.0;
+    }
+    out->front_ms = -1.0;
+    if (r->front_timed) {
+        float ms = 0.0f;
+        R_HIP(hipEventElapsedTime(&ms, r->front_ev[0], r->front_ev[1]));
+        out->front_ms = ms;
     }
     // mark k closes the work named mark_name[k] that began at mark k - 1
     out->kernels = 0;

@@ -754,8 +754,14 @@ static unsigned grid_for(int64_t n) {
 // ---------------------------------------------------------------------------
 // Front end: SH colour + projection + compaction into build records, one pass (FrontArgs).
 // ---------------------------------------------------------------------------
+// Waves per SIMD the compiler is asked to fit into its register budget (0: its own choice);
+// a build-time override for sweeps.
+#ifndef GR_FRONT_WPE
+#define GR_FRONT_WPE 0
+#endif
 template <int DEG, typename T>
-__global__ __launch_bounds__(kFrontThreads) void front_kernel(FrontArgs a) {
+__global__ __launch_bounds__(kFrontThreads) __attribute__((amdgpu_waves_per_eu(GR_FRONT_WPE))) void front_kernel(
+    FrontArgs a) {
     constexpr int NC = 3 * (DEG + 1) * (DEG + 1);
     constexpr int WV = kFrontThreads / 64;
     __shared__ uint64_t s_bits[kFrontMaxViews][WV];

@@ -13,21 +13,23 @@ serving workloads (C4 Poisson sweep, C5 bursty under its HBM budget) and the
 CPU oracle timed on a bounded sample of the same workload.
 
 Timed loops: the headline frames record no per-stage events (stats off); the
-only events in them are the two around the tile-sort passes that the roofline
+only events in them are the two around the front-end kernel that the roofline
 object reads.  A second loop renders the same frames with per-stage stats
-(RenderStats) and a third with an event after every kernel (the per-kernel
-table); both are reported beside the headline, never as it.
+(RenderStats), a third with an event after every kernel (the per-kernel table)
+and a fourth with events around the tile-sort passes; all are reported beside
+the headline, never as it.
 
-Roofline: the dominant stage is the tile-bit onesweep sort (`onesweep_keys_kernel`), two launches per
-frame (one launch is shorter than SH, record_count or raster, but the pair
-takes the most time of any stage).  The renderer's tile list holds 8-byte
-entries (Gaussian index << 32 | tile id; the entries are in depth order before
-the tile sort, so the depth word needs no storage), sorted on the tile bits,
-and a pass reads and writes each entry once: 16 B per tile-list entry, so
-achieved = 16 * K / (average pass duration), the duration measured with HIP
-events around the passes on the renderer's stream in every timed frame.
-`traffic` is the measured HBM bytes per pass launch, taken from the committed
-rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated).
+Roofline: the dominant kernel is the front end (`front_kernel`: SH colour,
+projection and compaction into build records in one pass), the longest launch
+of the frame.  It is HBM-bound: per Gaussian it reads position, scale/rotation
+and 48 SH coefficients (224 B) and writes colour, clip xy, clip z and conic
+(44 B); per visible Gaussian it writes a 32-byte build record.  achieved =
+(268 N + 32 V) / (front duration), the duration measured with HIP events around
+the launch on the renderer's stream in every timed frame.  `traffic` is the
+measured HBM bytes per launch, from the committed rocprofv3 FETCH_SIZE /
+WRITE_SIZE summary of this workload (copy-calibrated).  The tile-bit sort pass
+(`onesweep_keys_kernel`, two launches per frame, 16 B per tile-list entry)
+gets the same treatment in `roofline_tile_sort_pass`, timed in the fourth loop.
 """
 from __future__ import annotations
 
@@ -71,7 +73,9 @@ def parse():
     ap.add_argument("--serving-accuracy", type=int, default=1,
                     help="one served view per resolution against the FP64 render")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU oracle sample length (frames cycle poses)")
-    ap.add_argument("--traffic-profile", default=str(ROOT / "profiles" / "tile_sort_pass_6M_1080p.md"),
+    ap.add_argument("--traffic-profile", default=str(ROOT / "profiles" / "front_6M_1080p.md"),
+                    help="rocprofv3 summary holding the measured HBM bytes per front-end launch")
+    ap.add_argument("--sort-traffic-profile", default=str(ROOT / "profiles" / "tile_sort_pass_6M_1080p.md"),
                     help="rocprofv3 summary holding the measured HBM bytes per tile-sort pass")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 launches), to scale "
@@ -89,12 +93,12 @@ def cpu_model() -> str:
     return platform.processor() or platform.machine()
 
 
-def profiled_traffic(path: str):
-    """HBM bytes per tile-sort pass launch from the summary line
-    `traffic_bytes_per_launch tile-sort pass = <bytes>` (None if absent)."""
+def profiled_traffic(path: str, what: str):
+    """HBM bytes per launch from the summary line `traffic_bytes_per_launch <what> = <bytes>`
+    (None if absent)."""
     try:
         for line in Path(path).read_text().splitlines():
-            if line.strip().startswith("traffic_bytes_per_launch tile-sort pass"):
+            if line.strip().startswith(f"traffic_bytes_per_launch {what} ="):
                 return int(float(line.split("=")[1].strip()))
     except OSError:
         return None
@@ -348,18 +352,18 @@ def main():
     optr = out.data_ptr()
     print(f"[bench] scene ready ({gen_s:.1f} s generate, {upload_s:.1f} s upload)", file=sys.stderr, flush=True)
 
-    # headline: stats off, events only around the tile-sort passes (the live roofline)
-    r.set_timing("sort_passes")
+    # headline: stats off, events only around the front-end kernel (the live roofline)
+    r.set_timing("front")
     for i in range(args.warmup):
         r.render(cams[i % len(cams)], W, H, out=optr, stats=False)
     torch.cuda.synchronize()
-    ks, pass_ms = [], []
+    ks, front_ms = [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
         r.render(cams[i % len(cams)], W, H, out=optr, stats=False)
         kt = r.kernel_times()
         ks.append(kt["tile_list_size"])
-        pass_ms.append(kt["tile_sort_pass_ms"])
+        front_ms.append(kt["front_ms"])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     fps = args.steps / elapsed
@@ -385,8 +389,16 @@ def main():
     for i in range(args.kernel_frames):
         r.render(cams[i % len(cams)], W, H, out=optr, stats=False)
         kframes.append(r.kernel_times())
-    r.set_timing("none")
     ktab, V = kernel_table(kframes, N, float(np.mean([f["tile_list_size"] for f in kframes])), T, passes, W, H)
+    # the tile-sort passes (two events around them), the same frames
+    r.set_timing("sort_passes")
+    pass_ms, pass_k = [], []
+    for i in range(args.steps):
+        r.render(cams[i % len(cams)], W, H, out=optr, stats=False)
+        kt = r.kernel_times()
+        pass_ms.append(kt["tile_sort_pass_ms"])
+        pass_k.append(kt["tile_list_size"])
+    r.set_timing("none")
 
     # the same frames with the image copied to host memory (PCIe-inclusive; not the metric)
     host_n = min(args.steps, 16)
@@ -407,15 +419,26 @@ def main():
         "sort_ms": K * 16.0 * passes,
     }
     gbs = {k: alg[k] / (mean[k] * 1e-3) / 1e9 for k in alg if mean[k] > 0}
-    # dominant kernel: one tile-bit onesweep pass; per-entry bytes x the entries of one launch
-    # tile_sort_pass_ms spans all passes of a frame (back to back between two events)
-    pass_avg_ms = float(np.mean(pass_ms)) / passes if passes > 0 else float("nan")
-    achieved = 16.0 * K / (pass_avg_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": "onesweep_keys_kernel (tile-bit sort pass)",
+    # dominant kernel: the front end; bytes per launch over its HIP-event duration in the timed frames
+    front_avg_ms = float(np.mean(front_ms))
+    front_bytes = 268.0 * N + 32.0 * V
+    achieved = front_bytes / (front_avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": "front_kernel (SH colour + projection + compaction, one pass)",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.traffic_profile),
-            "algorithmic_bytes_per_launch": int(16.0 * K), "avg_launch_us": round(pass_avg_ms * 1e3, 2),
-            "launches_per_frame": passes}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.traffic_profile, "front"),
+            "algorithmic_bytes_per_launch": int(front_bytes), "avg_launch_us": round(front_avg_ms * 1e3, 2),
+            "launches_per_frame": 1}
+    # tile-bit onesweep pass: per-entry bytes x the entries of one launch; tile_sort_pass_ms spans
+    # all passes of a frame (back to back between two events)
+    pass_avg_ms = float(np.mean(pass_ms)) / passes if passes > 0 else float("nan")
+    Kp = float(np.mean(pass_k))
+    pach = 16.0 * Kp / (pass_avg_ms * 1e-3) / 1e9
+    roof_sort = {"bound": "hbm", "kernel": "onesweep_keys_kernel (tile-bit sort pass)",
+                 "achieved": round(pach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(pach / HBM_PEAK_GBS, 4),
+                 "traffic": profiled_traffic(args.sort_traffic_profile, "tile-sort pass"),
+                 "algorithmic_bytes_per_launch": int(16.0 * Kp), "avg_launch_us": round(pass_avg_ms * 1e3, 2),
+                 "launches_per_frame": passes}
     print(f"[bench] frames: {fps:.1f}/s headline, {stats_fps:.1f}/s with stats", file=sys.stderr, flush=True)
     # the reference's frame sequence on the same kernels (stage-level C-ABI), for comparison
     r.close()
@@ -493,6 +516,7 @@ def main():
         "host_output_frames_per_s": round(host_fps, 2),
         "stage_path_frames_per_s": round(stage_fps, 2),
         "roofline": roof,
+        "roofline_tile_sort_pass": roof_sort,
         "cpu_baseline": cpu,
         "serving": serving or None,
         "setup_s": {"generate": round(gen_s, 2), "upload": round(upload_s, 2)},

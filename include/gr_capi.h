@@ -355,10 +355,14 @@ GR_API gr_status gr_renderer_frame_view(gr_renderer* r, gr_frame_view* out);
  *   GR_TIMING_SORT_PASSES                 two events around the tile-sort passes
  *   GR_TIMING_KERNELS                     an event after every kernel (or small group of
  *                                         kernels) of the frame, and the visible count
+ *   GR_TIMING_FRONT                       two events around the front-end kernel (SH
+ *                                         colour, projection and compaction), the frame's
+ *                                         longest
  * A frame with stats == NULL at GR_TIMING_NONE (the default) records none. */
 #define GR_TIMING_NONE 0
 #define GR_TIMING_SORT_PASSES 1
 #define GR_TIMING_KERNELS 2
+#define GR_TIMING_FRONT 3
 GR_API gr_status gr_renderer_set_timing(gr_renderer* r, int32_t level);
 
 /* Per-kernel timing of the last frame, from HIP events on the renderer's
@@ -371,7 +375,8 @@ GR_API gr_status gr_renderer_set_timing(gr_renderer* r, int32_t level);
  * "numbering" (record numbers and staging slots), "record_count",
  * "depth_sort", "offsets", "emit", "tile_sort_scan", "tile_sort_passes",
  * "ranges", "raster") and
- * visible the number of Gaussians with a non-empty tile rect (else -1). */
+ * visible the number of Gaussians with a non-empty tile rect (else -1).  At
+ * GR_TIMING_FRONT, front_ms is the front-end kernel's time (else -1). */
 #define GR_KERNEL_TIMES_MAX 24
 typedef struct gr_kernel_times {
     double tile_sort_pass_ms;
@@ -381,6 +386,7 @@ typedef struct gr_kernel_times {
     int64_t visible;
     double kernel_ms[GR_KERNEL_TIMES_MAX];
     char kernel_name[GR_KERNEL_TIMES_MAX][24];
+    double front_ms;
 } gr_kernel_times;
 GR_API gr_status gr_renderer_kernel_times(gr_renderer* r, gr_kernel_times* out);
 

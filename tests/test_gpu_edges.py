@@ -434,8 +434,8 @@ def test_raster_rejects_missing_buffers(torch_cuda):
 
 def test_timing_levels_render_the_same_frame(torch_cuda):
     """A frame without stats or events, one timing the tile-sort passes, one with per-kernel
-    marks and one with per-stage stats render the same image and tile list; the timed ones
-    report their intervals."""
+    marks, one timing the front-end kernel and one with per-stage stats render the same image
+    and tile list; the timed ones report their intervals."""
     say("timing levels")
     s = gr.generate_scene(2, 200_000, 3, 71)
     c = s["centroid"]
@@ -445,7 +445,7 @@ def test_timing_levels_render_the_same_frame(torch_cuda):
     base, st = r.render(cam, 1280, 720)
     keys0, vals0, _ = frame_list(r)
     assert st["total_ms"] > 0 and st["tile_list_size"] == len(keys0)
-    for level in ("none", "sort_passes", "kernels"):
+    for level in ("none", "sort_passes", "kernels", "front"):
         r.set_timing(level)
         img, st = r.render(cam, 1280, 720, stats=False)
         assert st is None
@@ -453,10 +453,11 @@ def test_timing_levels_render_the_same_frame(torch_cuda):
         kt = r.kernel_times()
         keys, vals, _ = frame_list(r)
         assert np.array_equal(keys, keys0) and np.array_equal(vals, vals0)
-        if level == "none":
+        if level in ("none", "front"):
             assert kt["tile_sort_pass_ms"] == -1.0 and not kt["kernel_ms"]
         else:
             assert kt["tile_sort_pass_ms"] > 0
+        assert (kt["front_ms"] > 0) if level == "front" else (kt["front_ms"] == -1.0)
         if level == "kernels":
             names = list(kt["kernel_ms"])
             for want in ("front", "numbering", "record_count", "depth_sort", "emit", "tile_sort_passes", "raster"):
