@@ -80,7 +80,19 @@ struct ProjectArgs {
 // Gaussians, measured.)
 constexpr int kFrontThreads = 256;
 constexpr int kFrontMaxViews = 64;
+// Words to set before a kernel sequence runs: the front end sets them in passing (grid-stride
+// over its lanes) so that the frame needs no separate fill or clear launches -- the tile ranges
+// (-1 for tiles without entries) and a sort's zeroed scratch (radix_sort_zero_spans).
+struct FillSpan {
+    uint32_t* p;
+    int64_t n;
+    uint32_t value;
+};
+constexpr int kFrontFills = 5;
+
 struct FrontArgs {
+    FillSpan fills[kFrontFills];    // set before any other work of the frame reads them
+    int32_t nfills;
     const float4* position;
     const float4* scale_rotation;   // FP32 form
     const uint2* packed;            // precision form (scale + opacity, 16-bit)
@@ -262,13 +274,18 @@ constexpr int kSortHistCopyStride = 8 * 256;
 // Used for the renderer's tile-list entries (Gaussian index << 32 | tile id),
 // sorted on the tile bits.
 hipError_t radix_sort_prepare(int32_t count, void* temp, size_t temp_bytes, uint32_t** hist, hipStream_t stream);
+// The words radix_sort_prepare (and the pair sorts' own first step) zero, as two spans, for a
+// caller that zeroes them in an earlier kernel of its own (FrontArgs fills); *hist as above.
+hipError_t radix_sort_zero_spans(int32_t count, void* temp, size_t temp_bytes, FillSpan spans[2], uint32_t** hist);
 hipError_t radix_sort_keys_prepared(uint64_t* keys[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
                                     int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes,
                                     hipStream_t stream, hipEvent_t passes_begin = nullptr,
                                     hipEvent_t passes_end = nullptr);
 // The same for 32-bit keys (depth words); same temp layout and size.
+// zeroed: the spans radix_sort_zero_spans names for `count` were zeroed by the caller after the
+// scratch's last use (the sort then skips that step).
 hipError_t radix_sort_pairs_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                                 const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
-                                size_t temp_bytes, hipStream_t stream);
+                                size_t temp_bytes, hipStream_t stream, bool zeroed = false);
 
 }  // namespace gr

@@ -126,8 +126,10 @@ struct gr_renderer {
     DevBuf<float4> records;         // staged build records of the visible Gaussians (2 x float4 each)
     DevBuf<uint32_t> blocks;        // front-end block counts (then offsets), the record total, scan scratch
     DevBuf<uint32_t> slots;         // staging slot of each record
-    void* sort_temp = nullptr;
+    void* sort_temp = nullptr;      // the tile-bit sort's scratch
     size_t sort_temp_bytes = 0;
+    void* dsort_temp = nullptr;     // the depth sort's scratch (both are zeroed by the front end)
+    size_t dsort_temp_bytes = 0;
     int32_t sel = 0;
     int64_t last_k = 0;
     int32_t last_w = 0, last_h = 0;
@@ -171,6 +173,7 @@ gr_status gr_renderer_destroy(gr_renderer* r) {
     r->rec_offsets.release(); r->emit_recs.release();
     r->records.release(); r->blocks.release(); r->slots.release();
     if (r->sort_temp) (void)hipFree(r->sort_temp);
+    if (r->dsort_temp) (void)hipFree(r->dsort_temp);
     for (auto& e : r->ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : r->pass_ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : r->front_ev) if (e) (void)hipEventDestroy(e);
@@ -287,8 +290,16 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     R_CHECK(r->slots.ensure(nn));
     // one scratch serves the depth sort and the tile-bit sort; sized before anything is enqueued
     const int32_t capacity = (int32_t)std::min<size_t>(std::min(r->entries[0].cap, r->entries[1].cap), (1u << 30) - 1);
-    R_CHECK(ensure_temp(r->sort_temp, r->sort_temp_bytes,
-                        std::max(radix_sort_temp_bytes((int32_t)nn, 0, 32), radix_sort_temp_bytes(capacity, 0, 32))));
+    R_CHECK(ensure_temp(r->sort_temp, r->sort_temp_bytes, radix_sort_temp_bytes(capacity, 0, 32)));
+    R_CHECK(ensure_temp(r->dsort_temp, r->dsort_temp_bytes, radix_sort_temp_bytes((int32_t)nn, 0, 32)));
+    // what the frame needs set before it runs -- the tile ranges to -1 (tiles without entries) and
+    // both sorts' histograms, tickets and first look-back regions to 0 -- the front end sets in
+    // passing (no fill launches)
+    FillSpan dzero[2], tzero[2];
+    uint32_t* dhist = nullptr;
+    uint32_t* thist = nullptr;
+    R_HIP(radix_sort_zero_spans((int32_t)nn, r->dsort_temp, r->dsort_temp_bytes, dzero, &dhist));
+    R_HIP(radix_sort_zero_spans(capacity, r->sort_temp, r->sort_temp_bytes, tzero, &thist));
 
     ViewDesc vd{};
     vd.cam = camera_args(*cam);
@@ -321,6 +332,12 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     {
         // SH colour + projection + compaction of the visible Gaussians into build records
         FrontArgs a{};
+        a.fills[0] = {reinterpret_cast<uint32_t*>(r->tile_range.p), 2 * (int64_t)tiles, 0xffffffffu};
+        a.fills[1] = dzero[0];
+        a.fills[2] = dzero[1];
+        a.fills[3] = tzero[0];
+        a.fills[4] = tzero[1];
+        a.nfills = 5;
         a.position = r->position.p;
         a.scale_rotation = r->scale_rot.p;
         a.packed = r->packed.p;
@@ -371,7 +388,8 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     {
         uint32_t* dk[2] = {r->dkeys[0].p, r->dkeys[1].p};
         int32_t* dv[2] = {r->dorder[0].p, r->dorder[1].p};
-        R_HIP(radix_sort_pairs_u32(dk, dv, &dsel, n, vis_total, 0, 32, r->sort_temp, r->sort_temp_bytes, stream));
+        R_HIP(radix_sort_pairs_u32(dk, dv, &dsel, n, vis_total, 0, 32, r->dsort_temp, r->dsort_temp_bytes, stream,
+                                   /*zeroed=*/true));
     }
     R_HIP(stage(4));
     R_HIP(mark("depth_sort"));
@@ -382,7 +400,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     R_HIP(mark("offsets"));
     b.record_offsets = r->rec_offsets.p;
     const int tile_bits = tile_bits_for(tiles);
-    R_HIP(radix_sort_prepare(capacity, r->sort_temp, r->sort_temp_bytes, &b.tile_hist, stream));
+    b.tile_hist = thist;   // zeroed by the front end
     b.tile_passes = (tile_bits + 7) / 8;
     b.tile_digit_bits = sort_digit_bits(0, tile_bits);
     R_HIP(launch_build_emit(b, stream));
@@ -418,8 +436,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         }
     }
     r->sel = sel;
-    R_HIP(launch_fill_i32(r->tile_range.p, -1, 2 * (int64_t)tiles, stream));
-    {
+    {   // tiles without entries keep the -1 the front end set
         RangeArgs ra{};
         ra.keys = r->entries[sel].p;
         ra.tile_shift = 0;

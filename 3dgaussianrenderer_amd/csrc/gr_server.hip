@@ -503,7 +503,17 @@ gr_status enqueue_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     }
     const int nb = front_blocks(N);
     uint32_t* vis = w.blocks.as<uint32_t>() + nb;   // records of the batch
+    // the front end sets the tile ranges to -1 (tiles without entries) and zeroes the depth
+    // sort's histograms, tickets and first look-back region in passing
+    FillSpan dzero[2];
+    uint32_t* dhist = nullptr;
+    S_HIP(radix_sort_zero_spans((int32_t)std::max<int64_t>((int64_t)V * N, 1), w.dsort_temp.p, w.dsort_temp.bytes,
+                                dzero, &dhist));
     FrontArgs f{};
+    f.fills[0] = {w.ranges.as<uint32_t>(), 2 * (int64_t)sl.tiles_total, 0xffffffffu};
+    f.fills[1] = dzero[0];
+    f.fills[2] = dzero[1];
+    f.nfills = 3;
     f.position = sc.position.as<float4>();
     f.scale_rotation = sc.scale_rot.as<float4>();
     f.packed = sc.packed.as<uint2>();
@@ -531,7 +541,8 @@ gr_status enqueue_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     uint32_t* dk[2] = {w.dkeys0.as<uint32_t>(), w.dkeys1.as<uint32_t>()};
     int32_t* dv[2] = {w.dorder0.as<int32_t>(), w.dorder1.as<int32_t>()};
     int32_t dsel = 0;
-    S_HIP(radix_sort_pairs_u32(dk, dv, &dsel, V * N, vis, 0, 32, w.dsort_temp.p, w.dsort_temp.bytes, sl.stream));
+    S_HIP(radix_sort_pairs_u32(dk, dv, &dsel, V * N, vis, 0, 32, w.dsort_temp.p, w.dsort_temp.bytes, sl.stream,
+                               /*zeroed=*/true));
     uint32_t* counts = w.counts.as<uint32_t>();
     const int vn = V * N;
     // output offsets: the counts carried by the sorted words, scanned in depth order (K) and
@@ -558,7 +569,7 @@ gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
     int32_t sel = 0;
     S_HIP(radix_sort_keys_prepared(kb, &sel, (int32_t)sl.capacity, counts, 0, sl.tile_bits, w.sort_temp.p,
                                    w.sort_temp.bytes, sl.stream));
-    S_HIP(launch_fill_i32(w.ranges.as<int32_t>(), -1, 2 * (int64_t)sl.tiles_total, sl.stream));
+    // tiles without entries keep the -1 the front end set in phase A
     RangeArgs r{};
     r.keys = kb[sel];
     r.tile_shift = 0;

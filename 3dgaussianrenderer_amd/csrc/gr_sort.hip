@@ -680,14 +680,16 @@ static hipError_t sort_passes_t(KeyT* keys[2], int32_t* values[2], int32_t* sel,
 template <typename KeyT>
 static hipError_t sort_pairs_t(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                                const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
-                               size_t temp_bytes, hipStream_t s) {
+                               size_t temp_bytes, hipStream_t s, bool zeroed = false) {
     if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
     hipError_t e = sort_check<KeyT>(count, begin_bit, end_bit, temp, temp_bytes);
     if (e != hipSuccess) return e;
     const SortLayout l = layout_for<KeyT>(count);
     char* tb = static_cast<char*>(temp);
-    e = sort_prepare(l, tb, s);
-    if (e != hipSuccess) return e;
+    if (!zeroed) {
+        e = sort_prepare(l, tb, s);
+        if (e != hipSuccess) return e;
+    }
     int hist_blocks = (count + kHistChunk - 1) / kHistChunk;
     if (hist_blocks > kHistBlocks) hist_blocks = kHistBlocks;
     hipLaunchKernelGGL(sort_hist_kernel<KeyT>, dim3(hist_blocks), dim3(kHistThreads), 0, s, keys[*sel], count,
@@ -704,8 +706,25 @@ hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel,
 
 hipError_t radix_sort_pairs_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                                 const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
-                                size_t temp_bytes, hipStream_t s) {
-    return sort_pairs_t<uint32_t>(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
+                                size_t temp_bytes, hipStream_t s, bool zeroed) {
+    return sort_pairs_t<uint32_t>(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s,
+                                  zeroed);
+}
+
+// The words sort_prepare zeroes: the histogram copies and the partition tickets (one span), and
+// the first look-back region's slots of the sort's partitions.
+static void zero_spans(const SortLayout& l, char* tb, FillSpan spans[2]) {
+    spans[0] = {reinterpret_cast<uint32_t*>(tb + l.hist_off),
+                (int64_t)((l.ticket_off + kMaxPasses * sizeof(uint32_t) - l.hist_off) / sizeof(uint32_t)), 0u};
+    spans[1] = {reinterpret_cast<uint32_t*>(tb + l.status_off[0]), (int64_t)l.partitions * kRadix, 0u};
+}
+
+hipError_t radix_sort_zero_spans(int32_t count, void* temp, size_t temp_bytes, FillSpan spans[2], uint32_t** hist) {
+    const SortLayout l = layout_for<uint64_t>(count);
+    if (temp_bytes < l.total || temp == nullptr || hist == nullptr || spans == nullptr) return hipErrorInvalidValue;
+    *hist = reinterpret_cast<uint32_t*>(static_cast<char*>(temp) + l.hist_off);
+    zero_spans(l, static_cast<char*>(temp), spans);
+    return hipSuccess;
 }
 
 hipError_t radix_sort_prepare(int32_t count, void* temp, size_t temp_bytes, uint32_t** hist, hipStream_t s) {

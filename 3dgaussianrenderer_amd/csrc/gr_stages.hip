@@ -754,6 +754,17 @@ static unsigned grid_for(int64_t n) {
 // ---------------------------------------------------------------------------
 // Front end: SH colour + projection + compaction into build records, one pass (FrontArgs).
 // ---------------------------------------------------------------------------
+// The frame's fills (FrontArgs::fills), grid-stride over the front end's lanes.
+__device__ __forceinline__ void front_fills(const FrontArgs& a, int64_t gid, int64_t lanes) {
+    for (int f = 0; f < a.nfills; ++f) {
+        const FillSpan sp = a.fills[f];
+        for (int64_t j = gid; j < sp.n; j += lanes) sp.p[j] = sp.value;
+    }
+}
+__global__ __launch_bounds__(kFrontThreads) void front_fills_kernel(FrontArgs a) {
+    front_fills(a, (int64_t)blockIdx.x * kFrontThreads + threadIdx.x, (int64_t)gridDim.x * kFrontThreads);
+}
+
 // Waves per SIMD the compiler is asked to fit into its register budget (0: its own choice);
 // a build-time override for sweeps.
 #ifndef GR_FRONT_WPE
@@ -768,6 +779,7 @@ __global__ __launch_bounds__(kFrontThreads) __attribute__((amdgpu_waves_per_eu(G
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int blk = blockIdx.x;
     const int i = blk * kFrontThreads + t;
+    front_fills(a, (int64_t)i, (int64_t)gridDim.x * kFrontThreads);
     const bool in = i < a.count;
     float4 pd = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (in) pd = a.position[i];
@@ -855,7 +867,17 @@ static hipError_t launch_front_t(const FrontArgs& a, unsigned blocks, hipStream_
 }
 
 hipError_t launch_front(const FrontArgs& a, hipStream_t s) {
-    if (a.count <= 0) return hipSuccess;
+    if (a.nfills < 0 || a.nfills > kFrontFills) return hipErrorInvalidValue;
+    for (int f = 0; f < a.nfills; ++f)
+        if (a.fills[f].n < 0 || (a.fills[f].n > 0 && !a.fills[f].p)) return hipErrorInvalidValue;
+    if (a.count <= 0) {
+        // no Gaussians: the fills alone
+        if (a.nfills > 0) {
+            hipLaunchKernelGGL(front_fills_kernel, dim3(256), dim3(kFrontThreads), 0, s, a);
+            return hipGetLastError();
+        }
+        return hipSuccess;
+    }
     if (a.nviews < 1 || a.nviews > kFrontMaxViews || !a.records || !a.block_counts ||
         (a.nviews > 1 && !a.ellipse) || (a.degree > 0 && (!a.sh || !a.color)))
         return hipErrorInvalidValue;

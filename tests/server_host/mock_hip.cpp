@@ -236,7 +236,10 @@ hipError_t launch_front(const FrontArgs& a, hipStream_t s) {
     if (a.nviews < 1 || a.nviews > kFrontMaxViews || !a.records || !a.block_counts ||
         (a.nviews > 1 && !a.ellipse) || (a.degree > 0 && (!a.sh || !a.color)))
         return hipErrorInvalidValue;
+    if (a.nfills < 0 || a.nfills > kFrontFills) return hipErrorInvalidValue;
     run_or_record(s, [=] {
+        for (int f = 0; f < a.nfills; ++f)
+            for (int64_t j = 0; j < a.fills[f].n; ++j) a.fills[f].p[j] = a.fills[f].value;
         const size_t n = (size_t)a.count, vn = (size_t)a.nviews * n;
         rd(a.position, n * 16);
         if (a.precision) rd(a.packed, n * 8); else rd(a.scale_rotation, n * 16);
@@ -457,9 +460,22 @@ hipError_t radix_sort_keys_prepared(uint64_t* keys[2], int32_t* sel, int32_t cou
 
 
 // moves the data like an even (end-begin <= 32 bits: 4 passes) or odd number of passes would
+// the words radix_sort_prepare zeroes (the histograms the mock sorts use), then the rest of the
+// scratch as the look-back region: every span lies inside the scratch
+hipError_t radix_sort_zero_spans(int32_t count, void* temp, size_t temp_bytes, FillSpan spans[2], uint32_t** hist) {
+    if (count < 1) count = 1;
+    if (temp_bytes < (size_t)count * 4 + 4096 || !temp || !hist) return hipErrorInvalidValue;
+    *hist = static_cast<uint32_t*>(temp);
+    const int64_t words = (int64_t)(temp_bytes / 4);
+    const int64_t head = std::min<int64_t>(words, 8 * 256);
+    spans[0] = {static_cast<uint32_t*>(temp), head, 0u};
+    spans[1] = {static_cast<uint32_t*>(temp) + head, words - head, 0u};
+    return hipSuccess;
+}
+
 hipError_t radix_sort_pairs_u32(uint32_t* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                                 const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, void* temp,
-                                size_t temp_bytes, hipStream_t s) {
+                                size_t temp_bytes, hipStream_t s, bool) {
     return mock_sort(keys, values, sel, count, count_dev, begin_bit, end_bit, temp, temp_bytes, s);
 }
 
