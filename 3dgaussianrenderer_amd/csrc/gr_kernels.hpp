@@ -102,8 +102,8 @@ struct FrontArgs {
     int32_t precision;              // 0 FP32, 1 FP16, 2 BF16
     float4* color;                  // [nviews][count] (degree > 0)
     float2* clip_xy;                // [nviews][count]
-    float* clip_z;
-    float4* ellipse;                // [nviews][count]; NULL allowed when nviews == 1
+    float* clip_z;                  // [nviews][count]; NULL: not stored (the records hold the depth)
+    float4* ellipse;                // [nviews][count]; NULL: not stored (the records hold it)
     float4* conic;
     float4* records;                // [blocks][256 * nviews][2] staged build records
     uint32_t* block_counts;         // [blocks] records of each block

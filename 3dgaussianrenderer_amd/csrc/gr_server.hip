@@ -153,7 +153,7 @@ struct Pending {
 enum class SlotState { Idle, PhaseA, WaitMemory, PhaseB };
 
 struct Workspace {
-    Dev color, xy, z, ell, con;      // [V][N]
+    Dev color, xy, con;              // [V][N]
     Dev blocks;                       // front-end block counts (then offsets), the record count, scan scratch
     Dev dkeys0, dkeys1, dorder0, dorder1;   // [<= V*N] depth words and packed record words of visible Gaussians
     Dev records;                      // [blocks][256 * V][2] float4 staged build records
@@ -171,7 +171,7 @@ struct Workspace {
     Pinned img_host[2];               // image staging, alternating between batches
     uint64_t generation = 0;          // bumps when a pointer a captured graph holds changes
     std::vector<Dev*> devs() {
-        return {&color, &xy, &z, &ell, &con, &blocks, &dkeys0, &dkeys1, &dorder0, &dorder1, &records, &slots,
+        return {&color, &xy, &con, &blocks, &dkeys0, &dkeys1, &dorder0, &dorder1, &records, &slots,
                 &emit_recs, &rec_offsets,
                 &dsort_temp, &counts, &list0, &list1, &sort_temp, &ranges, &images, &views};
     }
@@ -373,8 +373,6 @@ std::vector<PhaseANeed> phase_a_needs(Slot& sl, const Scene& sc, size_t nviews, 
     const int64_t V = (int64_t)nviews, N = sc.n, nb = front_blocks(sc.n);
     return {{&w.color, V * N * 16},
             {&w.xy, V * N * 8},
-            {&w.z, V * N * 4},
-            {&w.ell, V * N * 16},
             {&w.con, V * N * 16},
             {&w.blocks, 4 * (nb + 1 + scan_scratch_words((int32_t)nb))},
             {&w.dkeys0, V * N * 4},
@@ -523,8 +521,8 @@ gr_status enqueue_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     f.precision = sc.precision;
     f.color = w.color.as<float4>();
     f.clip_xy = w.xy.as<float2>();
-    f.clip_z = w.z.as<float>();
-    f.ellipse = w.ell.as<float4>();
+    f.clip_z = nullptr;    // the build records carry each visible Gaussian's depth and ellipse
+    f.ellipse = nullptr;
     f.conic = w.con.as<float4>();
     f.records = w.records.as<float4>();
     f.block_counts = w.blocks.as<uint32_t>();

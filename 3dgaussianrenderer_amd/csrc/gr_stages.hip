@@ -808,7 +808,7 @@ __global__ __launch_bounds__(kFrontThreads) __attribute__((amdgpu_waves_per_eu(G
                 const size_t o = (size_t)v * a.count + i;
                 const ProjOut po = project_view(pd, sigma, vd);
                 a.clip_xy[o] = make_float2(po.cx, po.cy);
-                a.clip_z[o] = po.cz;
+                if (a.clip_z) a.clip_z[o] = po.cz;
                 if (a.ellipse) a.ellipse[o] = po.ellipse;
                 a.conic[o] = po.conic;
                 vis = has_tiles(po, vd.geom);
@@ -839,12 +839,13 @@ __global__ __launch_bounds__(kFrontThreads) __attribute__((amdgpu_waves_per_eu(G
             const uint32_t r = base + before + (uint32_t)__popcll(m & lanemask_lt(lane));
             float4 ra = ra0, rb = rb0;
             if (v > 0) {
-                // written by this lane in the loop above
-                const size_t o = (size_t)v * a.count + i;
-                const float2 xy = a.clip_xy[o];
-                const float4 e = a.ellipse[o];
-                ra = make_float4(xy.x, xy.y, e.x, e.y);
-                rb = make_float4(e.z, e.w, __uint_as_float(depth_word(a.clip_z[o])), __int_as_float(v * a.count + i));
+                // the projection of view v again (the same arithmetic, so the same bits), rather
+                // than a store and reload of its ellipse and depth per Gaussian and view
+                const ProjOut po = project_view(pd, world_sigma<T>(pd, a.scale_rotation, a.packed, i),
+                                                view_of(v, a.v0, a.views));
+                ra = make_float4(po.cx, po.cy, po.ellipse.x, po.ellipse.y);
+                rb = make_float4(po.ellipse.z, po.ellipse.w, __uint_as_float(depth_word(po.cz)),
+                                 __int_as_float(v * a.count + i));
             }
             slice[2 * (size_t)r] = ra;
             slice[2 * (size_t)r + 1] = rb;
@@ -878,8 +879,8 @@ hipError_t launch_front(const FrontArgs& a, hipStream_t s) {
         }
         return hipSuccess;
     }
-    if (a.nviews < 1 || a.nviews > kFrontMaxViews || !a.records || !a.block_counts ||
-        (a.nviews > 1 && !a.ellipse) || (a.degree > 0 && (!a.sh || !a.color)))
+    if (a.nviews < 1 || a.nviews > kFrontMaxViews || !a.records || !a.block_counts || !a.clip_xy || !a.conic ||
+        (a.degree > 0 && (!a.sh || !a.color)))
         return hipErrorInvalidValue;
     const unsigned blocks = (unsigned)front_blocks(a.count);
     switch (a.precision) {

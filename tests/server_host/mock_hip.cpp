@@ -233,8 +233,8 @@ hipError_t launch_project(const ProjectArgs& a, hipStream_t s) {
 // Front end: every Gaussian of every view visible; each block's records in (view, index) order
 // in its own staging slice.
 hipError_t launch_front(const FrontArgs& a, hipStream_t s) {
-    if (a.nviews < 1 || a.nviews > kFrontMaxViews || !a.records || !a.block_counts ||
-        (a.nviews > 1 && !a.ellipse) || (a.degree > 0 && (!a.sh || !a.color)))
+    if (a.nviews < 1 || a.nviews > kFrontMaxViews || !a.records || !a.block_counts || !a.clip_xy || !a.conic ||
+        (a.degree > 0 && (!a.sh || !a.color)))
         return hipErrorInvalidValue;
     if (a.nfills < 0 || a.nfills > kFrontFills) return hipErrorInvalidValue;
     run_or_record(s, [=] {
@@ -248,7 +248,7 @@ hipError_t launch_front(const FrontArgs& a, hipStream_t s) {
             touch(a.color, vn * 16);
         }
         touch(a.clip_xy, vn * 8);
-        touch(a.clip_z, vn * 4);
+        if (a.clip_z) touch(a.clip_z, vn * 4);
         if (a.ellipse) touch(a.ellipse, vn * 16);
         touch(a.conic, vn * 16);
         const int nb = front_blocks(a.count);
