@@ -272,7 +272,10 @@ constexpr int kSortHistCopyStride = 8 * 256;
 // runs the passes.  A device-side count above `count` then means the producer
 // did not write the list: the prepared sort leaves the buffers untouched.
 // Used for the renderer's tile-list entries (Gaussian index << 32 | tile id),
-// sorted on the tile bits.
+// sorted on the tile bits [0, end_bit).  ranges (optional; tile_shift 0, begin_bit 0): the
+// sort also writes the tile ranges of the sorted list into ranges->tile_range for tiles
+// [0, ranges->tiles), as launch_ranges would (its keys, size and size_dev are the sort's own);
+// tiles without entries keep what they held, which must be -1 for both words.
 hipError_t radix_sort_prepare(int32_t count, void* temp, size_t temp_bytes, uint32_t** hist, hipStream_t stream);
 // The words radix_sort_prepare (and the pair sorts' own first step) zero, as two spans, for a
 // caller that zeroes them in an earlier kernel of its own (FrontArgs fills); *hist as above.
@@ -280,7 +283,7 @@ hipError_t radix_sort_zero_spans(int32_t count, void* temp, size_t temp_bytes, F
 hipError_t radix_sort_keys_prepared(uint64_t* keys[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
                                     int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes,
                                     hipStream_t stream, hipEvent_t passes_begin = nullptr,
-                                    hipEvent_t passes_end = nullptr);
+                                    hipEvent_t passes_end = nullptr, const RangeArgs* ranges = nullptr);
 // The same for 32-bit keys (depth words); same temp layout and size.
 // zeroed: the spans radix_sort_zero_spans names for `count` were zeroed by the caller after the
 // scratch's last use (the sort then skips that step).

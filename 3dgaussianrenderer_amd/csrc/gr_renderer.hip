@@ -15,8 +15,9 @@
 //   -> emit (records in index order): each record's entries, as 8-byte
 //      (Gaussian index << 32 | tile id) words, at its offset, + tile digit
 //      histograms
-//   -> keys-only onesweep sort of the tile bits (bits [0, tile bits))
-//   -> ranges -> raster -> image and K readback.
+//   -> keys-only onesweep sort of the tile bits (bits [0, tile bits)), whose
+//      last pass also writes the tile ranges
+//   -> raster -> image and K readback.
 // Entries sit in (depth word, Gaussian index) order before the last sort, so
 // the stable tile-bit sort yields exactly the order of the reference's
 // full-key sort of (tile << 32 | depth word) keys emitted in index order,
@@ -400,8 +401,8 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     R_HIP(mark("offsets"));
     b.record_offsets = r->rec_offsets.p;
     const int tile_bits = tile_bits_for(tiles);
-    b.tile_hist = thist;   // zeroed by the front end
     b.tile_passes = (tile_bits + 7) / 8;
+    b.tile_hist = b.tile_passes ? thist : nullptr;   // zeroed by the front end; one tile: no pass
     b.tile_digit_bits = sort_digit_bits(0, tile_bits);
     R_HIP(launch_build_emit(b, stream));
     R_HIP(mark("emit"));
@@ -418,8 +419,13 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
             pe = r->pass_ev[1];
         }
         uint64_t* kb[2] = {r->entries[0].p, r->entries[1].p};
+        // the last pass writes the tile ranges; tiles without entries keep the -1 the front end set
+        RangeArgs ra{};
+        ra.tile_shift = 0;
+        ra.tile_range = r->tile_range.p;
+        ra.tiles = tiles;
         R_HIP(radix_sort_keys_prepared(kb, &sel, capacity, k_total, 0, tile_bits, r->sort_temp, r->sort_temp_bytes,
-                                       stream, pb, pe));
+                                       stream, pb, pe, &ra));
         r->last_passes = capacity > 1 ? (tile_bits + 7) / 8 : 0;
         if (r->last_passes == 0 && pb) {   // no passes ran: the events still bracket the (empty) sort
             R_HIP(hipEventRecord(pb, stream));
@@ -436,17 +442,6 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
         }
     }
     r->sel = sel;
-    {   // tiles without entries keep the -1 the front end set
-        RangeArgs ra{};
-        ra.keys = r->entries[sel].p;
-        ra.tile_shift = 0;
-        ra.tile_range = r->tile_range.p;
-        ra.size_dev = k_total;
-        ra.size = capacity;
-        ra.tiles = tiles;
-        R_HIP(launch_ranges(ra, stream));
-    }
-    R_HIP(mark("ranges"));
     {
         RasterArgs a{};
         a.tile_range = r->tile_range.p;
@@ -483,15 +478,16 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     r->last_h = height;
     if (stats) {
         // events: 0 front end (SH + projection + compaction) 1 count 2,3 depth sort 4 offsets +
-        // emit + the tile sort's histogram scan 5 tile-bit sort passes 6 ranges + raster 7
+        // emit + the tile sort's histogram scan 5 tile-bit sort passes (the last one writes the
+        // tile ranges) 6 raster 7
         float t[8];
         for (int k = 0; k < 7; ++k) R_HIP(hipEventElapsedTime(&t[k], r->ev[k], r->ev[k + 1]));
         R_HIP(hipEventElapsedTime(&t[7], r->ev[0], r->ev[7]));
         stats->sh_eval_ms = t[0];   // the fused front end: SH, projection and compaction
         stats->project_ms = 0.0;    // folded into the front end
         stats->build_tile_ms = t[1] + t[4];   // numbering, count, offsets, emit (+ histogram scan)
-        stats->sort_ms = t[3] + t[5];         // depth sort + tile-bit sort
-        stats->raster_ms = t[6];              // ranges + raster
+        stats->sort_ms = t[3] + t[5];         // depth sort + tile-bit sort (with the tile ranges)
+        stats->raster_ms = t[6];              // raster
         stats->total_ms = t[7];
         stats->tile_list_size = need;
     }

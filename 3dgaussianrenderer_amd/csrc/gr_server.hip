@@ -562,20 +562,18 @@ gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
     S_HIP(radix_sort_prepare((int32_t)sl.capacity, w.sort_temp.p, w.sort_temp.bytes, &b.tile_hist, sl.stream));
     b.tile_passes = (sl.tile_bits + 7) / 8;
     b.tile_digit_bits = sort_digit_bits(0, sl.tile_bits);
+    if (!b.tile_passes) b.tile_hist = nullptr;   // one tile in the batch: no pass, no histograms
     S_HIP(launch_build_emit(b, sl.stream));
     uint64_t* kb[2] = {w.list0.as<uint64_t>(), w.list1.as<uint64_t>()};
     int32_t sel = 0;
-    S_HIP(radix_sort_keys_prepared(kb, &sel, (int32_t)sl.capacity, counts, 0, sl.tile_bits, w.sort_temp.p,
-                                   w.sort_temp.bytes, sl.stream));
-    // tiles without entries keep the -1 the front end set in phase A
+    // the last pass writes the tile ranges; tiles without entries keep the -1 the front end set
+    // in phase A
     RangeArgs r{};
-    r.keys = kb[sel];
     r.tile_shift = 0;
     r.tile_range = w.ranges.as<int32_t>();
-    r.size_dev = counts;
-    r.size = (int32_t)sl.capacity;
     r.tiles = sl.tiles_total;
-    S_HIP(launch_ranges(r, sl.stream));
+    S_HIP(radix_sort_keys_prepared(kb, &sel, (int32_t)sl.capacity, counts, 0, sl.tile_bits, w.sort_temp.p,
+                                   w.sort_temp.bytes, sl.stream, nullptr, nullptr, &r));
     RasterArgs ra{};
     ra.tile_range = w.ranges.as<int32_t>();
     ra.entries = kb[sel];

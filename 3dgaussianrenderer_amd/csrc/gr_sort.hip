@@ -447,6 +447,16 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::
 // the owner of the earliest unresolved partition never waits on an unresolved one, so it
 // finishes it.  The digit tables are RW = 2^RB wide (7-bit digits fit more keys per partition
 // in the LDS that keeps two workgroups on a CU).
+//
+// Tile ranges (ranges != NULL, last pass of a tile-list sort: the key's low 32 bits are the tile
+// id, which holds every sorted bit): the output is sorted by tile, and the entries of one
+// partition's digit run sit contiguously in the output in LDS order, so an entry whose LDS
+// neighbour has another tile (or none) is the first or last of its tile within that run.  Such
+// an entry lowers its tile's start to its position (atomicMin, as unsigned: the ranges hold -1
+// beforehand) or raises its tile's end to its position + 1 (atomicMax); the runs of one tile in
+// several partitions are adjacent in the output, so the minimum and maximum are the tile's
+// range.  Tiles without entries keep -1.  Every write is an atomic, so writes to one tile from
+// workgroups on different XCDs combine.
 template <int THREADS, int ITEMS, int RB>
 struct KeysShape {
     static constexpr int kThreads = THREADS;
@@ -471,7 +481,7 @@ template <class S>
 __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(4))) void onesweep_keys_kernel(
     const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout, int32_t n, const uint32_t* count_dev, int strict,
     int shift, int bits, const uint32_t* __restrict__ digit_off, uint32_t* status, uint32_t* status_next,
-    uint32_t* ticket, const uint32_t* trivial) {
+    uint32_t* ticket, const uint32_t* trivial, int32_t* ranges, int32_t ntiles) {
     constexpr int T = S::kThreads, WV = S::kWaves, IT = S::kItems, TILE = S::kTile;
     constexpr int RW = 1 << S::kRadixBits;
     __shared__ uint32_t s_part;
@@ -500,10 +510,20 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(4))
     if (X >= parts) return;
     if (*trivial) {
         // every key has the same digit: the pass is the identity, partitions are copied through
+        // (ranges: each boundary is found against the neighbours in the list itself, one plain
+        // store per start and end)
         for (;;) {
             const int64_t base = (int64_t)X * TILE, end = min((int64_t)n, base + TILE);
             if (status_next && t < kRadix) status_next[(size_t)X * kRadix + t] = 0u;
-            for (int64_t i = base + t; i < end; i += T) kout[i] = kin[i];
+            for (int64_t i = base + t; i < end; i += T) {
+                const uint64_t k = kin[i];
+                kout[i] = k;
+                const uint32_t tile = (uint32_t)k;
+                if (ranges && tile < (uint32_t)ntiles) {
+                    if (i == 0 || (uint32_t)kin[i - 1] != tile) ranges[2 * (size_t)tile] = (int32_t)i;
+                    if (i + 1 == n || (uint32_t)kin[i + 1] != tile) ranges[2 * (size_t)tile + 1] = (int32_t)(i + 1);
+                }
+            }
             X = take();
             if (X >= parts) return;
         }
@@ -562,7 +582,19 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(4))
         const int nvalid = (int)min((int64_t)TILE, (int64_t)n - (int64_t)X * TILE);
         for (int j = t; j < nvalid; j += T) {
             const uint64_t k = s_keys[j];
-            kout[s_gbase[(uint32_t)(k >> shift) & dmask] + (uint32_t)j] = k;
+            const uint32_t o = s_gbase[(uint32_t)(k >> shift) & dmask] + (uint32_t)j;
+            kout[o] = k;
+            if (ranges) {
+                // equal tiles have equal digits: a neighbour with the same tile is in the same run
+                // (both neighbours read unconditionally, clamped to the partition)
+                const uint32_t tile = (uint32_t)k;
+                const uint32_t tp = (uint32_t)s_keys[j > 0 ? j - 1 : j];
+                const uint32_t tn = (uint32_t)s_keys[j + 1 < nvalid ? j + 1 : j];
+                if (tile < (uint32_t)ntiles) {
+                    if (j == 0 || tp != tile) atomicMin(reinterpret_cast<uint32_t*>(ranges) + 2 * (size_t)tile, o);
+                    if (j + 1 == nvalid || tn != tile) atomicMax(ranges + 2 * (size_t)tile + 1, (int32_t)(o + 1u));
+                }
+            }
         }
         if (!haveY) return;
         __syncthreads();
@@ -627,7 +659,8 @@ static hipError_t sort_check(int32_t count, int32_t begin_bit, int32_t end_bit, 
 template <typename KeyT>
 static hipError_t sort_passes_t(KeyT* keys[2], int32_t* values[2], int32_t* sel, int32_t count,
                                 const uint32_t* count_dev, int32_t begin_bit, int32_t end_bit, char* tb, int strict,
-                                hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) {
+                                hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr,
+                                int32_t* ranges = nullptr, int32_t ntiles = 0) {
     using S = typename ShapeOf<KeyT>::type;
     const SortLayout l = layout_for<KeyT>(count);
     const int passes = sort_passes(begin_bit, end_bit);
@@ -653,17 +686,21 @@ static hipError_t sort_passes_t(KeyT* keys[2], int32_t* values[2], int32_t* sel,
                                keys[cur], values[cur], keys[cur ^ 1], values[cur ^ 1], count, count_dev, strict,
                                shift, bits, hist + p * kRadix, st, st_next, tickets + p, trivial + p);
         } else if constexpr (sizeof(KeyT) == 8) {
-            // keys only: the persistent kernel, with 7-bit digit tables where the digit fits them
+            // keys only: the persistent kernel, with 7-bit digit tables where the digit fits them;
+            // the last pass also writes the tile ranges when asked
+            int32_t* rg = (p + 1 == passes) ? ranges : nullptr;
             if (bits <= 7) {
                 const int parts = (count + KeysShape7::kTile - 1) / KeysShape7::kTile;
                 hipLaunchKernelGGL(onesweep_keys_kernel<KeysShape7>, dim3(std::min(keys_grid<KeysShape7>(), parts)),
                                    dim3(KeysShape7::kThreads), 0, s, keys[cur], keys[cur ^ 1], count, count_dev,
-                                   strict, shift, bits, hist + p * kRadix, st, st_next, tickets + p, trivial + p);
+                                   strict, shift, bits, hist + p * kRadix, st, st_next, tickets + p, trivial + p,
+                                   rg, ntiles);
             } else {
                 const int parts = (count + KeysShape8::kTile - 1) / KeysShape8::kTile;
                 hipLaunchKernelGGL(onesweep_keys_kernel<KeysShape8>, dim3(std::min(keys_grid<KeysShape8>(), parts)),
                                    dim3(KeysShape8::kThreads), 0, s, keys[cur], keys[cur ^ 1], count, count_dev,
-                                   strict, shift, bits, hist + p * kRadix, st, st_next, tickets + p, trivial + p);
+                                   strict, shift, bits, hist + p * kRadix, st, st_next, tickets + p, trivial + p,
+                                   rg, ntiles);
             }
         } else {
             return hipErrorInvalidValue;   // keys-only sorts are 64-bit
@@ -736,13 +773,23 @@ hipError_t radix_sort_prepare(int32_t count, void* temp, size_t temp_bytes, uint
 
 hipError_t radix_sort_keys_prepared(uint64_t* keys[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
                                     int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes, hipStream_t s,
-                                    hipEvent_t passes_begin, hipEvent_t passes_end) {
-    if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
+                                    hipEvent_t passes_begin, hipEvent_t passes_end, const RangeArgs* ranges) {
+    if (ranges && (ranges->tile_shift != 0 || ranges->tile_range == nullptr || begin_bit != 0))
+        return hipErrorInvalidValue;
+    if (count <= 1 || end_bit <= begin_bit) {
+        if (!ranges) return hipSuccess;
+        RangeArgs r = *ranges;   // no pass runs: the list is ranged as it stands
+        r.keys = keys[*sel];
+        r.size = count;
+        r.size_dev = count_dev;
+        return launch_ranges(r, s);
+    }
     hipError_t e = sort_check<uint64_t>(count, begin_bit, end_bit, temp, temp_bytes);
     if (e != hipSuccess) return e;
     int32_t* none[2] = {nullptr, nullptr};
     return sort_passes_t<uint64_t>(keys, none, sel, count, count_dev, begin_bit, end_bit, static_cast<char*>(temp),
-                                   1, s, passes_begin, passes_end);
+                                   1, s, passes_begin, passes_end, ranges ? ranges->tile_range : nullptr,
+                                   ranges ? ranges->tiles : 0);
 }
 
 }  // namespace gr

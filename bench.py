@@ -291,12 +291,12 @@ def kernel_bytes(N, V, K, T, P, W, H):
         # coefficients 224 in; colour, xy, z, conic 44 out; a 32-byte record per visible Gaussian
         "front": 268.0 * N + 32.0 * V,
         "numbering": 8.0 * N / 256 + 4.0 * V,  # block counts scanned; a staging slot per record
-        "record_count": 56.0 * V,             # slot + record 36 in; count, mask, depth value, depth key 20 out
+        "record_count": 60.0 * V,             # slot + record 36 in; depth pair 8 + emit record 16 out
         "depth_sort": (4.0 + 16.0 * 4) * V,   # histogram read + 4 passes over (depth word, value)
         "offsets": 12.0 * V,                  # packed words read twice, offsets written
-        "emit": 52.0 * V + 8.0 * K,           # slot, record, mask, offset, count in; K 8-byte entries out
-        "tile_sort_passes": 16.0 * K * P,     # each pass reads and writes every entry
-        "ranges": 8.0 * K + 8.0 * T,
+        "emit": 20.0 * V + 8.0 * K,           # emit record + offset in; K 8-byte entries out
+        # each pass reads and writes every entry; the last one writes the tile ranges too
+        "tile_sort_passes": 16.0 * K * P + 8.0 * T,
         "raster": 48.0 * K + 4.0 * W * H,     # entry + xy + conic + colour gathers (upper bound), RGBA8 out
     }
 

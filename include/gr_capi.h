@@ -373,8 +373,8 @@ GR_API gr_status gr_renderer_set_timing(gr_renderer* r, int32_t level);
  * kernel_ms[i] is the time of the work named kernel_name[i] (in frame order:
  * "front" (SH colour, projection and block-local compaction in one pass),
  * "numbering" (record numbers and staging slots), "record_count",
- * "depth_sort", "offsets", "emit", "tile_sort_scan", "tile_sort_passes",
- * "ranges", "raster") and
+ * "depth_sort", "offsets", "emit", "tile_sort_scan", "tile_sort_passes"
+ * (the last pass also writes the tile ranges), "raster") and
  * visible the number of Gaussians with a non-empty tile rect (else -1).  At
  * GR_TIMING_FRONT, front_ms is the front-end kernel's time (else -1). */
 #define GR_KERNEL_TIMES_MAX 24

@@ -443,19 +443,27 @@ hipError_t radix_sort_pairs(uint64_t* keys[2], int32_t* values[2], int32_t* sel,
 
 hipError_t radix_sort_keys_prepared(uint64_t* keys[2], int32_t* sel, int32_t count, const uint32_t* count_dev,
                                     int32_t begin_bit, int32_t end_bit, void* temp, size_t temp_bytes, hipStream_t s,
-                                    hipEvent_t, hipEvent_t) {
-    if (count <= 1 || end_bit <= begin_bit) return hipSuccess;
-    if (temp_bytes < radix_sort_temp_bytes(count, begin_bit, end_bit)) return hipErrorInvalidValue;
-    const int passes = (end_bit - begin_bit + 7) / 8;
-    uint64_t* k0 = keys[*sel];
-    uint64_t* k1 = keys[*sel ^ 1];
-    run_or_record(s, [=] {
-        const uint32_t n = count_dev ? std::min<uint32_t>(*count_dev, (uint32_t)count) : (uint32_t)count;
-        rd(k0, (size_t)n * 8);
-        touch(k1, (size_t)n * 8);
-    });
-    if (passes & 1) *sel ^= 1;
-    return hipSuccess;
+                                    hipEvent_t, hipEvent_t, const RangeArgs* ranges) {
+    if (ranges && (ranges->tile_shift != 0 || !ranges->tile_range || begin_bit != 0)) return hipErrorInvalidValue;
+    if (count > 1 && end_bit > begin_bit) {
+        if (temp_bytes < radix_sort_temp_bytes(count, begin_bit, end_bit)) return hipErrorInvalidValue;
+        const int passes = (end_bit - begin_bit + 7) / 8;
+        uint64_t* k0 = keys[*sel];
+        uint64_t* k1 = keys[*sel ^ 1];
+        run_or_record(s, [=] {
+            const uint32_t n = count_dev ? std::min<uint32_t>(*count_dev, (uint32_t)count) : (uint32_t)count;
+            rd(k0, (size_t)n * 8);
+            touch(k1, (size_t)n * 8);
+        });
+        if (passes & 1) *sel ^= 1;
+    }
+    if (!ranges) return hipSuccess;
+    // the last pass (or, with no pass, the ranges kernel) ranges the sorted list
+    RangeArgs r = *ranges;
+    r.keys = keys[*sel];
+    r.size = count;
+    r.size_dev = count_dev;
+    return launch_ranges(r, s);
 }
 
 

@@ -39,9 +39,47 @@ def test_renderer_frames_of_changing_size(torch_cuda, orc):
         img, st = r.render(cam, W, H)
         ref = orc.render(scene["position"], scene["scale_rot"], shal, None, n, 3, cam.packed(), W, H)
         assert st["tile_list_size"] == ref["k"], f"frame {i} {W}x{H}: K"
-        keys, vals, _ = frame_list(r)
+        keys, vals, rng = frame_list(r)
         assert np.array_equal(keys, ref["keys"]) and np.array_equal(vals, ref["values"]), f"frame {i}: list"
+        assert np.array_equal(rng, orc.ranges(ref["keys"], len(rng) // 2)), f"frame {i}: tile ranges"
         assert np.array_equal(img, ref["image"]), f"frame {i} {W}x{H}: image"
+    r.close()
+
+
+def test_renderer_tile_ranges_small_frames(torch_cuda, orc):
+    """The tile ranges the tile sort's last pass writes, on frames that take its other paths: one
+    tile (no sort pass: the ranges kernel runs instead), 288 tiles (two passes of 5 + 4 bits) with
+    every entry in the first tile row (the last pass sees one digit and copies the list through)
+    and with entries in the first and last rows (both passes sort)."""
+    say("renderer: tile ranges on frames of 1 and 288 tiles")
+    base = gr.generate_scene(1, 256, 3, 5)
+
+    def row(y, count=40):
+        return [(x, y, 0.0) for x in np.linspace(-3.5, 3.5, count)]
+
+    cases = [("one tile", 16, 16, [(0.0, 0.0, 0.0), (0.01, 0.0, 0.2), (-0.01, 0.01, -0.3)], lambda t: True),
+             ("first row only", 512, 144, row(-1.5), lambda t: t.max() < 32),
+             ("first and last rows", 512, 144, row(-1.5) + row(1.5), lambda t: t.min() < 32 and t.max() >= 256)]
+    r = gr.Renderer()
+    for name, W, H, pts, check in cases:
+        n = len(pts)
+        sc = {k: (v[:n].copy() if isinstance(v, np.ndarray) and v.ndim and k != "centroid" else v)
+              for k, v in base.items()}
+        sc["count"] = n
+        sc["position"][:, :3] = np.asarray(pts, np.float32)
+        sc["scale_rot"][:, :3] = 0.01
+        cam = gr.camera_make((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), np.deg2rad(60.0), W, H, 0.01, 100.0)
+        r.upload(sc)
+        img, st = r.render(cam, W, H)
+        ref = orc.render(sc["position"], sc["scale_rot"], orc.realign_sh(sc["sh"], n, 48), sc["color"], n, 3,
+                         cam.packed(), W, H)
+        tiles = (ref["keys"] >> np.uint64(32)).astype(np.int64)
+        assert ref["k"] > 1 and check(tiles), f"{name}: the frame does not take the intended path"
+        assert st["tile_list_size"] == ref["k"], f"{name}: K"
+        keys, vals, rng = frame_list(r)
+        assert np.array_equal(keys, ref["keys"]) and np.array_equal(vals, ref["values"]), f"{name}: list"
+        assert np.array_equal(rng, orc.ranges(ref["keys"], len(rng) // 2)), f"{name}: tile ranges"
+        assert np.array_equal(img, ref["image"]), f"{name}: image"
     r.close()
 
 

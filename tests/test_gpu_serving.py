@@ -89,6 +89,26 @@ def test_alone_equals_in_mix(torch_cuda, graphs):
     srv.close()
 
 
+def test_tiny_views_served(torch_cuda):
+    """Views of one tile (a batch of one such view has no tile-sort pass) and of a few tiles,
+    alone and fused, against the renderer."""
+    say("serving one-tile and few-tile views")
+    s, views = scene_and_views(1, 10_000, 23, 4, [(16, 16), (16, 16), (40, 24), (16, 9)])
+    ref = renderer_images(s, views)
+    assert all(img.any() for img in ref), "a tiny view saw nothing"
+    srv = gr.Server(slots=2, max_views=4, use_graphs=False)
+    sid = srv.add_scene(s)
+    rid = srv.submit(sid, *views[0])
+    d, img = srv.wait_all([rid])[rid]
+    assert d["status"] == 0 and d["batch_views"] == 1 and np.array_equal(img, ref[0])
+    ids = [srv.submit(sid, cam, W, H) for cam, W, H in views]
+    got = srv.wait_all(ids)
+    for i, rid in enumerate(ids):
+        d, img = got[rid]
+        assert d["status"] == 0 and np.array_equal(img, ref[i]), f"tiny view {i} differs"
+    srv.close()
+
+
 @pytest.mark.parametrize("precision", [1, 2], ids=["fp16", "bf16"])
 def test_precision_serving_matches_renderer(torch_cuda, precision):
     say(f"serving precision {precision}")
