@@ -171,7 +171,10 @@ struct BuildArgs {
                                  // record holds it)
     uint4* emit_records;         // count pass: emit record of record p
     int32_t index_bits;          // packed_index_bits(nviews * count)
-    const uint32_t* record_offsets; // emit pass: output offset per record
+    const uint32_t* sorted_words;   // emit pass: the packed record words in depth order
+    const uint32_t* depth_prefix;   // emit pass: output offset of each depth-order position
+    const uint32_t* windows;        // emit pass: depth-order position holding entry w * kEmitWindow
+    const uint32_t* entry_total;    // emit pass: device-side number of entries (K)
     uint32_t* tile_hist;         // record emit: 256-bin histograms of the tile's digits (NULL: none)
     int32_t tile_passes;         // digits counted into tile_hist (<= 3)
     int32_t tile_digit_bits;     // their width: sort_digit_bits(0, tile bits)
@@ -210,6 +213,10 @@ struct RasterArgs {
 // Gaussians per build batch: one wave64 walks the candidate tiles of 64
 // consecutive Gaussians.
 constexpr int kBuildBatch = 64;
+// Entries per window of the record emit (one wave writes one window of the list at a time).
+constexpr int kEmitWindow = 256;
+// Words of the window table for a list of `capacity` entries.
+inline int64_t emit_window_words(int64_t capacity) { return capacity / kEmitWindow + 1; }
 constexpr int kBuildThreads = 256;
 // Workgroups of the record emit pass (its waves take batches grid-stride).  At 6M Gaussians
 // and 1080p, 4096 workgroups build a frame's list in 0.535 ms against 0.559 at 2048, 0.564
@@ -232,10 +239,16 @@ int packed_index_bits(int64_t n);
 // Exclusive scan of the counts held by packed record words (record index in
 // bits [0, index_bits), count above it, a saturated count taken from the
 // record's emit record): the first *valid words, in their order; the prefix
-// of word j goes to offsets[index of word j] and the total to *total.
+// of word j goes to prefix[j], the total to *total, and windows[w] (w <
+// nwindows; nwindows 0: no table) receives the j whose entries [prefix[j],
+// prefix[j] + count) hold entry w * kEmitWindow.
 hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint32_t* valid, const uint4* emit_records,
-                                    int32_t index_bits, uint32_t* total, uint32_t* scratch, uint32_t* offsets,
-                                    hipStream_t stream);
+                                    int32_t index_bits, uint32_t* total, uint32_t* scratch, uint32_t* prefix,
+                                    uint32_t* windows, int64_t nwindows, hipStream_t stream);
+// The same window table from the scanned prefix of the first *valid of n records (K at *total),
+// for a caller that sizes the table after the scan (nwindows 0 in launch_build_scan_packed).
+hipError_t launch_emit_windows(const uint32_t* prefix, const uint32_t* valid, int32_t n, const uint32_t* total,
+                               uint32_t* windows, int64_t nwindows, hipStream_t stream);
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t stream);
 // entries (index << 32 | tile) -> keys[i] = tile << 32 | depth word of clip_z[index], values[i] = index
 hipError_t launch_expand_keys(const uint64_t* entries, const float* clip_z, uint64_t* keys, int32_t* values,

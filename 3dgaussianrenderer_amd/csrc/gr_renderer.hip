@@ -121,7 +121,7 @@ struct gr_renderer {
     DevBuf<uchar4> out;
     DevBuf<uint32_t> batch_counts;  // total K, scan scratch
     DevBuf<uint4> emit_recs;        // emit record per build record
-    DevBuf<uint32_t> rec_offsets;   // output offset per build record
+    DevBuf<uint32_t> rec_prefix;    // output offset per depth-order position of the records
     DevBuf<uint32_t> dkeys[2];      // depth words of the visible Gaussians
     DevBuf<int32_t> dorder[2];      // their packed record words (record index | entry count)
     DevBuf<float4> records;         // staged build records of the visible Gaussians (2 x float4 each)
@@ -171,7 +171,7 @@ gr_status gr_renderer_destroy(gr_renderer* r) {
     r->full_values.release(); r->out.release();
     r->batch_counts.release();
     r->dkeys[0].release(); r->dkeys[1].release(); r->dorder[0].release(); r->dorder[1].release();
-    r->rec_offsets.release(); r->emit_recs.release();
+    r->rec_prefix.release(); r->emit_recs.release();
     r->records.release(); r->blocks.release(); r->slots.release();
     if (r->sort_temp) (void)hipFree(r->sort_temp);
     if (r->dsort_temp) (void)hipFree(r->dsort_temp);
@@ -279,7 +279,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     const size_t nn = (size_t)std::max(n, 1);
     R_CHECK(r->batch_counts.ensure(2 + scan_scratch_words((int32_t)nn)));
     R_CHECK(r->emit_recs.ensure(nn));
-    R_CHECK(r->rec_offsets.ensure(nn));
+    R_CHECK(r->rec_prefix.ensure(nn));
     for (int k = 0; k < 2; ++k) {
         R_CHECK(r->entries[k].ensure(1));
         R_CHECK(r->dkeys[k].ensure(nn));
@@ -394,12 +394,19 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     }
     R_HIP(stage(4));
     R_HIP(mark("depth_sort"));
-    // output offsets: the counts carried by the sorted words, scanned in depth order (total K)
-    // and scattered back to the records
-    R_HIP(launch_build_scan_packed(reinterpret_cast<const uint32_t*>(r->dorder[dsel].p), n, vis_total,
-                                   r->emit_recs.p, b.index_bits, k_total, k_total + 1, r->rec_offsets.p, stream));
+    // output offsets: the counts carried by the sorted words, scanned in depth order (total K),
+    // and the emit windows' first records.  The window table lives in the second list buffer,
+    // which the tile sort's first pass overwrites only after the emit (capacity / 256 + 1 words
+    // in capacity * 2 words).
+    const uint32_t* sorted_words = reinterpret_cast<const uint32_t*>(r->dorder[dsel].p);
+    uint32_t* windows = reinterpret_cast<uint32_t*>(r->entries[1].p);
+    R_HIP(launch_build_scan_packed(sorted_words, n, vis_total, r->emit_recs.p, b.index_bits, k_total, k_total + 1,
+                                   r->rec_prefix.p, windows, emit_window_words(capacity), stream));
     R_HIP(mark("offsets"));
-    b.record_offsets = r->rec_offsets.p;
+    b.sorted_words = sorted_words;
+    b.depth_prefix = r->rec_prefix.p;
+    b.windows = windows;
+    b.entry_total = k_total;
     const int tile_bits = tile_bits_for(tiles);
     b.tile_passes = (tile_bits + 7) / 8;
     b.tile_hist = b.tile_passes ? thist : nullptr;   // zeroed by the front end; one tile: no pass

@@ -161,7 +161,7 @@ struct Workspace {
     Dev dsort_temp;                   // depth sort scratch
     Dev counts;                       // K (device) + scan scratch
     Dev emit_recs;                    // [V*N] emit record per record
-    Dev rec_offsets;                  // [V*N] output offset per record
+    Dev rec_prefix;                   // [V*N] output offset per depth-order position of the records
     Dev list0, list1;                 // tile lists: Gaussian index << 32 | global tile id
     Dev sort_temp;
     Dev ranges;                       // 2 * tiles
@@ -172,7 +172,7 @@ struct Workspace {
     uint64_t generation = 0;          // bumps when a pointer a captured graph holds changes
     std::vector<Dev*> devs() {
         return {&color, &xy, &con, &blocks, &dkeys0, &dkeys1, &dorder0, &dorder1, &records, &slots,
-                &emit_recs, &rec_offsets,
+                &emit_recs, &rec_prefix,
                 &dsort_temp, &counts, &list0, &list1, &sort_temp, &ranges, &images, &views};
     }
 };
@@ -198,6 +198,7 @@ struct Slot {
     int64_t img_bytes = 0;
     uint32_t k = 0;
     uint32_t capacity = 0;
+    int32_t dsel = 0;                 // depth-sort buffer holding the sorted record words
     double start_ms = 0.0;
     int img_cur = 0;                  // staging buffer of the batch in flight
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -384,7 +385,7 @@ std::vector<PhaseANeed> phase_a_needs(Slot& sl, const Scene& sc, size_t nviews, 
             {&w.dsort_temp, (int64_t)radix_sort_temp_bytes((int32_t)std::max<int64_t>(V * N, 1), 0, 32)},
             {&w.counts, 4 * (2 + scan_scratch_words((int32_t)(V * N)))},
             {&w.emit_recs, V * N * 16},
-            {&w.rec_offsets, V * N * 4},
+            {&w.rec_prefix, V * N * 4},
             {&w.images, img_bytes},
             {&w.ranges, 8 * (int64_t)tiles_total},
             {&w.views, V * (int64_t)sizeof(ViewDesc)}};
@@ -475,7 +476,12 @@ BuildArgs record_build_args(Slot& sl, const Scene& sc) {
     b.depth_keys = w.dkeys0.as<uint32_t>();
     b.depth_words = w.dorder0.as<uint32_t>();
     b.emit_records = w.emit_recs.as<uint4>();
-    b.record_offsets = w.rec_offsets.as<uint32_t>();
+    b.sorted_words = (sl.dsel ? w.dorder1 : w.dorder0).as<uint32_t>();
+    b.depth_prefix = w.rec_prefix.as<uint32_t>();
+    // the window table lives in the second list buffer, which the tile sort's first pass
+    // overwrites only after the emit (capacity / 256 + 1 words in capacity * 2 words)
+    b.windows = w.list1.as<uint32_t>();
+    b.entry_total = w.counts.as<uint32_t>();
     b.index_bits = packed_index_bits(std::max<int64_t>((int64_t)V * N, 1));
     b.count = N;
     b.capacity = (int32_t)sl.capacity;
@@ -541,12 +547,14 @@ gr_status enqueue_phase_a(gr_server* s, Slot& sl, const Scene& sc) {
     int32_t dsel = 0;
     S_HIP(radix_sort_pairs_u32(dk, dv, &dsel, V * N, vis, 0, 32, w.dsort_temp.p, w.dsort_temp.bytes, sl.stream,
                                /*zeroed=*/true));
+    sl.dsel = dsel;
     uint32_t* counts = w.counts.as<uint32_t>();
     const int vn = V * N;
-    // output offsets: the counts carried by the sorted words, scanned in depth order (K) and
-    // scattered back to the records
+    // output offsets: the counts carried by the sorted words, scanned in depth order (K); the
+    // emit windows are found in phase B, once the list (which holds their table) is sized
     S_HIP(launch_build_scan_packed(reinterpret_cast<const uint32_t*>(dv[dsel]), vn, vis, w.emit_recs.as<uint4>(),
-                                   b.index_bits, counts, counts + 2, w.rec_offsets.as<uint32_t>(), sl.stream));
+                                   b.index_bits, counts, counts + 2, w.rec_prefix.as<uint32_t>(), nullptr, 0,
+                                   sl.stream));
     S_HIP(hipMemcpyAsync(w.k_host.p, counts, 4, hipMemcpyDeviceToHost, sl.stream));
     return GR_OK;
 }
@@ -563,6 +571,8 @@ gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
     b.tile_passes = (sl.tile_bits + 7) / 8;
     b.tile_digit_bits = sort_digit_bits(0, sl.tile_bits);
     if (!b.tile_passes) b.tile_hist = nullptr;   // one tile in the batch: no pass, no histograms
+    S_HIP(launch_emit_windows(b.depth_prefix, b.record_total, V * N, counts, const_cast<uint32_t*>(b.windows),
+                              emit_window_words(sl.capacity), sl.stream));
     S_HIP(launch_build_emit(b, sl.stream));
     uint64_t* kb[2] = {w.list0.as<uint64_t>(), w.list1.as<uint64_t>()};
     int32_t sel = 0;

@@ -137,9 +137,13 @@ __global__ void sort_init_kernel(uint32_t* hist, uint32_t* tickets, uint32_t* st
 #define GR_HIST_BLOCKS 1024
 #endif
 
+#ifndef GR_HIST_SEG
+#define GR_HIST_SEG 16
+#endif
+
 constexpr int kHistBlocks = GR_HIST_BLOCKS;   // workgroups of the histogram read (each flushes once)
 constexpr int kHistThreads = 256;
-constexpr int kHistSeg = 16;
+constexpr int kHistSeg = GR_HIST_SEG;         // consecutive keys per lane
 constexpr int kHistChunk = kHistThreads * kHistSeg;
 
 template <typename KeyT>

@@ -548,28 +548,36 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
     }
 }
 
-// Record walk, emit pass: the same 64-record batches (index order) as the
-// count pass, each wave taking batches grid-stride (one tile-digit histogram
-// flush per workgroup), the emit records and offsets read contiguously.
-// Record o's entries go to record_offsets[o] + 0, 1, ... in candidate order,
-// as (Gaussian index << 32 | tile id) entries: the offsets follow depth
-// order, so the depth word of the key is carried by the position and the
-// stable tile sort that follows needs the tile bits alone.  Footprints of at
-// most 64 candidates come with their passing-candidate mask (entry k is
-// candidate select_bit(mask, k): no tests, no footprint); larger ones fetch
-// their build record and run the overlap tests again.
-__global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a) {
+// Record walk, emit pass, in depth order.  The list is cut into windows of kEmitWindow entries;
+// a wave writes one window at a time (windows taken grid-stride, one tile-digit histogram flush
+// per workgroup).  windows[w] is the depth-order position of the record holding the window's
+// first entry; from there the wave reads the records' packed words and output offsets
+// contiguously, 64 positions at a time, fetches their emit records, and writes the window's
+// entries in order: window entry e belongs to the last record starting at or before it and is
+// that record's entry k = e - (its start).  Footprints of at most 64 candidates give entry k as
+// candidate select_bit(mask, k) (no tests, no footprint); larger ones fetch their build record,
+// run the overlap tests again and keep the passing candidates whose entry index falls in the
+// window.  Entries are (Gaussian index << 32 | tile id); their positions follow depth order, so
+// the depth word of the key is carried by the position and the stable tile sort that follows
+// needs the tile bits alone.  Every window holds the same number of entries, so the work is
+// spread evenly whatever the footprint sizes, and the list is written front to back.
+#ifndef GR_EMIT_WPE
+#define GR_EMIT_WPE 4
+#endif
+__global__ __launch_bounds__(kBuildThreads) __attribute__((amdgpu_waves_per_eu(GR_EMIT_WPE))) void record_emit_kernel(
+    BuildArgs a) {
     constexpr int W = kBuildThreads / 64;
-    __shared__ int s_excl[W][64];
+    __shared__ int s_rel[W][64];         // record start relative to the window (negative: earlier)
+    __shared__ int s_excl[W][64];        // first candidate of a large footprint in the step order
     __shared__ int4 s_rect[W][64];       // global tile of the rect corner, nx, tiles_x, -
     __shared__ uint64_t s_mask[W][64];
-    __shared__ uint32_t s_off[W][64];
     __shared__ int s_gi[W][64];
-    __shared__ int2 s_xy[W][64];         // rect corner in view tiles (large footprints)
+    __shared__ uint32_t s_xy[W][64];     // rect corner in view tiles, x | y << 16 (large footprints)
     __shared__ EllipseRec s_ell[W][64];  // large footprints only
-    __shared__ float2 s_clip[W][64];
+    __shared__ int s_view[W][64];
     __shared__ uint32_t s_done[W][64];
-    __shared__ int s_cand[W][64];        // candidates of a large footprint
+    __shared__ int s_cand[W][64];        // candidates of a large footprint (0: small footprint)
+    __shared__ int2 s_keep[W][64];       // entry indices [lo, hi) of a large footprint in the window
     __shared__ float4 s_fast[W][64];     // kv, kh, lo of the bounded test (large footprints)
     __shared__ uint32_t s_mark[W][64];
     __shared__ uint32_t s_hist[3][256];
@@ -580,126 +588,186 @@ __global__ __launch_bounds__(kBuildThreads) void record_emit_kernel(BuildArgs a)
     if (hist)
         for (int i = threadIdx.x; i < 3 * 256; i += kBuildThreads) (&s_hist[0][0])[i] = 0u;
     __syncthreads();
-    const int limit = (int)min(*a.record_total, (uint32_t)a.nviews * (uint32_t)a.count);
-    const uint32_t cap = (uint32_t)a.capacity;
+    const uint32_t nrec = min(*a.record_total, (uint32_t)a.nviews * (uint32_t)a.count);
+    const uint32_t kend = min(*a.entry_total, (uint32_t)a.capacity);
+    const int64_t nwin = ((int64_t)kend + kEmitWindow - 1) / kEmitWindow;
+    const int ib = a.index_bits;
+    const uint32_t imask = (1u << ib) - 1u;
+    const uint32_t esc = 0xffffffffu >> ib;
     auto put = [&](uint32_t p, uint32_t tile, int o) {
-        if (p < cap) {
-            a.entries[p] = ((uint64_t)(uint32_t)s_gi[w][o] << 32) | tile;
-            if (hist)
-                for (int k = 0; k < a.tile_passes; ++k) atomicAdd(&s_hist[k][(tile >> (db * k)) & dm], 1u);
-        }
+        a.entries[p] = ((uint64_t)(uint32_t)s_gi[w][o] << 32) | tile;
+        if (hist)
+            for (int k = 0; k < a.tile_passes; ++k) atomicAdd(&s_hist[k][(tile >> (db * k)) & dm], 1u);
     };
-    // the next batch's emit records are fetched while the current batch is written out
-    const int stride = gridDim.x * W * kBuildBatch;
-    int first = (blockIdx.x * W + w) * kBuildBatch;
-    uint4 n_er = make_uint4(0u, 0u, 0u, 0u);
-    uint32_t n_off = 0u;
-    auto fetch = [&](int f) {
-        const int q = f + lane;
-        if (f < limit && q < limit) {
-            n_off = a.record_offsets[q];
-            n_er = a.emit_records[q];
-        }
+    // a lane's depth-order position q + lane: its output offset and packed word, then (when it
+    // starts before kmax, the end of its window) its emit record
+    auto load_words = [&](uint32_t q, uint32_t& p, uint32_t& word) {
+        const uint32_t qi = q + (uint32_t)lane;
+        p = qi < nrec ? a.depth_prefix[qi] : 0xffffffffu;
+        word = qi < nrec ? a.sorted_words[qi] : 0u;
     };
-    fetch(first);
-    for (; first < limit; first += stride) {
-        const int pos = first + lane;
-        const uint4 er = n_er;
-        const uint32_t off = n_off;
-        fetch(first + stride);
-        int ecnt = 0, big = 0;
-        if (pos < limit) {
-            const int pg = (int)er.z;
-            const int view = pg / a.count;
-            const ViewDesc& vd = view_of(view, a.v0, a.views);
-            s_off[w][lane] = off;
-            s_gi[w][lane] = pg - view * a.count;
-            if (!(er.w & kEmitBig)) {
+    auto load_rec = [&](uint32_t p, uint32_t word, uint32_t kmax) {
+        return p < kmax ? a.emit_records[word & imask] : make_uint4(0u, 0u, 0u, 0u);
+    };
+    auto win_end = [&](int64_t wv) { return min((uint32_t)wv * (uint32_t)kEmitWindow + (uint32_t)kEmitWindow, kend); };
+    // loads run two windows ahead: while window i is written, the emit records of window i + 1's
+    // first 64 positions, the offsets and words of window i + 2's and the first position of
+    // window i + 3 are in flight (each load has a whole window's work to arrive)
+    const int64_t wstride = (int64_t)gridDim.x * W;
+    int64_t win = (int64_t)blockIdx.x * W + w;
+    uint32_t q0 = 0u, p0 = 0xffffffffu, word0 = 0u;   // window i
+    uint4 er0 = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t q1 = 0u, p1 = 0xffffffffu, word1 = 0u;   // window i + 1
+    uint32_t q2 = 0u;                                 // window i + 2
+    if (win < nwin) {
+        q0 = a.windows[win];
+        load_words(q0, p0, word0);
+        er0 = load_rec(p0, word0, win_end(win));
+    }
+    if (win + wstride < nwin) {
+        q1 = a.windows[win + wstride];
+        load_words(q1, p1, word1);
+    }
+    if (win + 2 * wstride < nwin) q2 = a.windows[win + 2 * wstride];
+    for (; win < nwin; win += wstride) {
+        const uint32_t k0 = (uint32_t)win * (uint32_t)kEmitWindow;
+        const int wlen = (int)(win_end(win) - k0);
+        uint32_t q = q0, p = p0, word = word0;
+        uint4 er = er0;
+        // advance the pipeline
+        uint4 er1 = make_uint4(0u, 0u, 0u, 0u);
+        uint32_t p2 = 0xffffffffu, word2 = 0u, q3 = 0u;
+        if (win + wstride < nwin) er1 = load_rec(p1, word1, win_end(win + wstride));
+        if (win + 2 * wstride < nwin) load_words(q2, p2, word2);
+        if (win + 3 * wstride < nwin) q3 = a.windows[win + 3 * wstride];
+        for (bool first_chunk = true;; first_chunk = false) {
+            if (!first_chunk) {
+                load_words(q, p, word);
+                er = load_rec(p, word, k0 + (uint32_t)wlen);
+            }
+            // the 64 positions from q; those starting inside the window take part
+            const bool in = p < k0 + (uint32_t)wlen;
+            int rel = 0, lo = 0, hi = 0, big = 0;
+            if (in) {
+                const bool small = !(er.w & kEmitBig);
                 const uint64_t mask = (uint64_t)er.x | ((uint64_t)er.y << 32);
-                ecnt = __popcll(mask);
-                s_mask[w][lane] = mask;
-                const int corner = (int)(er.w & ((1u << kEmitCornerBits) - 1u));
-                s_rect[w][lane] = make_int4(vd.tile_base + corner, (int)(er.w >> kEmitCornerBits) + 1,
-                                            vd.geom.tiles_x, 0);
-            } else {
-                const float4 r0 = a.records[2 * (size_t)er.x];
-                const float4 r1 = a.records[2 * (size_t)er.x + 1];
-                const Footprint f = footprint(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, vd.geom);
-                big = f.nx * f.ny;
-                s_cand[w][lane] = big;
-                s_rect[w][lane] = make_int4(vd.tile_base + f.y0 * vd.geom.tiles_x + f.x0, f.nx, vd.geom.tiles_x, 0);
-                s_xy[w][lane] = make_int2(f.x0, f.y0);
-                const EllipseRec el{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, 1.0f / r1.x, 1.0f / r1.y};
-                s_ell[w][lane] = el;
-                const FastEllipse fe = fast_ellipse(el, f, vd.geom.tile_clip_x, vd.geom.tile_clip_y);
-                s_fast[w][lane] = make_float4(fe.kv, fe.kh, fe.lo, 0.0f);
-                s_clip[w][lane] = make_float2(vd.geom.tile_clip_x, vd.geom.tile_clip_y);
+                const uint32_t pc = word >> ib;
+                const int cnt = (int)(pc != esc ? pc : (small ? (uint32_t)__popcll(mask) : er.y));
+                rel = (int)(p - k0);   // negative for the record that began in an earlier window
+                lo = max(rel, 0);
+                hi = (int)min((int64_t)rel + cnt, (int64_t)wlen);
+                const int pg = (int)er.z;
+                const int view = pg / a.count;
+                const ViewDesc& vd = view_of(view, a.v0, a.views);
+                s_gi[w][lane] = pg - view * a.count;
+                s_rel[w][lane] = rel;
+                if (small) {
+                    s_mask[w][lane] = mask;
+                    const int corner = (int)(er.w & ((1u << kEmitCornerBits) - 1u));
+                    s_rect[w][lane] = make_int4(vd.tile_base + corner, (int)(er.w >> kEmitCornerBits) + 1,
+                                                vd.geom.tiles_x, 0);
+                    s_cand[w][lane] = 0;
+                } else if (hi > lo) {
+                    const float4 r0 = a.records[2 * (size_t)er.x];
+                    const float4 r1 = a.records[2 * (size_t)er.x + 1];
+                    const Footprint f = footprint(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, vd.geom);
+                    big = f.nx * f.ny;
+                    s_cand[w][lane] = big;
+                    s_keep[w][lane] = make_int2(lo - rel, hi - rel);
+                    s_rect[w][lane] =
+                        make_int4(vd.tile_base + f.y0 * vd.geom.tiles_x + f.x0, f.nx, vd.geom.tiles_x, 0);
+                    s_xy[w][lane] = (uint32_t)f.x0 | ((uint32_t)f.y0 << 16);
+                    s_view[w][lane] = view;
+                    const EllipseRec el{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, 1.0f / r1.x, 1.0f / r1.y};
+                    s_ell[w][lane] = el;
+                    const FastEllipse fe = fast_ellipse(el, f, vd.geom.tile_clip_x, vd.geom.tile_clip_y);
+                    s_fast[w][lane] = make_float4(fe.kv, fe.kh, fe.lo, 0.0f);
+                } else {
+                    s_cand[w][lane] = 0;
+                }
             }
-        }
-        // entries of the small footprints
-        int incl = wave_incl_scan(ecnt, lane);
-        int total = __shfl(incl, 63, 64);
-        int excl = incl - ecnt;
-        s_excl[w][lane] = excl;
-        s_mark[w][lane] = ~0u;
-        wave_sync();
-        for (int base = 0; base < total; base += 64) {
-            const int e = base + lane;
-            const int o = step_owner(s_mark[w], base, excl, ecnt, lane);
-            if (e < total) {
-                const int k = e - s_excl[w][o];
-                const int l = select_bit(s_mask[w][o], k);
-                const int4 rc = s_rect[w][o];
-                int q, r;
-                divmod_small(l, rc.y, q, r);
-                put(s_off[w][o] + (uint32_t)k, (uint32_t)(rc.x + q * rc.z + r), o);
+            // the window entries [first, last) these records hold, one per lane and step; entries of
+            // large footprints are left to the candidate loop below
+            const int span = hi > lo ? hi - lo : 0;
+            const uint64_t held = __ballot(span > 0);
+            const int first = held ? __shfl(lo, __builtin_ctzll(held), 64) : 0;
+            const int last = held ? __shfl(hi, 63 - __clzll(held), 64) : 0;
+            s_mark[w][lane] = ~0u;
+            wave_sync();
+            for (int base = first & ~63; base < last; base += 64) {
+                const int e = base + lane;
+                const int o = step_owner(s_mark[w], base, lo, span, lane);
+                if (e >= first && e < last && s_cand[w][o] == 0) {
+                    const int k = e - s_rel[w][o];
+                    const int l = select_bit(s_mask[w][o], k);
+                    const int4 rc = s_rect[w][o];
+                    int qq, rr;
+                    divmod_small(l, rc.y, qq, rr);
+                    put(k0 + (uint32_t)e, (uint32_t)(rc.x + qq * rc.z + rr), o);
+                }
+                wave_sync();
+            }
+            // candidates of the large footprints, laid end to end
+            wave_sync();
+            const int incl = wave_incl_scan(big, lane);
+            const int total = __shfl(incl, 63, 64);
+            const int excl = incl - big;
+            s_excl[w][lane] = excl;
+            s_done[w][lane] = 0u;
+            s_mark[w][lane] = ~0u;
+            wave_sync();
+            for (int base = 0; base < total; base += 64) {
+                const int cc = base + lane;
+                bool pass = false;
+                int l = 0;
+                const int o = step_owner(s_mark[w], base, excl, big, lane);
+                uint32_t tile = 0u;
+                if (cc < total) {
+                    l = cc - s_excl[w][o];
+                    const int4 rc = s_rect[w][o];
+                    const uint32_t xy = s_xy[w][o];
+                    int qq, rr;
+                    divmod_small(l, rc.y, qq, rr);
+                    const FrameGeom& vg = view_of(s_view[w][o], a.v0, a.views).geom;
+                    FrameGeom og;
+                    og.tile_clip_x = vg.tile_clip_x;
+                    og.tile_clip_y = vg.tile_clip_y;
+                    const float4 sf = s_fast[w][o];
+                    const EllipseRec& el = s_ell[w][o];
+                    pass = tile_overlap_fast((int)(xy & 0xffffu) + rr, (int)(xy >> 16) + qq, el,
+                                             fast_ellipse_from(el, sf.x, sf.y, sf.z), og);
+                    tile = (uint32_t)(rc.x + qq * rc.z + rr);
+                }
+                const uint64_t m = __ballot(pass);
+                if (pass) {
+                    // lanes of this record in this step start at lane max(0, excl - base)
+                    const int lo2 = max(0, s_excl[w][o] - base);
+                    const uint64_t before = m & lanemask_lt(lane) & ~lanemask_lt(lo2);
+                    const int k = (int)s_done[w][o] + __popcll(before);   // the record's entry index
+                    const int2 keep = s_keep[w][o];
+                    if (k >= keep.x && k < keep.y) put(k0 + (uint32_t)(s_rel[w][o] + k), tile, o);
+                }
+                wave_sync();
+                if (cc < total && (l == 0 || lane == 0)) {
+                    const int len = min(s_excl[w][o] + s_cand[w][o], base + 64) - cc;
+                    s_done[w][o] += (uint32_t)__popcll((m >> lane) & low_bits(len));
+                }
+                wave_sync();
             }
             wave_sync();
+            // more records when all 64 took part and the last one ends inside the window
+            if (__ballot(in) != ~0ull) break;
+            if (__shfl(hi, 63, 64) >= wlen) break;
+            q += 64u;
         }
-        // candidates of the large footprints
-        wave_sync();
-        incl = wave_incl_scan(big, lane);
-        total = __shfl(incl, 63, 64);
-        excl = incl - big;
-        s_excl[w][lane] = excl;
-        s_done[w][lane] = 0u;
-        s_mark[w][lane] = ~0u;
-        wave_sync();
-        for (int base = 0; base < total; base += 64) {
-            const int cc = base + lane;
-            bool pass = false;
-            int l = 0;
-            const int o = step_owner(s_mark[w], base, excl, big, lane);
-            uint32_t tile = 0u;
-            if (cc < total) {
-                l = cc - s_excl[w][o];
-                const int4 rc = s_rect[w][o];
-                const int2 xy = s_xy[w][o];
-                int q, r;
-                divmod_small(l, rc.y, q, r);
-                FrameGeom og;
-                og.tile_clip_x = s_clip[w][o].x;
-                og.tile_clip_y = s_clip[w][o].y;
-                const float4 sf = s_fast[w][o];
-                const EllipseRec& el = s_ell[w][o];
-                pass = tile_overlap_fast(xy.x + r, xy.y + q, el, fast_ellipse_from(el, sf.x, sf.y, sf.z), og);
-                tile = (uint32_t)(rc.x + q * rc.z + r);
-            }
-            const uint64_t m = __ballot(pass);
-            if (pass) {
-                // lanes of this record in this step start at lane max(0, excl - base)
-                const int lo = max(0, s_excl[w][o] - base);
-                const uint64_t before = m & lanemask_lt(lane) & ~lanemask_lt(lo);
-                put(s_off[w][o] + s_done[w][o] + (uint32_t)__popcll(before), tile, o);
-            }
-            wave_sync();
-            if (cc < total && (l == 0 || lane == 0)) {
-                const int len = min(s_excl[w][o] + s_cand[w][o], base + 64) - cc;
-                s_done[w][o] += (uint32_t)__popcll((m >> lane) & low_bits(len));
-            }
-            wave_sync();
-        }
-        wave_sync();
+        q0 = q1;
+        p0 = p1;
+        word0 = word1;
+        er0 = er1;
+        q1 = q2;
+        p1 = p2;
+        word1 = word2;
+        q2 = q3;
     }
     if (hist) {
         __syncthreads();
@@ -733,12 +801,13 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
 hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     if (a.count <= 0 || a.nviews <= 0) return hipSuccess;
     if (a.records) {
-        if (!a.record_total || !a.record_offsets || !a.emit_records || !a.records || !a.entries ||
-            a.tile_passes > 3 ||
-            (a.tile_hist && (a.tile_digit_bits < 1 || a.tile_digit_bits > 8)))
+        if (!a.record_total || !a.entry_total || !a.sorted_words || !a.depth_prefix || !a.windows ||
+            !a.emit_records || !a.records || !a.entries || a.index_bits < 1 || a.index_bits > 31 ||
+            a.tile_passes > 3 || (a.tile_hist && (a.tile_digit_bits < 1 || a.tile_digit_bits > 8)))
             return hipErrorInvalidValue;
         constexpr int W = kBuildThreads / 64;
-        const unsigned blocks = (unsigned)std::max(1, std::min((a.batches + W - 1) / W, kEmitBlocks));
+        const int64_t wins = emit_window_words(a.capacity);
+        const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((wins + W - 1) / W, kEmitBlocks));
         hipLaunchKernelGGL(record_emit_kernel, dim3(blocks), dim3(kBuildThreads), 0, s, a);
     } else {
         hipLaunchKernelGGL(build_kernel<true>, build_grid(a), dim3(kBuildThreads), 0, s, a);
@@ -1030,26 +1099,32 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data,
     }
 }
 
-// The exclusive prefix of packed word j (j < *valid) goes to offsets[index of word j].
+// The exclusive prefix of packed word j (j < *valid) goes to prefix[j]; every window start
+// w * kEmitWindow (w < nwin) inside word j's entries [prefix, prefix + count) gets windows[w] = j.
 __global__ __launch_bounds__(kScanThreads) void scan_down_packed_kernel(const uint32_t* words, int32_t n,
                                                                         const uint32_t* sums, const uint32_t* valid,
-                                                                        const uint4* erec, int ib,
-                                                                        uint32_t* offsets) {
+                                                                        const uint4* erec, int ib, uint32_t* prefix,
+                                                                        uint32_t* windows, int64_t nwin) {
     __shared__ uint32_t s_w[4];
     const int64_t my = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
     const int64_t nv = (int64_t)min(*valid, (uint32_t)n);
-    uint32_t word[kScanItems], v[kScanItems];
+    uint32_t v[kScanItems];
     uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        word[k] = (my + k < nv) ? words[my + k] : 0u;
-        v[k] = (my + k < nv) ? packed_count(word[k], ib, erec) : 0u;
+        v[k] = (my + k < nv) ? packed_count(words[my + k], ib, erec) : 0u;
         sum += v[k];
     }
     uint32_t run = block_excl_256(sum, sums[blockIdx.x], s_w);
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        if (my + k < nv) offsets[word[k] & ((1u << ib) - 1u)] = run;
+        if (my + k < nv) {
+            prefix[my + k] = run;
+            const uint64_t end = (uint64_t)run + v[k];
+            for (uint64_t b = ((uint64_t)run + kEmitWindow - 1) / kEmitWindow; b * kEmitWindow < end && (int64_t)b < nwin;
+                 ++b)
+                windows[b] = (uint32_t)(my + k);   // nwin == 0: no table
+        }
         run += v[k];
     }
 }
@@ -1075,20 +1150,46 @@ int packed_index_bits(int64_t n) {
 }
 
 hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint32_t* valid, const uint4* erec,
-                                    int32_t index_bits, uint32_t* total, uint32_t* scratch, uint32_t* offsets,
-                                    hipStream_t s) {
+                                    int32_t index_bits, uint32_t* total, uint32_t* scratch, uint32_t* prefix,
+                                    uint32_t* windows, int64_t nwindows, hipStream_t s) {
     if (index_bits < 1 || index_bits > 31 || (n > 0 && ((int64_t)1 << index_bits) < n)) return hipErrorInvalidValue;
     if (n <= 0) {
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, total, 0u, (int64_t)1);
         return hipGetLastError();
     }
-    if (!words || !valid || !erec || !scratch || !offsets) return hipErrorInvalidValue;
+    if (!words || !valid || !erec || !scratch || !prefix || nwindows < 0 || (nwindows > 0 && !windows))
+        return hipErrorInvalidValue;
     const int nb = (n + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(scan_reduce_packed_kernel, dim3(nb), dim3(kScanThreads), 0, s, words, n, scratch, valid, erec,
                        index_bits);
     hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, scratch, nb, total);
     hipLaunchKernelGGL(scan_down_packed_kernel, dim3(nb), dim3(kScanThreads), 0, s, words, n, scratch, valid, erec,
-                       index_bits, offsets);
+                       index_bits, prefix, windows, nwindows);
+    return hipGetLastError();
+}
+
+// The emit windows' first records from the scanned offsets (for a caller that sizes the table
+// after the scan): windows[w] = the q whose entries [prefix[q], prefix[q + 1]) hold entry
+// w * kEmitWindow (the last record's entries end at *total).
+__global__ void emit_windows_kernel(const uint32_t* prefix, const uint32_t* valid, int32_t n, const uint32_t* total,
+                                    uint32_t* windows, int64_t nwin) {
+    const int64_t nv = (int64_t)min(*valid, (uint32_t)n);
+    const uint32_t kt = *total;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nv; q += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t start = prefix[q];
+        const uint64_t end = q + 1 < nv ? prefix[q + 1] : kt;
+        for (uint64_t b = ((uint64_t)start + kEmitWindow - 1) / kEmitWindow; b * kEmitWindow < end && (int64_t)b < nwin;
+             ++b)
+            windows[b] = (uint32_t)q;
+    }
+}
+
+hipError_t launch_emit_windows(const uint32_t* prefix, const uint32_t* valid, int32_t n, const uint32_t* total,
+                               uint32_t* windows, int64_t nwindows, hipStream_t s) {
+    if (n <= 0 || nwindows <= 0) return hipSuccess;
+    if (!prefix || !valid || !total || !windows) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(emit_windows_kernel, dim3(grid_for(n)), dim3(256), 0, s, prefix, valid, n, total, windows,
+                       nwindows);
     return hipGetLastError();
 }
 

@@ -294,7 +294,7 @@ def kernel_bytes(N, V, K, T, P, W, H):
         "record_count": 60.0 * V,             # slot + record 36 in; depth pair 8 + emit record 16 out
         "depth_sort": (4.0 + 16.0 * 4) * V,   # histogram read + 4 passes over (depth word, value)
         "offsets": 12.0 * V,                  # packed words read twice, offsets written
-        "emit": 20.0 * V + 8.0 * K,           # emit record + offset in; K 8-byte entries out
+        "emit": 24.0 * V + 8.0 * K,           # sorted word, offset, emit record in; K 8-byte entries out
         # each pass reads and writes every entry; the last one writes the tile ranges too
         "tile_sort_passes": 16.0 * K * P + 8.0 * T,
         "raster": 48.0 * K + 4.0 * W * H,     # entry + xy + conic + colour gathers (upper bound), RGBA8 out

@@ -308,10 +308,23 @@ int packed_index_bits(int64_t n) {
     return b;
 }
 
+constexpr uint32_t kMockWindow = 256;   // entries per emit window (kEmitWindow)
+
+// windows[w] = the position whose entries [prefix[q], prefix[q + 1]) hold entry w * 256
+static void mock_windows(const uint32_t* prefix, uint32_t nv, uint32_t total, uint32_t* windows, int64_t nwin) {
+    for (uint32_t q = 0; q < nv; ++q) {
+        const uint64_t end = q + 1 < nv ? prefix[q + 1] : total;
+        for (uint64_t b = ((uint64_t)prefix[q] + kMockWindow - 1) / kMockWindow; b * kMockWindow < end && (int64_t)b < nwin;
+             ++b)
+            windows[b] = q;
+    }
+}
+
 hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint32_t* valid, const uint4* erec,
-                                    int32_t index_bits, uint32_t* total, uint32_t* scratch, uint32_t* offsets,
-                                    hipStream_t s) {
+                                    int32_t index_bits, uint32_t* total, uint32_t* scratch, uint32_t* prefix,
+                                    uint32_t* windows, int64_t nwindows, hipStream_t s) {
     if (index_bits < 1 || index_bits > 31 || (n > 0 && ((int64_t)1 << index_bits) < n)) return hipErrorInvalidValue;
+    if (nwindows < 0 || (nwindows > 0 && !windows)) return hipErrorInvalidValue;
     run_or_record(s, [=] {
         const uint32_t nv = std::min<uint32_t>(*valid, (uint32_t)std::max(n, 0));
         const uint32_t esc = 0xffffffffu >> index_bits;
@@ -321,11 +334,25 @@ hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint
             const uint32_t idx = w & ((1u << index_bits) - 1u);
             if (idx >= nv) std::abort();   // a pair names a record that does not exist
             const uint32_t c = (w >> index_bits) == esc ? (uint32_t)__builtin_popcount(erec[idx].x) : (w >> index_bits);
-            offsets[idx] = acc;
+            prefix[i] = acc;
             acc += c;
         }
         *total = acc;
+        if (nwindows > 0) mock_windows(prefix, nv, acc, windows, nwindows);
         touch(scratch, (size_t)scan_scratch_words(n) * 4);
+    });
+    return hipSuccess;
+}
+
+hipError_t launch_emit_windows(const uint32_t* prefix, const uint32_t* valid, int32_t n, const uint32_t* total,
+                               uint32_t* windows, int64_t nwindows, hipStream_t s) {
+    if (n <= 0 || nwindows <= 0) return hipSuccess;
+    if (!prefix || !valid || !total || !windows) return hipErrorInvalidValue;
+    run_or_record(s, [=] {
+        const uint32_t nv = std::min<uint32_t>(*valid, (uint32_t)n);
+        rd(prefix, (size_t)nv * 4);
+        touch(windows, (size_t)std::min<int64_t>(nwindows, (int64_t)*total / kMockWindow + 1) * 4);
+        mock_windows(prefix, nv, *total, windows, nwindows);
     });
     return hipSuccess;
 }
@@ -353,10 +380,18 @@ hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
             const uint32_t nvis = std::min<uint32_t>(*a.record_total, (uint32_t)(a.nviews * a.count));
             K = 0;
             for (uint32_t q = 0; q < nvis; ++q) {
-                const uint4 e = a.emit_records[q];
-                if (a.record_offsets[q] > 3u * nvis) std::abort();   // offsets lie inside the list
-                if ((e.w & kEmitBig) || e.y != 0u || e.z != q) std::abort();
+                const uint32_t rec = a.sorted_words[q] & ((1u << a.index_bits) - 1u);
+                if (rec >= nvis) std::abort();
+                const uint4 e = a.emit_records[rec];
+                if (a.depth_prefix[q] > 3u * nvis) std::abort();   // offsets lie inside the list
+                if ((e.w & kEmitBig) || e.y != 0u || e.z != rec) std::abort();
                 K += (uint32_t)__builtin_popcount(e.x);
+            }
+            if (K != *a.entry_total) std::abort();
+            // every window's first record holds the window's first entry
+            for (uint32_t w0 = 0; w0 * kMockWindow < std::min<uint32_t>(K, (uint32_t)a.capacity); ++w0) {
+                const uint32_t q = a.windows[w0];
+                if (q >= nvis || a.depth_prefix[q] > w0 * kMockWindow) std::abort();
             }
             if (a.tile_hist)
                 for (int k = 0; k < a.tile_passes; ++k) a.tile_hist[k * 256] += std::min<uint32_t>(K, (uint32_t)a.capacity);
