@@ -213,14 +213,17 @@ struct RasterArgs {
 // Gaussians per build batch: one wave64 walks the candidate tiles of 64
 // consecutive Gaussians.
 constexpr int kBuildBatch = 64;
-// Entries per window of the record emit (one wave writes one window of the list at a time).
-constexpr int kEmitWindow = 256;
+// Entries per window of the record emit (one wave writes one window of the list at a time).  At
+// 6M Gaussians and 1080p: 128 / 256 / 512 / 1024 / 2048 entries, emit 181 / 154 / 146 / 144 / 150 us.
+#ifndef GR_EMIT_WINDOW
+#define GR_EMIT_WINDOW 512
+#endif
+constexpr int kEmitWindow = GR_EMIT_WINDOW;
 // Words of the window table for a list of `capacity` entries.
 inline int64_t emit_window_words(int64_t capacity) { return capacity / kEmitWindow + 1; }
 constexpr int kBuildThreads = 256;
-// Workgroups of the record emit pass (its waves take batches grid-stride).  At 6M Gaussians
-// and 1080p, 4096 workgroups build a frame's list in 0.535 ms against 0.559 at 2048, 0.564
-// at 1024 and 0.538 at 8192.
+// Workgroups of the record emit pass (its waves take windows grid-stride, one tile-digit
+// histogram flush per workgroup).
 #ifndef GR_EMIT_BLOCKS
 #define GR_EMIT_BLOCKS 4096
 #endif

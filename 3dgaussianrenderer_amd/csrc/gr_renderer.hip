@@ -396,7 +396,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     R_HIP(mark("depth_sort"));
     // output offsets: the counts carried by the sorted words, scanned in depth order (total K),
     // and the emit windows' first records.  The window table lives in the second list buffer,
-    // which the tile sort's first pass overwrites only after the emit (capacity / 256 + 1 words
+    // which the tile sort's first pass overwrites only after the emit (capacity / kEmitWindow + 1 words
     // in capacity * 2 words).
     const uint32_t* sorted_words = reinterpret_cast<const uint32_t*>(r->dorder[dsel].p);
     uint32_t* windows = reinterpret_cast<uint32_t*>(r->entries[1].p);

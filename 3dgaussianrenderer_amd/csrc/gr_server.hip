@@ -479,7 +479,7 @@ BuildArgs record_build_args(Slot& sl, const Scene& sc) {
     b.sorted_words = (sl.dsel ? w.dorder1 : w.dorder0).as<uint32_t>();
     b.depth_prefix = w.rec_prefix.as<uint32_t>();
     // the window table lives in the second list buffer, which the tile sort's first pass
-    // overwrites only after the emit (capacity / 256 + 1 words in capacity * 2 words)
+    // overwrites only after the emit (capacity / kEmitWindow + 1 words in capacity * 2 words)
     b.windows = w.list1.as<uint32_t>();
     b.entry_total = w.counts.as<uint32_t>();
     b.index_bits = packed_index_bits(std::max<int64_t>((int64_t)V * N, 1));

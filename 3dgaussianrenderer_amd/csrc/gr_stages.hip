@@ -708,14 +708,17 @@ __global__ __launch_bounds__(kBuildThreads) __attribute__((amdgpu_waves_per_eu(G
                 wave_sync();
             }
             // candidates of the large footprints, laid end to end
-            wave_sync();
-            const int incl = wave_incl_scan(big, lane);
-            const int total = __shfl(incl, 63, 64);
-            const int excl = incl - big;
-            s_excl[w][lane] = excl;
-            s_done[w][lane] = 0u;
-            s_mark[w][lane] = ~0u;
-            wave_sync();
+            int total = 0, excl = 0;
+            if (__ballot(big > 0)) {   // wave-uniform; most chunks have none
+                const int incl = wave_incl_scan(big, lane);
+                total = __shfl(incl, 63, 64);
+                excl = incl - big;
+                wave_sync();
+                s_excl[w][lane] = excl;
+                s_done[w][lane] = 0u;
+                s_mark[w][lane] = ~0u;
+                wave_sync();
+            }
             for (int base = 0; base < total; base += 64) {
                 const int cc = base + lane;
                 bool pass = false;

@@ -308,14 +308,12 @@ int packed_index_bits(int64_t n) {
     return b;
 }
 
-constexpr uint32_t kMockWindow = 256;   // entries per emit window (kEmitWindow)
-
-// windows[w] = the position whose entries [prefix[q], prefix[q + 1]) hold entry w * 256
+// windows[w] = the position whose entries [prefix[q], prefix[q + 1]) hold entry w * kEmitWindow
 static void mock_windows(const uint32_t* prefix, uint32_t nv, uint32_t total, uint32_t* windows, int64_t nwin) {
     for (uint32_t q = 0; q < nv; ++q) {
         const uint64_t end = q + 1 < nv ? prefix[q + 1] : total;
-        for (uint64_t b = ((uint64_t)prefix[q] + kMockWindow - 1) / kMockWindow; b * kMockWindow < end && (int64_t)b < nwin;
-             ++b)
+        const uint64_t ew = (uint64_t)kEmitWindow;
+        for (uint64_t b = ((uint64_t)prefix[q] + ew - 1) / ew; b * ew < end && (int64_t)b < nwin; ++b)
             windows[b] = q;
     }
 }
@@ -351,7 +349,7 @@ hipError_t launch_emit_windows(const uint32_t* prefix, const uint32_t* valid, in
     run_or_record(s, [=] {
         const uint32_t nv = std::min<uint32_t>(*valid, (uint32_t)n);
         rd(prefix, (size_t)nv * 4);
-        touch(windows, (size_t)std::min<int64_t>(nwindows, (int64_t)*total / kMockWindow + 1) * 4);
+        touch(windows, (size_t)std::min<int64_t>(nwindows, (int64_t)*total / (uint32_t)kEmitWindow + 1) * 4);
         mock_windows(prefix, nv, *total, windows, nwindows);
     });
     return hipSuccess;
@@ -389,9 +387,9 @@ hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
             }
             if (K != *a.entry_total) std::abort();
             // every window's first record holds the window's first entry
-            for (uint32_t w0 = 0; w0 * kMockWindow < std::min<uint32_t>(K, (uint32_t)a.capacity); ++w0) {
+            for (uint32_t w0 = 0; w0 * (uint32_t)kEmitWindow < std::min<uint32_t>(K, (uint32_t)a.capacity); ++w0) {
                 const uint32_t q = a.windows[w0];
-                if (q >= nvis || a.depth_prefix[q] > w0 * kMockWindow) std::abort();
+                if (q >= nvis || a.depth_prefix[q] > w0 * (uint32_t)kEmitWindow) std::abort();
             }
             if (a.tile_hist)
                 for (int k = 0; k < a.tile_passes; ++k) a.tile_hist[k * 256] += std::min<uint32_t>(K, (uint32_t)a.capacity);
