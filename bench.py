@@ -68,7 +68,7 @@ def parse():
     ap.add_argument("--serving-rates", default="400,800,1200,1600,2000,2400,3200",
                     help="C4 Poisson arrival rates (req/s), swept upwards until p99 passes --p99-bound-ms")
     ap.add_argument("--p99-bound-ms", type=float, default=50.0)
-    ap.add_argument("--c5-binding-budget-gib", type=float, default=24.0,
+    ap.add_argument("--c5-binding-budget-gib", type=float, default=16.0,
                     help="C5 again under resident scenes + this many GiB (a budget that binds)")
     ap.add_argument("--serving-accuracy", type=int, default=1,
                     help="one served view per resolution against the FP64 render")
