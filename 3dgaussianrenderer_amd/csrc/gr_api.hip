@@ -448,7 +448,6 @@ gr_status gr_rasterize_tile(gr_context* ctx, gr_timer* timer) {
     a.nviews = 1;
     a.count = g.splat_count;
     a.tiles = a.v0.tiles;
-    a.tiles_per_xcd = (a.tiles + 7) / 8;
     a.tile_range = g.tile_range;
     a.values = ctx->tl.values;
     a.clip_xy = reinterpret_cast<const float2*>(g.position_clip_xy);

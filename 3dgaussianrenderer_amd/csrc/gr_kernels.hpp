@@ -204,7 +204,7 @@ struct RasterArgs {
     const float4* color;
     int32_t count;
     int32_t tiles;               // all views
-    int32_t tiles_per_xcd;
+    const uint32_t* order;       // tile of workgroup b (NULL: tile b)
     int32_t nviews;
     ViewDesc v0;
     const ViewDesc* views;
@@ -259,6 +259,8 @@ hipError_t launch_expand_keys(const uint64_t* entries, const float* clip_z, uint
 hipError_t launch_fill_i32(int32_t* p, int32_t value, int64_t n, hipStream_t stream);
 hipError_t launch_ranges(const RangeArgs& a, hipStream_t stream);
 hipError_t launch_raster(const RasterArgs& a, hipStream_t stream);
+// order[0..tiles) = the tiles by decreasing entry count (classes of powers of two), for RasterArgs::order
+hipError_t launch_tile_order(const int32_t* tile_range, int32_t tiles, uint32_t* order, hipStream_t stream);
 
 // radix sort (gr_sort.hip)
 size_t radix_sort_temp_bytes(int32_t count, int32_t begin_bit, int32_t end_bit);

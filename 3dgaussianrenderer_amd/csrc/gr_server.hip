@@ -387,7 +387,7 @@ std::vector<PhaseANeed> phase_a_needs(Slot& sl, const Scene& sc, size_t nviews, 
             {&w.emit_recs, V * N * 16},
             {&w.rec_prefix, V * N * 4},
             {&w.images, img_bytes},
-            {&w.ranges, 8 * (int64_t)tiles_total},
+            {&w.ranges, 12 * (int64_t)tiles_total},   // ranges, then the raster's tile order
             {&w.views, V * (int64_t)sizeof(ViewDesc)}};
 }
 
@@ -586,13 +586,15 @@ gr_status enqueue_phase_b(gr_server* s, Slot& sl, const Scene& sc) {
                                    w.sort_temp.bytes, sl.stream, nullptr, nullptr, &r));
     RasterArgs ra{};
     ra.tile_range = w.ranges.as<int32_t>();
+    uint32_t* order = w.ranges.as<uint32_t>() + 2 * (size_t)sl.tiles_total;
+    S_HIP(launch_tile_order(ra.tile_range, sl.tiles_total, order, sl.stream));
+    ra.order = order;
     ra.entries = kb[sel];
     ra.clip_xy = w.xy.as<float2>();
     ra.conic = w.con.as<float4>();
     ra.color = w.color.as<float4>();
     ra.count = N;
     ra.tiles = sl.tiles_total;
-    ra.tiles_per_xcd = (sl.tiles_total + 7) / 8;
     ra.nviews = V;
     ra.v0 = vhost[0];
     ra.views = vdev;

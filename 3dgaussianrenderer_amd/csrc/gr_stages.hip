@@ -1323,7 +1323,10 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
     __shared__ int s_alive[2][4];
 
     const int bid = blockIdx.x;
-    const int tile = (bid & 7) * a.tiles_per_xcd + (bid >> 3);
+    // consecutive workgroups go to the 8 XCDs in turn, so each XCD takes every 8th tile: the
+    // XCDs' loads stay even wherever the heavy tiles are (giving each XCD a contiguous eighth of
+    // the tiles, for L2 reuse of shared Gaussians, measured 219 against 213 us at 6M/1080p)
+    const int tile = a.order ? (int)a.order[bid] : bid;
     if (tile >= a.tiles) return;   // uniform over the workgroup
     int view = 0;
     if (a.views) {
@@ -1383,7 +1386,11 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
                 };
                 int j = b0;
                 for (; j + 1 < b1; j += 2) {
-                    const RasterEntry* pe = &s_e[j];
+                    // the pair's LDS address in a vector register (an empty asm keeps the compiler
+                    // from rebuilding it from the scalar index before every read)
+                    int jv = j * (int)sizeof(RasterEntry);
+                    asm volatile("" : "+v"(jv));
+                    const RasterEntry* pe = reinterpret_cast<const RasterEntry*>(reinterpret_cast<const char*>(s_e) + jv);
                     blend(pe[0]);
                     blend(pe[1]);
                 }
@@ -1412,9 +1419,40 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
     }
 }
 
+// Raster order: the tiles by decreasing entry count, in classes of powers of two (order within a
+// class unspecified), so that the heaviest tiles start first and the lightest fill in at the end.
+// One workgroup; tiles without entries (range -1) come last.
+__global__ __launch_bounds__(1024) void tile_order_kernel(const int32_t* range, int32_t tiles, uint32_t* order) {
+    __shared__ uint32_t s_n[33];
+    __shared__ uint32_t s_at[33];
+    if (threadIdx.x < 33) s_n[threadIdx.x] = 0u;
+    __syncthreads();
+    auto cls = [&](int t) {   // leading zeros of the entry count: 0 = most entries, 32 = none
+        return __clz(max(range[2 * (size_t)t + 1] - range[2 * (size_t)t], 0));
+    };
+    for (int t = threadIdx.x; t < tiles; t += blockDim.x) atomicAdd(&s_n[cls(t)], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0u;
+        for (int c = 0; c < 33; ++c) {
+            s_at[c] = run;
+            run += s_n[c];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < tiles; t += blockDim.x) order[atomicAdd(&s_at[cls(t)], 1u)] = (uint32_t)t;
+}
+
+hipError_t launch_tile_order(const int32_t* tile_range, int32_t tiles, uint32_t* order, hipStream_t s) {
+    if (tiles <= 0) return hipSuccess;
+    if (!tile_range || !order) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(1024), 0, s, tile_range, tiles, order);
+    return hipGetLastError();
+}
+
 hipError_t launch_raster(const RasterArgs& a, hipStream_t s) {
     if (a.tiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(raster_kernel, dim3(8 * a.tiles_per_xcd), dim3(kRasterThreads), 0, s, a);
+    hipLaunchKernelGGL(raster_kernel, dim3((unsigned)a.tiles), dim3(kRasterThreads), 0, s, a);
     return hipGetLastError();
 }
 

@@ -430,9 +430,26 @@ hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
     return hipSuccess;
 }
 
+hipError_t launch_tile_order(const int32_t* tile_range, int32_t tiles, uint32_t* order, hipStream_t s) {
+    if (tiles <= 0) return hipSuccess;
+    if (!tile_range || !order) return hipErrorInvalidValue;
+    run_or_record(s, [=] {
+        rd(tile_range, (size_t)tiles * 8);
+        for (int32_t t = 0; t < tiles; ++t) order[t] = (uint32_t)t;
+    });
+    return hipSuccess;
+}
+
 hipError_t launch_raster(const RasterArgs& a, hipStream_t s) {
     run_or_record(s, [=] {
         rd(a.tile_range, (size_t)a.tiles * 8);
+        if (a.order) {   // a permutation of the tiles
+            std::vector<char> seen((size_t)a.tiles, 0);
+            for (int32_t t = 0; t < a.tiles; ++t) {
+                if (a.order[t] >= (uint32_t)a.tiles || seen[a.order[t]]) std::abort();
+                seen[a.order[t]] = 1;
+            }
+        }
         const bool empty = a.tile_range[0] == kNoRanges;   // every tile empty: RGBA 0
         int tiles = 0;
         for (int v = 0; v < a.nviews; ++v) {
