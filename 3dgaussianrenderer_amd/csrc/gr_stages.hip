@@ -1300,6 +1300,14 @@ hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
 // reaches, so such an entry's skip threshold is -inf (never skipped).
 // ---------------------------------------------------------------------------
 constexpr int kRasterThreads = 256;
+// Entries staged through LDS per round (a multiple of the 32-entry vote).  Most tiles stop after
+// 100-400 entries, so smaller rounds gather fewer entries that are never blended: at 6M/1080p,
+// 256 / 128 / 64 entries per round, raster 210 / 207 / 211 us.
+#ifndef GR_RASTER_STAGE
+#define GR_RASTER_STAGE 128
+#endif
+constexpr int kRasterStage = GR_RASTER_STAGE;
+static_assert(kRasterStage % 32 == 0 && kRasterStage <= kRasterThreads, "stage whole vote batches, one per lane");
 constexpr int kRasterBatch = 32;
 
 // blend_alpha with the clamp of e to [-26, 0] as one v_med3_f32: the same value
@@ -1319,7 +1327,7 @@ struct __align__(16) RasterEntry {
 };
 
 __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
-    __shared__ RasterEntry s_e[kRasterThreads];
+    __shared__ RasterEntry s_e[kRasterStage];
     __shared__ int s_alive[2][4];
 
     const int bid = blockIdx.x;
@@ -1351,8 +1359,8 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
     int parity = 0;
     if (n > 0) {
         bool stop = false;
-        for (int r0 = 0; r0 < n && !stop; r0 += kRasterThreads) {
-            const int nr = min(kRasterThreads, n - r0);
+        for (int r0 = 0; r0 < n && !stop; r0 += kRasterStage) {
+            const int nr = min(kRasterStage, n - r0);
             __syncthreads();
             if ((int)threadIdx.x < nr) {
                 const int k = start + r0 + threadIdx.x;
