@@ -1108,21 +1108,44 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_packed_kernel(const ui
                                                                         const uint32_t* sums, const uint32_t* valid,
                                                                         const uint4* erec, int ib, uint32_t* prefix,
                                                                         uint32_t* windows, int64_t nwin) {
+    static_assert(kScanItems == 8, "two 16-byte loads and stores per lane");
     __shared__ uint32_t s_w[4];
     const int64_t my = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
     const int64_t nv = (int64_t)min(*valid, (uint32_t)n);
+    const bool whole = my + kScanItems <= nv;   // the lane's 8 words in range: 16-byte accesses
+    uint32_t wd[kScanItems];
+    if (whole) {
+        const uint4 lo = *reinterpret_cast<const uint4*>(words + my);
+        const uint4 hi = *reinterpret_cast<const uint4*>(words + my + 4);
+        wd[0] = lo.x; wd[1] = lo.y; wd[2] = lo.z; wd[3] = lo.w;
+        wd[4] = hi.x; wd[5] = hi.y; wd[6] = hi.z; wd[7] = hi.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) wd[k] = (my + k < nv) ? words[my + k] : 0u;
+    }
     uint32_t v[kScanItems];
     uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (my + k < nv) ? packed_count(words[my + k], ib, erec) : 0u;
+        v[k] = (my + k < nv) ? packed_count(wd[k], ib, erec) : 0u;
         sum += v[k];
     }
     uint32_t run = block_excl_256(sum, sums[blockIdx.x], s_w);
+    uint32_t pf[kScanItems];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        pf[k] = run;
+        run += v[k];
+    }
+    if (whole) {
+        *reinterpret_cast<uint4*>(prefix + my) = make_uint4(pf[0], pf[1], pf[2], pf[3]);
+        *reinterpret_cast<uint4*>(prefix + my + 4) = make_uint4(pf[4], pf[5], pf[6], pf[7]);
+    }
+    run = pf[0];
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         if (my + k < nv) {
-            prefix[my + k] = run;
+            if (!whole) prefix[my + k] = run;
             const uint64_t end = (uint64_t)run + v[k];
             for (uint64_t b = ((uint64_t)run + kEmitWindow - 1) / kEmitWindow; b * kEmitWindow < end && (int64_t)b < nwin;
                  ++b)
