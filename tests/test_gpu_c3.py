@@ -68,11 +68,12 @@ def test_c3_fp32(torch_cuda, orc, c3, size):
     r = gr.Renderer()
     r.upload(scene)
     img, st = r.render(cam, W, H)
-    keys, vals, _ = frame_list(r)
+    keys, vals, rng = frame_list(r)
     r.close()
     ref = orc.render(scene["position"], scene["scale_rot"], shal, None, N, 3, cam.packed(), W, H)
     assert st["tile_list_size"] == ref["k"]
     assert np.array_equal(keys, ref["keys"]) and np.array_equal(vals, ref["values"])
+    assert np.array_equal(rng, orc.ranges(ref["keys"], len(rng) // 2)), "tile ranges"
     assert np.array_equal(img, ref["image"])
     r64 = fp64(orc, scene, shal, size)
     p = orc.psnr(img, r64["image"])
@@ -115,7 +116,8 @@ def test_c3_precision(torch_cuda, orc, c3, size, precision):
 
 def test_c3_fp32_all_poses_1080p(torch_cuda, orc, c3):
     """The other seven seeded poses of the 1080p workload (pose 0 is test_c3_fp32): each frame's
-    tile list and image bit-exact against the FP32 oracle, and >= 50 dB against the FP64 render."""
+    tile list, tile ranges and image bit-exact against the FP32 oracle, and >= 50 dB against the
+    FP64 render."""
     scene, shal = c3
     W, H = SIZES["1080p"]
     r = gr.Renderer()
@@ -125,10 +127,11 @@ def test_c3_fp32_all_poses_1080p(torch_cuda, orc, c3):
             say(f"c3 fp32 1080p pose {pose}")
             cam = camera(scene, W, H, pose)
             img, st = r.render(cam, W, H)
-            keys, vals, _ = frame_list(r)
+            keys, vals, rng = frame_list(r)
             ref = orc.render(scene["position"], scene["scale_rot"], shal, None, N, 3, cam.packed(), W, H)
             assert st["tile_list_size"] == ref["k"], f"pose {pose}"
             assert np.array_equal(keys, ref["keys"]) and np.array_equal(vals, ref["values"]), f"pose {pose}"
+            assert np.array_equal(rng, orc.ranges(ref["keys"], len(rng) // 2)), f"pose {pose}: tile ranges"
             assert np.array_equal(img, ref["image"]), f"pose {pose}"
             r64 = orc.render64(scene["position"], scene["scale_rot"], shal, scene["color"], N, 3, cam.packed(), W, H)
             p = orc.psnr(img, r64["image"])
@@ -139,3 +142,26 @@ def test_c3_fp32_all_poses_1080p(torch_cuda, orc, c3):
     finally:
         r.close()
 
+
+def test_c3_fp32_all_poses_4k(torch_cuda, orc, c3):
+    """The other seven seeded poses at 3840x2160 (15 tile bits: two passes of 8 + 7): tile list,
+    tile ranges and image bit-exact against the FP32 oracle (the FP64 comparison of the 4K frame
+    is test_c3_fp32 at pose 0)."""
+    scene, shal = c3
+    W, H = SIZES["4k"]
+    r = gr.Renderer()
+    r.upload(scene)
+    try:
+        for pose in range(1, 8):
+            say(f"c3 fp32 4k pose {pose}")
+            cam = camera(scene, W, H, pose)
+            img, st = r.render(cam, W, H)
+            keys, vals, rng = frame_list(r)
+            ref = orc.render(scene["position"], scene["scale_rot"], shal, None, N, 3, cam.packed(), W, H)
+            assert st["tile_list_size"] == ref["k"], f"pose {pose}"
+            assert np.array_equal(keys, ref["keys"]) and np.array_equal(vals, ref["values"]), f"pose {pose}"
+            assert np.array_equal(rng, orc.ranges(ref["keys"], len(rng) // 2)), f"pose {pose}: tile ranges"
+            assert np.array_equal(img, ref["image"]), f"pose {pose}"
+            say(f"[c3] fp32 4k pose {pose}: K {ref['k']}, tile list, ranges and image == FP32 oracle")
+    finally:
+        r.close()
