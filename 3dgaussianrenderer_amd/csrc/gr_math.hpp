@@ -192,30 +192,31 @@ GR_HD int tile_bits_for(int tiles) {
 // exp2 used by the blend.  The reference calls CUDA's __expf (closed code, its
 // bits are not reproducible here).  This build defines density = 2^e with its
 // own polynomial so that the GPU and the host oracle agree bit for bit:
-//   n = rint(e), f = e - n in [-1/2, 1/2], 2^f ~ p(f), p(0) = 1 exactly, a
-//   degree-5 near-minimax relative-error fit (Lawson-weighted least squares on
-//   Chebyshev nodes, tests/golden/fit_exp2.py; max relative error 1.7e-7 with FP32
-//   Horner/FMA evaluation), result = ldexp(p(f), n).  p(f) <= 1 for f <= 0,
-//   so the density never exceeds 1.
-// Callers pass e in [-26, 0] (see blend_alpha below), so n fits an int
-// without saturation.
+//   n = floor(e), f = min(e - n, 1 - 2^-24) in [0, 1) (e - n rounded to nearest
+//   even), 2^f ~ p(f), p(0) = 1 exactly, a degree-4 near-minimax relative-error
+//   fit on [0, 1] (Lawson-weighted least squares on Chebyshev nodes,
+//   tests/golden/fit_exp2.py; max relative error 3.0e-6 with FP32 Horner/FMA
+//   evaluation), result = ldexp(p(f), n).  p(f) < 2 for every f < 1, so the
+//   density never exceeds 1 for e <= 0, and 2^n is exact at integers.
+// On gfx950 f is one v_fract_f32 (the same rounding and clamp) and n one
+// v_cvt_flr_i32_f32.  Callers pass e in [-26, 0] (see blend_alpha_dev in the
+// raster), so n fits an int without saturation.
 // ---------------------------------------------------------------------------
 constexpr float kExp2C0 = 1.0f;
-constexpr float kExp2C1 = 6.931470037e-01f;
-constexpr float kExp2C2 = 2.402224243e-01f;
-constexpr float kExp2C3 = 5.550733581e-02f;
-constexpr float kExp2C4 = 9.671512991e-03f;
-constexpr float kExp2C5 = 1.326472731e-03f;
+constexpr float kExp2C1 = 6.930448413e-01f;
+constexpr float kExp2C2 = 2.412801981e-01f;
+constexpr float kExp2C3 = 5.224247277e-02f;
+constexpr float kExp2C4 = 1.342668477e-02f;
 
-GR_HD float exp2_poly(float e) {
-    const float n = rintf(e);
-    const float f = e - n;
-    float p = fmaf(kExp2C5, f, kExp2C4);
-    p = fmaf(p, f, kExp2C3);
+__device__ __forceinline__ float exp2_poly(float e) {
+    const float f = __builtin_amdgcn_fractf(e);
+    int n;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(n) : "v"(e));
+    float p = fmaf(kExp2C4, f, kExp2C3);
     p = fmaf(p, f, kExp2C2);
     p = fmaf(p, f, kExp2C1);
     p = fmaf(p, f, kExp2C0);
-    return ldexpf(p, (int)n);
+    return ldexpf(p, n);
 }
 
 // Blend weights.  conic (a, b, c) of the reference becomes
@@ -231,12 +232,6 @@ constexpr float kTransmittanceStop = 0.02f;           // core/gaussian_render.cu
 
 GR_HD float blend_exponent(float qa, float qb, float qc, float dx, float dy) {
     return fmaf(dx, fmaf(qa, dx, qb * dy), (qc * dy) * dy);
-}
-GR_HD float blend_alpha(float e, float opacity) {
-    float ec = (e < 0.0f) ? e : 0.0f;          // NaN -> 0 (discarded below)
-    ec = (ec > -26.0f) ? ec : -26.0f;          // keeps rint(ec) in int range
-    const float a = opacity * exp2_poly(ec);
-    return (e >= kMinExponent) ? a : 0.0f;
 }
 
 }  // namespace gr

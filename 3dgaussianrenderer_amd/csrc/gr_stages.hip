@@ -1314,13 +1314,18 @@ hipError_t launch_ranges(const RangeArgs& a, hipStream_t s) {
 // blended front to back; after every 32 entries of the tile the whole tile
 // stops if no pixel has T > 0.02 (:739-748).  Pixels of edge tiles outside the
 // frame take part in the vote but are not written.  The blend arithmetic is
-// the one of gr::blend_exponent / gr::blend_alpha:
+// the one of gr::blend_exponent / blend_density below:
 //   w = T * alpha;  C = fma(rgb, w, C);  T = T - w    (= C += rgb*T*alpha, T *= 1-alpha)
-// A wave skips an entry outright when every lane's exponent is below -25,
-// which is exactly the per-lane rule (alpha = 0 leaves C and T unchanged) as
-// long as the entry's colour is finite: a NaN or infinite channel times a zero
-// weight is NaN, which the reference's blend (:729-736) adds to every pixel it
-// reaches, so such an entry's skip threshold is -inf (never skipped).
+// alpha is 0 for a pair whose exponent is below -25 (or NaN).  While T is finite
+// and the entry's colour is finite, such a pair leaves C and T as they are (up
+// to the sign of a zero, which the output cannot show), so the lane simply sits
+// it out: the blend runs under the mask of the lanes at or above the cutoff,
+// and a wave whose lanes are all below skips the entry.  T stays in [0, 1] as
+// long as every opacity is in [0, 1].  An entry whose colour is not finite (a
+// NaN or infinite channel times a zero weight is NaN, which the reference's
+// blend (:729-736) adds to every pixel it reaches) or whose opacity is outside
+// [0, 1] (after which T need not be finite) is blended by every lane with the
+// explicit zero, and so is every later entry of the tile.
 // ---------------------------------------------------------------------------
 constexpr int kRasterThreads = 256;
 // Entries staged through LDS per round (a multiple of the 32-entry vote).  Most tiles stop after
@@ -1333,20 +1338,20 @@ constexpr int kRasterStage = GR_RASTER_STAGE;
 static_assert(kRasterStage % 32 == 0 && kRasterStage <= kRasterThreads, "stage whole vote batches, one per lane");
 constexpr int kRasterBatch = 32;
 
-// blend_alpha with the clamp of e to [-26, 0] as one v_med3_f32: the same value
-// for every e that is not NaN, and a NaN e gives alpha 0 in both forms.
-__device__ __forceinline__ float blend_alpha_dev(float e, float opacity) {
-    const float ec = __builtin_amdgcn_fmed3f(e, -26.0f, 0.0f);
-    const float al = opacity * exp2_poly(ec);
-    return (e >= kMinExponent) ? al : 0.0f;
+// alpha = opacity * 2^min(e, 0) when e >= -25, else 0 (a pair whose density is below 2^-25
+// contributes nothing; NaN contributes nothing).  The clamp of e to [-26, 0] is one v_med3_f32
+// (min(e, 0) for the e >= -25 that reach it) and keeps floor(e) in int range.
+__device__ __forceinline__ float blend_density(float e) {
+    return exp2_poly(__builtin_amdgcn_fmed3f(e, -26.0f, 0.0f));
 }
 
-// One staged entry: all of it sits at one LDS address + immediate offsets.
+// One staged entry: the exponent's operands in the first 16 bytes and qc right after them, the
+// blend's in the next 16.
 struct __align__(16) RasterEntry {
     float4 a;   // cx, cy, qa, qb
-    float4 b;   // qc, skip threshold (-25, or -inf for a colour that is not finite), opacity, r
-    float2 c;   // g, b
-    float pad[2];
+    float4 b;   // opacity, r, g, b
+    float qc;
+    float pad[3];
 };
 
 __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
@@ -1382,50 +1387,69 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
     int parity = 0;
     if (n > 0) {
         bool stop = false;
+        bool full = false;   // from the round with the tile's first flagged entry on: the full rule
         for (int r0 = 0; r0 < n && !stop; r0 += kRasterStage) {
             const int nr = min(kRasterStage, n - r0);
             __syncthreads();
+            bool flagged = false;
             if ((int)threadIdx.x < nr) {
                 const int k = start + r0 + threadIdx.x;
                 const size_t gi = vo + (size_t)(a.values ? a.values[k] : (int32_t)(a.entries[k] >> 32));
                 const float2 c = a.clip_xy[gi];
                 const float4 q = a.conic[gi];
                 const float4 col = a.color[gi];
-                const bool finite = isfinite(col.x) && isfinite(col.y) && isfinite(col.z);
+                flagged = !(isfinite(col.x) && isfinite(col.y) && isfinite(col.z) && col.w >= 0.0f &&
+                            col.w <= 1.0f);
                 s_e[threadIdx.x].a = make_float4(c.x, c.y, q.x * kHalfLog2e, q.y * kLog2e);
-                s_e[threadIdx.x].b = make_float4(q.z * kHalfLog2e, finite ? kMinExponent : -INFINITY, col.w, col.x);
-                s_e[threadIdx.x].c = make_float2(col.y, col.z);
+                s_e[threadIdx.x].b = make_float4(col.w, col.x, col.y, col.z);
+                s_e[threadIdx.x].qc = q.z * kHalfLog2e;
             }
-            __syncthreads();
+            // the round's entries before the flagged one may take the full rule too: for them
+            // both rules give the same C and T
+            if (__syncthreads_or(flagged ? 1 : 0)) full = true;
             for (int b0 = 0; b0 < nr; b0 += kRasterBatch) {
                 const int b1 = min(b0 + kRasterBatch, nr);
-                // two entries per step from one LDS base address (immediate offsets)
-                auto blend = [&](const RasterEntry& en) {
+                auto exponent = [&](const RasterEntry& en) {
                     const float4 ea = en.a;
                     const float dx = pcx - ea.x, dy = pcy - ea.y;
-                    const float4 eb = en.b;
-                    const float e = blend_exponent(ea.z, ea.w, eb.x, dx, dy);
-                    if (__ballot(!(e < eb.y)) != 0ull) {   // some lane not below the threshold (or NaN)
-                        const float al = blend_alpha_dev(e, eb.z);
-                        const float wgt = T * al;
-                        const float2 gb = en.c;
-                        C0 = fmaf(eb.w, wgt, C0);
-                        C1 = fmaf(gb.x, wgt, C1);
-                        C2 = fmaf(gb.y, wgt, C2);
+                    return blend_exponent(ea.z, ea.w, en.qc, dx, dy);
+                };
+                if (full) {
+                    // every lane applies the rule, alpha 0 below the cutoff included
+                    for (int j = b0; j < b1; ++j) {
+                        const float e = exponent(s_e[j]);
+                        const float4 eb = s_e[j].b;
+                        const float wgt = T * ((e >= kMinExponent) ? eb.x * blend_density(e) : 0.0f);
+                        C0 = fmaf(eb.y, wgt, C0);
+                        C1 = fmaf(eb.z, wgt, C1);
+                        C2 = fmaf(eb.w, wgt, C2);
                         T = T - wgt;
                     }
-                };
-                int j = b0;
-                for (; j + 1 < b1; j += 2) {
-                    // the pair's LDS address in a vector register (an empty asm keeps the compiler
-                    // from rebuilding it from the scalar index before every read)
-                    int jv = j * (int)sizeof(RasterEntry);
-                    asm volatile("" : "+v"(jv));
-                    const RasterEntry* pe = reinterpret_cast<const RasterEntry*>(reinterpret_cast<const char*>(s_e) + jv);
-                    blend(pe[0]);
-                    blend(pe[1]);
+                } else {
+                    auto blend = [&](const RasterEntry& en) {
+                        const float e = exponent(en);
+                        if (e >= kMinExponent) {   // lanes below the cutoff (or NaN) sit the entry out
+                            const float4 eb = en.b;
+                            const float wgt = T * (eb.x * blend_density(e));
+                            C0 = fmaf(eb.y, wgt, C0);
+                            C1 = fmaf(eb.z, wgt, C1);
+                            C2 = fmaf(eb.w, wgt, C2);
+                            T = T - wgt;
+                        }
+                    };
+                    int j = b0;
+                    for (; j + 1 < b1; j += 2) {
+                        // the pair's LDS address in a vector register (an empty asm keeps the
+                        // compiler from rebuilding it from the scalar index before every read)
+                        int jv = j * (int)sizeof(RasterEntry);
+                        asm volatile("" : "+v"(jv));
+                        const RasterEntry* pe =
+                            reinterpret_cast<const RasterEntry*>(reinterpret_cast<const char*>(s_e) + jv);
+                        blend(pe[0]);
+                        blend(pe[1]);
+                    }
+                    if (j < b1) blend(s_e[j]);
                 }
-                if (j < b1) blend(s_e[j]);
                 const bool alive = __ballot(T > kTransmittanceStop) != 0ull;
                 if (lane == 0) s_alive[parity][w] = alive ? 1 : 0;
                 __syncthreads();

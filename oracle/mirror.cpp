@@ -495,17 +495,17 @@ void ranges(const uint64_t* keys, int64_t k, int tiles, int32_t* range) {
 }
 
 // ---- blend arithmetic of this build (FP32) ----
-constexpr float kE2[6] = {1.0f, 6.931470037e-01f, 2.402224243e-01f, 5.550733581e-02f, 9.671512991e-03f,
-                          1.326472731e-03f};
+// 2^e as gr::exp2_poly (gr_math.hpp): n = floor(e), f = min(e - n, 1 - 2^-24) (what gfx950's
+// v_fract_f32 returns), degree-4 polynomial of tests/golden/fit_exp2.py, ldexp.
+constexpr float kE2[5] = {1.0f, 6.930448413e-01f, 2.412801981e-01f, 5.224247277e-02f, 1.342668477e-02f};
 inline float exp2_poly(float e) {
-    const float nn = std::nearbyint(e);  // round half to even
-    const float f = e - nn;
-    float p = std::fma(kE2[5], f, kE2[4]);
-    p = std::fma(p, f, kE2[3]);
+    const float fl = std::floor(e);
+    const float f = std::fmin(e - fl, 0x1.fffffep-1f);
+    float p = std::fma(kE2[4], f, kE2[3]);
     p = std::fma(p, f, kE2[2]);
     p = std::fma(p, f, kE2[1]);
     p = std::fma(p, f, kE2[0]);
-    return std::ldexp(p, (int)nn);
+    return std::ldexp(p, (int)fl);
 }
 
 // Raster FP32 (core/gaussian_render.cu:679-759 with the blend above).

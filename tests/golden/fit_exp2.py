@@ -3,27 +3,32 @@
 oracle/mirror.cpp).
 
 The blend needs density = 2^e for e in [-26, 0].  The build evaluates it as
-n = rint(e), f = e - n in [-1/2, 1/2], 2^f ~ p(f), result = ldexp(p(f), n), so
-that the GPU and the host oracle compute the same bits (CUDA's __expf, which
-the reference calls at core/gaussian_render.cu:733, is closed code).
+n = floor(e), f = min(e - n, 1 - 2^-24) in [0, 1), 2^f ~ p(f),
+result = ldexp(p(f), n), so that the GPU and the host oracle compute the same
+bits (CUDA's __expf, which the reference calls at core/gaussian_render.cu:733,
+is closed code).  On gfx950 f is one v_fract_f32 (x - floor(x), rounded to
+nearest even and clamped below 1) and n one v_cvt_flr_i32_f32.
 
-p is a degree-5 polynomial with p(0) = 1 exactly and a near-minimax relative
-error on [-1/2, 1/2]: Lawson-weighted least squares (iteratively reweighted
-by the current error) on 4000 Chebyshev nodes.  The script prints the fitted
-FP32 coefficients (c0 .. c5, the constants kExp2C0 .. kExp2C5) and the
-maximum relative error of the FP32 Horner/FMA evaluation on a dense grid
-(about 1.7e-7).  The committed constants were produced by this script; the
-oracle's tests pin the polynomial's accuracy against numpy's exp2
-(tests/test_oracle.py).
+p is a degree-4 polynomial with p(0) = 1 exactly and a near-minimax relative
+error on [0, 1]: Lawson-weighted least squares (iteratively reweighted by the
+current error) on 4000 Chebyshev nodes.  The script prints the fitted FP32
+coefficients (c0 .. c4, the constants kExp2C0 .. kExp2C4), the maximum
+relative error of the FP32 Horner/FMA evaluation on a dense grid of [0, 1)
+(about 3.0e-6) and p at the largest f below 1, which must stay below 2 so that
+2^e <= 1 for every e <= 0.  The committed constants were produced by this
+script; the oracle's tests pin the polynomial's accuracy against numpy's exp2,
+its exact powers of two and its bound by 1 (tests/test_oracle.py).
 
     python3 tests/golden/fit_exp2.py
 """
 import numpy as np
 
+DEGREE = 4
 
-def fit(deg: int = 5, nodes: int = 4000, iters: int = 300) -> np.ndarray:
-    """FP32 coefficients c0..c_deg of p(f) = 1 + sum_{k>=1} c_k f^k."""
-    x = np.cos(np.pi * (np.arange(nodes) + 0.5) / nodes) * 0.5
+
+def fit(deg: int = DEGREE, nodes: int = 4000, iters: int = 300) -> np.ndarray:
+    """FP32 coefficients c0..c_deg of p(f) = 1 + sum_{k>=1} c_k f^k, fitted on [0, 1]."""
+    x = (np.cos(np.pi * (np.arange(nodes) + 0.5) / nodes) + 1.0) * 0.5
     y = 2.0 ** x
     w = np.ones_like(x)
     c = None
@@ -52,10 +57,11 @@ def horner_fp32(c32: np.ndarray, xx: np.ndarray) -> np.ndarray:
 
 def main() -> None:
     c32 = fit()
-    xx = np.linspace(-0.5, 0.5, 400001).astype(np.float32)
+    xx = np.linspace(0.0, 1.0, 400001).astype(np.float32)
+    xx[-1] = np.nextafter(np.float32(1.0), np.float32(0.0))   # the largest f
     p = horner_fp32(c32, xx)
     rel = np.abs(p.astype(np.float64) / 2.0 ** xx.astype(np.float64) - 1)
-    print("max rel err", rel.max(), "p(0)=", p[200000], "max p on [-.5,0]", p[:200001].max())
+    print("max rel err", rel.max(), "p(0) =", p[0], "p(1 - 2^-24) =", p[-1], "(< 2:", bool(p.max() < 2.0), ")")
     for v in c32:
         print("%.9e" % v)
 

@@ -417,12 +417,23 @@ def test_raster_brute_force_tiny_scenes(orc):
 
 
 def test_exp2_polynomial_accuracy(orc):
+    """The blend's 2^e (degree-4 polynomial on [0, 1) after floor / fraction, fitted by
+    tests/golden/fit_exp2.py): relative error against numpy's exp2 below 3.5e-6 on [-25, 0]
+    (a dropped or mistyped coefficient costs 1e-4 or more), exact at the integers (p(0) = 1 and
+    the exponent from floor), never above 1, and continuous across the integers (a floor/fraction
+    mix-up jumps by a factor of 2 there)."""
     e = np.linspace(-25.0, 0.0, 200_001).astype(np.float32)
     got = np.array([orc.exp2(float(v)) for v in e[::50]], np.float64)
     ref = np.exp2(e[::50].astype(np.float64))
-    assert np.max(np.abs(got / ref - 1)) < 2.5e-7
-    assert orc.exp2(0.0) == 1.0
-    assert orc.exp2(-0.5) <= 1.0 and orc.exp2(-1e-8) <= 1.0
+    assert np.max(np.abs(got / ref - 1)) < 3.5e-6
+    for k in range(0, 27):
+        assert orc.exp2(float(-k)) == 2.0 ** -k
+    assert orc.exp2(-0.5) <= 1.0 and orc.exp2(-1e-8) <= 1.0 and orc.exp2(-1e-30) <= 1.0
+    assert max(orc.exp2(float(v)) for v in e[-2000:]) <= 1.0
+    for k in range(1, 26):
+        below = orc.exp2(float(np.nextafter(np.float32(-k), np.float32(-30.0))))
+        above = orc.exp2(float(np.nextafter(np.float32(-k), np.float32(0.0))))
+        assert abs(below / 2.0 ** -k - 1) < 1e-5 and abs(above / 2.0 ** -k - 1) < 1e-5
 
 
 # ------------------------------------------------------------------ whole frame
