@@ -7,6 +7,8 @@ Everything is compiled in-tree with explicit hipcc / g++ command lines:
 * ``gr_headless``    -- headless C++ host (replacement for the GLFW app).
 * ``oracle/liborc_mirror.so`` -- CPU oracle, test infrastructure only
                         (built with the oracle's own Makefile).
+* ``tests/gpu_probe/libgr_exp2_probe.so`` -- test infrastructure: the
+                        exhaustive GPU check of the blend's exp2.
 
 HIP flags: ``-ffp-contract=off`` (no implicit FMA: the tile lists must match the
 host oracle bit for bit), IEEE division/sqrt and denormals, packed-FP32 VALU
@@ -27,6 +29,8 @@ INCLUDE = ROOT / "include"
 LIB = PKG / "libgr_hip.so"
 HEADLESS = PKG / "gr_headless"
 ORACLE_DIR = ROOT / "oracle"
+PROBE_DIR = ROOT / "tests" / "gpu_probe"
+PROBE_LIB = PROBE_DIR / "libgr_exp2_probe.so"
 
 HIP_SOURCES = ["gr_stages.hip", "gr_sort.hip", "gr_api.hip", "gr_renderer.hip", "gr_server.hip"]
 CXX_SOURCES = ["gr_scene.cpp", "gauss_render.cpp", "gr_ply.cpp", "gr_image.cpp"]
@@ -98,10 +102,19 @@ def build_oracle(verbose: bool = False) -> Path:
     return ORACLE_DIR / "liborc_mirror.so"
 
 
+def build_probes(verbose: bool = False) -> Path:
+    """Test infrastructure: the exhaustive GPU check of the blend's exp2 (tests/test_gpu_exp2.py)."""
+    tmp = PROBE_LIB.with_suffix(".so.tmp")
+    _run([HIPCC, *HIP_FLAGS, f"-I{CSRC}", "-shared", str(PROBE_DIR / "exp2_probe.hip"), "-o", str(tmp)], verbose)
+    os.replace(tmp, PROBE_LIB)
+    return PROBE_LIB
+
+
 def build_all(verbose: bool = False) -> None:
     build_library(verbose)
     build_headless(verbose)
     build_oracle(verbose)
+    build_probes(verbose)
 
 
 if __name__ == "__main__":

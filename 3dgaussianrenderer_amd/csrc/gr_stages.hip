@@ -984,9 +984,10 @@ hipError_t launch_record_slots(const uint32_t* offsets, const uint32_t* total, i
     return hipGetLastError();
 }
 
-// Exclusive scan of the per-batch counts, in place, in three launches
-// (reduce per 2048-element block, scan of the block sums, local scan + block
-// offset); the grand total (saturated at 2^32-1) goes to *total.
+// Exclusive scan of the per-batch counts, in place, in two launches (reduce
+// per 2048-element block; local scan + block offset, each block summing the
+// block sums before its own); the grand total (saturated at 2^32-1) goes to
+// *total.
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
@@ -1039,32 +1040,23 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_packed_kernel(const 
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(1024) void scan_sums_kernel(uint32_t* sums, int32_t nb, uint32_t* total) {
-    __shared__ uint64_t s_w[16];
-    __shared__ uint64_t s_carry;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (threadIdx.x == 0) s_carry = 0;
-    __syncthreads();
-    for (int base = 0; base < nb; base += 1024) {
-        const int i = base + threadIdx.x;
-        const uint64_t v = i < nb ? sums[i] : 0u;
-        unsigned long long incl = v;
+// The offset of this block's tile: the sum of the reduce pass's per-block sums before it,
+// saturated to 32 bits, summed by the block itself (so that no launch sits between the two
+// passes); the last block also writes the total (saturated).
+__device__ __forceinline__ uint32_t block_offset_256(const uint32_t* sums, uint32_t* total, uint64_t* s_w64) {
+    const int b = blockIdx.x;
+    const bool last = b + 1 == (int)gridDim.x;
+    const int upto = last ? b + 1 : b;
+    unsigned long long v = 0;
+    for (int i = threadIdx.x; i < upto; i += kScanThreads) v += sums[i];
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const unsigned long long o = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += o;
-        }
-        if (lane == 63) s_w[w] = incl;
-        __syncthreads();
-        uint64_t off = s_carry;
-        for (int k = 0; k < w; ++k) off += s_w[k];
-        const uint64_t ex = off + incl - v;
-        if (i < nb) sums[i] = (uint32_t)(ex > 0xffffffffull ? 0xffffffffull : ex);
-        __syncthreads();
-        if (threadIdx.x == 1023) s_carry = off + incl;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *total = (uint32_t)(s_carry > 0xffffffffull ? 0xffffffffull : s_carry);
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_down(v, d, 64);
+    if ((threadIdx.x & 63) == 0) s_w64[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const uint64_t all = s_w64[0] + s_w64[1] + s_w64[2] + s_w64[3];
+    const uint64_t excl = last ? all - sums[b] : all;
+    if (last && threadIdx.x == 0) *total = (uint32_t)(all > 0xffffffffull ? 0xffffffffull : all);
+    return (uint32_t)(excl > 0xffffffffull ? 0xffffffffull : excl);
 }
 
 // Local scan of kScanItems values per lane plus the block's offset from the
@@ -1084,8 +1076,10 @@ __device__ __forceinline__ uint32_t block_excl_256(uint32_t sum, uint32_t block_
     return run;
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data, int32_t n, const uint32_t* sums) {
+__global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data, int32_t n, const uint32_t* sums,
+                                                                 uint32_t* total) {
     __shared__ uint32_t s_w[4];
+    __shared__ uint64_t s_w64[4];
     const int64_t my = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
     uint32_t sum = 0;
@@ -1094,7 +1088,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data,
         v[k] = (my + k < n) ? data[my + k] : 0u;
         sum += v[k];
     }
-    uint32_t run = block_excl_256(sum, sums[blockIdx.x], s_w);
+    uint32_t run = block_excl_256(sum, block_offset_256(sums, total, s_w64), s_w);
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         if (my + k < n) data[my + k] = run;
@@ -1107,9 +1101,11 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(uint32_t* data,
 __global__ __launch_bounds__(kScanThreads) void scan_down_packed_kernel(const uint32_t* words, int32_t n,
                                                                         const uint32_t* sums, const uint32_t* valid,
                                                                         const uint4* erec, int ib, uint32_t* prefix,
-                                                                        uint32_t* windows, int64_t nwin) {
+                                                                        uint32_t* windows, int64_t nwin,
+                                                                        uint32_t* total) {
     static_assert(kScanItems == 8, "two 16-byte loads and stores per lane");
     __shared__ uint32_t s_w[4];
+    __shared__ uint64_t s_w64[4];
     const int64_t my = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
     const int64_t nv = (int64_t)min(*valid, (uint32_t)n);
     const bool whole = my + kScanItems <= nv;   // the lane's 8 words in range: 16-byte accesses
@@ -1130,7 +1126,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_packed_kernel(const ui
         v[k] = (my + k < nv) ? packed_count(wd[k], ib, erec) : 0u;
         sum += v[k];
     }
-    uint32_t run = block_excl_256(sum, sums[blockIdx.x], s_w);
+    uint32_t run = block_excl_256(sum, block_offset_256(sums, total, s_w64), s_w);
     uint32_t pf[kScanItems];
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
@@ -1164,8 +1160,7 @@ hipError_t launch_build_scan(uint32_t* data, int32_t n, uint32_t* total, uint32_
     }
     const int nb = (n + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch);
-    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, scratch, nb, total);
-    hipLaunchKernelGGL(scan_down_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch);
+    hipLaunchKernelGGL(scan_down_kernel, dim3(nb), dim3(kScanThreads), 0, s, data, n, scratch, total);
     return hipGetLastError();
 }
 
@@ -1188,9 +1183,8 @@ hipError_t launch_build_scan_packed(const uint32_t* words, int32_t n, const uint
     const int nb = (n + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(scan_reduce_packed_kernel, dim3(nb), dim3(kScanThreads), 0, s, words, n, scratch, valid, erec,
                        index_bits);
-    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, scratch, nb, total);
     hipLaunchKernelGGL(scan_down_packed_kernel, dim3(nb), dim3(kScanThreads), 0, s, words, n, scratch, valid, erec,
-                       index_bits, prefix, windows, nwindows);
+                       index_bits, prefix, windows, nwindows, total);
     return hipGetLastError();
 }
 
@@ -1482,10 +1476,27 @@ __global__ __launch_bounds__(1024) void tile_order_kernel(const int32_t* range, 
     __shared__ uint32_t s_at[33];
     if (threadIdx.x < 33) s_n[threadIdx.x] = 0u;
     __syncthreads();
-    auto cls = [&](int t) {   // leading zeros of the entry count: 0 = most entries, 32 = none
-        return __clz(max(range[2 * (size_t)t + 1] - range[2 * (size_t)t], 0));
+    const int lane = threadIdx.x & 63;
+    // the lanes of the wave in the same class as this one (6 ballots over the class bits), so that
+    // one lane per class and wave does the LDS atomic: most tiles fall into a few classes, whose
+    // counters would otherwise take one atomic per tile in turn
+    auto peers_of = [&](uint32_t c) {
+        uint64_t m = __ballot(1);
+#pragma unroll
+        for (int bit = 0; bit < 6; ++bit) {
+            const uint64_t v = __ballot((c >> bit) & 1u);
+            m &= ((c >> bit) & 1u) ? v : ~v;
+        }
+        return m;
     };
-    for (int t = threadIdx.x; t < tiles; t += blockDim.x) atomicAdd(&s_n[cls(t)], 1u);
+    auto cls = [&](int t) {   // leading zeros of the entry count: 0 = most entries, 32 = none
+        return (uint32_t)__clz(max(range[2 * (size_t)t + 1] - range[2 * (size_t)t], 0));
+    };
+    for (int t = threadIdx.x; t < tiles; t += blockDim.x) {
+        const uint32_t c = cls(t);
+        const uint64_t m = peers_of(c);
+        if (lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&s_n[c], (uint32_t)__popcll(m));
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t run = 0u;
@@ -1495,7 +1506,16 @@ __global__ __launch_bounds__(1024) void tile_order_kernel(const int32_t* range, 
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < tiles; t += blockDim.x) order[atomicAdd(&s_at[cls(t)], 1u)] = (uint32_t)t;
+    for (int t = threadIdx.x; t < tiles; t += blockDim.x) {
+        const uint32_t c = cls(t);
+        const uint64_t m = peers_of(c);
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0u;
+        if (lane == leader) base = atomicAdd(&s_at[c], (uint32_t)__popcll(m));
+        base = __shfl(base, leader, 64);
+        const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        order[base + rank] = (uint32_t)t;
+    }
 }
 
 hipError_t launch_tile_order(const int32_t* tile_range, int32_t tiles, uint32_t* order, hipStream_t s) {
