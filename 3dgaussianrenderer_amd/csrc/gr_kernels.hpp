@@ -127,14 +127,16 @@ hipError_t launch_record_slots(const uint32_t* offsets, const uint32_t* total, i
 // 64 records (footprint sizes spread evenly over the waves), and writes each
 // record's emit record and its (depth word, record index | entry count) pair;
 // the pairs are sorted stably by depth word; a scan of the counts taken in
-// depth order gives every record its output offset; the emit pass walks the
-// records in index order again, reading their emit records and offsets
-// contiguously, and writes each record's entries at its offset.  Entries of
-// one tile therefore sit in (depth word, index) order -- the order the stable
-// sort of (tile << 32 | depth) keys emitted in index order produces -- and
-// only the tile bits remain to be sorted.  (Walking the emit in depth order,
-// with contiguous offsets and stores but gathered emit records, measured
-// slower: the nearest, largest footprints then crowd into the first waves.)
+// depth order writes every record's output offset contiguously (prefix[q] for
+// the q-th record in depth order) and the window table (the record holding
+// every kEmitWindow-th entry); the emit pass walks the list in windows of
+// kEmitWindow entries, one wave per window, reading the records' offsets and
+// packed words contiguously and their emit records by record index, and
+// writes the window's entries in order.  Entries of one tile therefore sit in
+// (depth word, index) order -- the order the stable sort of (tile << 32 |
+// depth) keys emitted in index order produces -- and only the tile bits remain
+// to be sorted.  (An index-order emit, with the offsets scattered back to
+// index order, measured slower: the scattered 4-byte offset stores.)
 //
 // Build record (32 bytes):
 //   [0] centre x, centre y, cos, sin     [1] extent x, extent y,

@@ -11,13 +11,14 @@
 //      candidates, rect) and the depth sort's pair (depth word, record index
 //      with the entry count packed above it)
 //   -> 32-bit onesweep sort of the pairs -> the counts, now in depth order,
-//      are scanned (K) and scattered back as per-record output offsets
-//   -> emit (records in index order): each record's entries, as 8-byte
-//      (Gaussian index << 32 | tile id) words, at its offset, + tile digit
+//      are scanned (K) into contiguous output offsets, with the window table
+//      (the record holding every 512th entry)
+//   -> emit (depth order, one wave per 512-entry window): the entries, as
+//      8-byte (Gaussian index << 32 | tile id) words, in order, + tile digit
 //      histograms
 //   -> keys-only onesweep sort of the tile bits (bits [0, tile bits)), whose
 //      last pass also writes the tile ranges
-//   -> raster -> image and K readback.
+//   -> tile order (heaviest first) -> raster -> image and K readback.
 // Entries sit in (depth word, Gaussian index) order before the last sort, so
 // the stable tile-bit sort yields exactly the order of the reference's
 // full-key sort of (tile << 32 | depth word) keys emitted in index order,
