@@ -1472,32 +1472,17 @@ __global__ __launch_bounds__(kRasterThreads) void raster_kernel(RasterArgs a) {
 // Raster order: the tiles by decreasing entry count, in classes of powers of two (order within a
 // class unspecified), so that the heaviest tiles start first and the lightest fill in at the end.
 // One workgroup; tiles without entries (range -1) come last.
+// (One lane per class and wave doing the LDS atomic, the wave's class peers found with 6 ballots,
+// measured 12.1 against 8.3 us: the one workgroup is bound by its instructions, not the atomics.)
 __global__ __launch_bounds__(1024) void tile_order_kernel(const int32_t* range, int32_t tiles, uint32_t* order) {
     __shared__ uint32_t s_n[33];
     __shared__ uint32_t s_at[33];
     if (threadIdx.x < 33) s_n[threadIdx.x] = 0u;
     __syncthreads();
-    const int lane = threadIdx.x & 63;
-    // the lanes of the wave in the same class as this one (6 ballots over the class bits), so that
-    // one lane per class and wave does the LDS atomic: most tiles fall into a few classes, whose
-    // counters would otherwise take one atomic per tile in turn
-    auto peers_of = [&](uint32_t c) {
-        uint64_t m = __ballot(1);
-#pragma unroll
-        for (int bit = 0; bit < 6; ++bit) {
-            const uint64_t v = __ballot((c >> bit) & 1u);
-            m &= ((c >> bit) & 1u) ? v : ~v;
-        }
-        return m;
-    };
     auto cls = [&](int t) {   // leading zeros of the entry count: 0 = most entries, 32 = none
-        return (uint32_t)__clz(max(range[2 * (size_t)t + 1] - range[2 * (size_t)t], 0));
+        return __clz(max(range[2 * (size_t)t + 1] - range[2 * (size_t)t], 0));
     };
-    for (int t = threadIdx.x; t < tiles; t += blockDim.x) {
-        const uint32_t c = cls(t);
-        const uint64_t m = peers_of(c);
-        if (lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&s_n[c], (uint32_t)__popcll(m));
-    }
+    for (int t = threadIdx.x; t < tiles; t += blockDim.x) atomicAdd(&s_n[cls(t)], 1u);
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t run = 0u;
@@ -1507,16 +1492,7 @@ __global__ __launch_bounds__(1024) void tile_order_kernel(const int32_t* range, 
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < tiles; t += blockDim.x) {
-        const uint32_t c = cls(t);
-        const uint64_t m = peers_of(c);
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t base = 0u;
-        if (lane == leader) base = atomicAdd(&s_at[c], (uint32_t)__popcll(m));
-        base = __shfl(base, leader, 64);
-        const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        order[base + rank] = (uint32_t)t;
-    }
+    for (int t = threadIdx.x; t < tiles; t += blockDim.x) order[atomicAdd(&s_at[cls(t)], 1u)] = (uint32_t)t;
 }
 
 hipError_t launch_tile_order(const int32_t* tile_range, int32_t tiles, uint32_t* order, hipStream_t s) {

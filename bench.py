@@ -303,7 +303,7 @@ def kernel_bytes(N, V, K, T, P, W, H):
 
 KERNEL_BOUND = {"record_count": "valu", "raster": "valu", "tile_sort_passes": "hbm (look-back latency)",
                 "depth_sort": "hbm (look-back latency)", "emit": "hbm (latency)",
-                "offsets": "hbm (three dependent launches)"}
+                "offsets": "hbm (two dependent launches)"}
 
 
 def kernel_table(frames, N, K, T, P, W, H):
