@@ -131,8 +131,9 @@ def adversarial_scene(count=3000, seed=5):
     """A 10k-style scene (camera at z = 3 looking at the origin) with degenerate Gaussians
     planted at its front: non-finite positions, zero / tiny / screen-covering / needle /
     overflowing scales, Gaussians behind the camera, in its plane and before the near plane,
-    opacity 0 and 1, the extreme quaternion bytes, saturating and NaN SH coefficients,
-    far-away and duplicate Gaussians."""
+    opacity 0 and 1, opacities NaN, infinite, negative and above 1 (after which the raster's
+    tiles take the full per-lane blend rule), the extreme quaternion bytes, saturating and NaN SH
+    coefficients, far-away and duplicate Gaussians."""
     s = gr.generate_scene(1, count, 3, seed)
     p, sr, sh = s["position"], s["scale_rot"], s["sh"]
     q = sr.view(np.uint32)
@@ -155,6 +156,10 @@ def adversarial_scene(count=3000, seed=5):
     sr[18, :3] = (1e20, 1.0, 1.0)
     p[19, :3] = p[20, :3] = (0.05, 0.05, 0.0)
     sr[19, :3] = sr[20, :3] = 0.05
+    for i, (op, x, y) in enumerate(((nan, 0.15, 0.0), (inf, -0.15, 0.05), (-0.5, 0.0, 0.15), (3.0, 0.05, -0.15))):
+        p[21 + i, :3] = (x, y, 0.2)
+        p[21 + i, 3] = op
+        sr[21 + i, :3] = 0.08
     return s
 
 
