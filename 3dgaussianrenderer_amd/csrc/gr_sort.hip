@@ -434,6 +434,23 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::
     if (t < nd) s_gbase[t] = digit_off[t] + lb_resolve(status, part, t, total) - dstart;
     __syncthreads();
     const int nvalid = (int)min((int64_t)kSortTile, (int64_t)n - base);
+    if (nvalid == kSortTile) {
+        // a whole partition: the lane's LDS reads are issued together (one round trip per step
+        // instead of two per entry)
+        KeyT kk[kSortItems];
+        uint32_t gp[kSortItems];
+#pragma unroll
+        for (int i = 0; i < kSortItems; ++i) kk[i] = s_keys[t + i * kSortThreads];
+#pragma unroll
+        for (int i = 0; i < kSortItems; ++i)
+            gp[i] = s_gbase[(uint32_t)(kk[i] >> shift) & dmask] + (uint32_t)(t + i * kSortThreads);
+#pragma unroll
+        for (int i = 0; i < kSortItems; ++i) {
+            kout[gp[i]] = kk[i];
+            if constexpr (PAIRS) vout[gp[i]] = s_vals[t + i * kSortThreads];
+        }
+        return;
+    }
     for (int j = t; j < nvalid; j += kSortThreads) {
         const KeyT k = s_keys[j];
         const uint32_t d = (uint32_t)(k >> shift) & dmask;
@@ -480,6 +497,12 @@ using KeysShape7 = KeysShape<512, GR_KSORT_ITEMS7, 7>;
 using KeysShape8 = KeysShape<512, GR_KSORT_ITEMS8, 8>;
 static_assert(KeysShape7::kTile >= kMinTile && KeysShape8::kTile >= kMinTile,
               "the look-back regions hold count / kMinTile partitions");
+
+// Entries per lane whose LDS reads the store of a whole partition issues together.
+#ifndef GR_KSORT_STORE_GROUP
+#define GR_KSORT_STORE_GROUP 3
+#endif
+constexpr int kStoreGroup = GR_KSORT_STORE_GROUP;
 
 template <class S>
 __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(4))) void onesweep_keys_kernel(
@@ -584,20 +607,49 @@ __global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(4))
         __syncthreads();
         if (haveY) finish(Y, totY, dsY);
         const int nvalid = (int)min((int64_t)TILE, (int64_t)n - (int64_t)X * TILE);
-        for (int j = t; j < nvalid; j += T) {
-            const uint64_t k = s_keys[j];
-            const uint32_t o = s_gbase[(uint32_t)(k >> shift) & dmask] + (uint32_t)j;
+        // entry j of the reordered partition goes to its digit's base + j; with ranges, equal tiles
+        // have equal digits, so a neighbour with the same tile is in the same run (neighbours'
+        // tiles read clamped to the partition)
+        const uint32_t* s_lo = reinterpret_cast<const uint32_t*>(s_keys);   // low words: the tiles
+        auto store = [&](int j, uint64_t k, uint32_t o, uint32_t tp, uint32_t tn) {
             kout[o] = k;
-            if (ranges) {
-                // equal tiles have equal digits: a neighbour with the same tile is in the same run
-                // (both neighbours read unconditionally, clamped to the partition)
-                const uint32_t tile = (uint32_t)k;
-                const uint32_t tp = (uint32_t)s_keys[j > 0 ? j - 1 : j];
-                const uint32_t tn = (uint32_t)s_keys[j + 1 < nvalid ? j + 1 : j];
-                if (tile < (uint32_t)ntiles) {
-                    if (j == 0 || tp != tile) atomicMin(reinterpret_cast<uint32_t*>(ranges) + 2 * (size_t)tile, o);
-                    if (j + 1 == nvalid || tn != tile) atomicMax(ranges + 2 * (size_t)tile + 1, (int32_t)(o + 1u));
+            const uint32_t tile = (uint32_t)k;
+            if (ranges && tile < (uint32_t)ntiles) {
+                if (j == 0 || tp != tile) atomicMin(reinterpret_cast<uint32_t*>(ranges) + 2 * (size_t)tile, o);
+                if (j + 1 == nvalid || tn != tile) atomicMax(ranges + 2 * (size_t)tile + 1, (int32_t)(o + 1u));
+            }
+        };
+        if (nvalid == TILE) {
+            // a whole partition: the LDS reads of kStoreGroup entries per lane are issued together
+            // (one LDS round trip per group instead of two per entry)
+#pragma unroll
+            for (int i0 = 0; i0 < IT; i0 += kStoreGroup) {
+                uint64_t kk[kStoreGroup];
+                uint32_t oo[kStoreGroup], tp[kStoreGroup], tn[kStoreGroup];
+#pragma unroll
+                for (int u = 0; u < kStoreGroup; ++u) {
+                    if (i0 + u < IT) {
+                        const int j = t + (i0 + u) * T;
+                        kk[u] = s_keys[j];
+                        if (ranges) {
+                            tp[u] = s_lo[2 * (j > 0 ? j - 1 : j)];
+                            tn[u] = s_lo[2 * (j + 1 < TILE ? j + 1 : j)];
+                        }
+                    }
                 }
+#pragma unroll
+                for (int u = 0; u < kStoreGroup; ++u)
+                    if (i0 + u < IT) oo[u] = s_gbase[(uint32_t)(kk[u] >> shift) & dmask] + (uint32_t)(t + (i0 + u) * T);
+#pragma unroll
+                for (int u = 0; u < kStoreGroup; ++u)
+                    if (i0 + u < IT) store(t + (i0 + u) * T, kk[u], oo[u], ranges ? tp[u] : 0u, ranges ? tn[u] : 0u);
+            }
+        } else {
+            for (int j = t; j < nvalid; j += T) {
+                const uint64_t k = s_keys[j];
+                const uint32_t o = s_gbase[(uint32_t)(k >> shift) & dmask] + (uint32_t)j;
+                store(j, k, o, ranges ? s_lo[2 * (j > 0 ? j - 1 : j)] : 0u,
+                      ranges ? s_lo[2 * (j + 1 < nvalid ? j + 1 : j)] : 0u);
             }
         }
         if (!haveY) return;
