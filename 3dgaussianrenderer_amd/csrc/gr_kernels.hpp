@@ -149,10 +149,14 @@ hipError_t launch_record_slots(const uint32_t* offsets, const uint32_t* total, i
 //     z     view * count + index
 //     w     local tile index of the rect's corner (y0 * tiles_x + x0, < 2^20)
 //           | (rect width - 1) << 20
-//   larger footprints (their candidates are tested again by the emit):
-//     x     staging slot of the build record, y exact entry count,
-//     z     view * count + index, w kEmitBig
+//   larger footprints:
+//     x     (w & kEmitSide) the word of BuildArgs::side from which the count pass
+//           listed the record's passing tiles (global tile ids, in entry order);
+//           otherwise the staging slot of the build record, whose candidates the
+//           emit then tests again
+//     y     exact entry count, z view * count + index, w kEmitBig [| kEmitSide]
 constexpr uint32_t kEmitBig = 1u << 31;
+constexpr uint32_t kEmitSide = 1u << 30;
 constexpr int kEmitCornerBits = 20;
 
 struct BuildArgs {
@@ -172,6 +176,11 @@ struct BuildArgs {
                                  // the count saturated at 2^(32 - index_bits) - 1, then the emit
                                  // record holds it)
     uint4* emit_records;         // count pass: emit record of record p
+    uint32_t* side;              // count pass, one view: the passing tiles of footprints over 64
+                                 // candidates, listed for the emit (NULL: the emit tests their
+                                 // candidates again)
+    int64_t side_words;          // capacity of side, cut into equal slices, one per count-pass wave;
+                                 // a record that does not fit its wave's slice is not listed
     int32_t index_bits;          // packed_index_bits(nviews * count)
     const uint32_t* sorted_words;   // emit pass: the packed record words in depth order
     const uint32_t* depth_prefix;   // emit pass: output offset of each depth-order position

@@ -329,6 +329,12 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     };
     uint32_t* vis_total = r->blocks.p + nb;   // visible Gaussians (records)
     R_HIP(stage(0));
+    // the second list buffer (capacity * 2 words) holds, until the tile sort's first pass, the
+    // emit's window table and after it the count pass's listed tiles of footprints over 64
+    // candidates
+    uint32_t* const list1 = reinterpret_cast<uint32_t*>(r->entries[1].p);
+    const int64_t side_at = (emit_window_words(capacity) + 63) / 64 * 64;
+    const int64_t side_words = std::min<int64_t>(2 * (int64_t)capacity - side_at, UINT32_MAX);
     R_HIP(mark("begin"));
     if (front_ev) R_HIP(hipEventRecord(r->front_ev[0], stream));
     {
@@ -376,6 +382,10 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     b.emit_records = r->emit_recs.p;
     b.index_bits = packed_index_bits(std::max(n, 1));
     b.entries = r->entries[0].p;
+    if (side_words > 0) {
+        b.side = list1 + side_at;
+        b.side_words = side_words;
+    }
     b.count = n;
     b.capacity = capacity;
     b.batches = batches;
@@ -400,7 +410,7 @@ gr_status gr_renderer_render(gr_renderer* r, const gr_camera_data* cam, int32_t 
     // which the tile sort's first pass overwrites only after the emit (capacity / kEmitWindow + 1 words
     // in capacity * 2 words).
     const uint32_t* sorted_words = reinterpret_cast<const uint32_t*>(r->dorder[dsel].p);
-    uint32_t* windows = reinterpret_cast<uint32_t*>(r->entries[1].p);
+    uint32_t* windows = list1;
     R_HIP(launch_build_scan_packed(sorted_words, n, vis_total, r->emit_recs.p, b.index_bits, k_total, k_total + 1,
                                    r->rec_prefix.p, windows, emit_window_words(capacity), stream));
     R_HIP(mark("offsets"));

@@ -509,6 +509,18 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
     const int total = __shfl(incl, 63, 64);
     const int excl = incl - cnt;
     s_excl[w][lane] = excl;
+    // footprints over 64 candidates take cnt words each of the wave's slice of a.side (slices of
+    // equal size, one per wave of the largest record count: no atomics); those that fit list
+    // their passing tiles there, in entry order
+    uint32_t side_at = ~0u;
+    if (a.side) {
+        const int64_t slice = a.side_words / (((int64_t)a.count + kBuildBatch - 1) / kBuildBatch);
+        const int want = cnt > 64 ? cnt : 0;
+        const int wexcl = wave_incl_scan(want, lane) - want;
+        if (want > 0 && (int64_t)wexcl + want <= slice)
+            side_at = (uint32_t)((int64_t)(first / kBuildBatch) * slice + wexcl);
+    }
+    const bool any_listed = __ballot(side_at != ~0u) != 0ull;   // wave-uniform
     wave_sync();
     // this lane's record: its entries and the mask of its passing candidates 0..63, gathered
     // from each step's ballot
@@ -518,11 +530,20 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
         const int cc = base + lane;
         bool pass = false;
         const int o = step_owner(s_mark[w], base, excl, cnt, lane);
+        // a listed record's list word and entries found so far, from its own lane (every lane
+        // active for the shuffles)
+        uint32_t o_side = ~0u, o_done = 0u;
+        if (any_listed) {
+            o_side = __shfl(side_at, o, 64);
+            o_done = __shfl(my_cnt, o, 64);
+        }
+        uint32_t tile = 0u;
         if (cc < total) {
             const int l = cc - s_excl[w][o];
             const int4 rc = s_rect[w][o];
             int q, r;
             divmod_small(l, rc.z, q, r);
+            if (any_listed) tile = (uint32_t)((rc.y + q) * a.v0.geom.tiles_x + rc.x + r);   // one view
             const float4 sf = s_fast[w][o];
             const EllipseRec& er = s_ell[w][o];
             FrameGeom og;
@@ -531,6 +552,12 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
             pass = tile_overlap_fast(rc.x + r, rc.y + q, er, fast_ellipse_from(er, sf.x, sf.y, sf.z), og);
         }
         const uint64_t m = __ballot(pass);
+        if (any_listed && pass && o_side != ~0u) {
+            // the record's lanes in this step start at lane max(0, its first candidate - base)
+            const int lo2 = max(0, s_excl[w][o] - base);
+            const uint64_t mo = m & lanemask_lt(lane) & ~lanemask_lt(lo2);
+            a.side[o_side + o_done + (uint32_t)__popcll(mo)] = tile;
+        }
         if (cnt > 0 && excl < base + 64 && excl + cnt > base) {
             const int st = max(excl, base);
             const uint64_t bits = (m >> (st - base)) & low_bits(min(excl + cnt, base + 64) - st);
@@ -543,8 +570,9 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
         const uint32_t esc = 0xffffffffu >> a.index_bits;
         a.depth_keys[pos] = dword;
         a.depth_words[pos] = (uint32_t)pos | (min(my_cnt, esc) << a.index_bits);
-        a.emit_records[pos] = cnt <= 64 ? make_uint4((uint32_t)my_mask, (uint32_t)(my_mask >> 32), pg, corner)
-                                        : make_uint4(slot, my_cnt, pg, kEmitBig);
+        a.emit_records[pos] = cnt <= 64      ? make_uint4((uint32_t)my_mask, (uint32_t)(my_mask >> 32), pg, corner)
+                              : side_at != ~0u ? make_uint4(side_at, my_cnt, pg, kEmitBig | kEmitSide)
+                                               : make_uint4(slot, my_cnt, pg, kEmitBig);
     }
 }
 
@@ -555,9 +583,9 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
 // contiguously, 64 positions at a time, fetches their emit records, and writes the window's
 // entries in order: window entry e belongs to the last record starting at or before it and is
 // that record's entry k = e - (its start).  Footprints of at most 64 candidates give entry k as
-// candidate select_bit(mask, k) (no tests, no footprint); larger ones fetch their build record,
-// run the overlap tests again and keep the passing candidates whose entry index falls in the
-// window.  Entries are (Gaussian index << 32 | tile id); their positions follow depth order, so
+// candidate select_bit(mask, k) (no tests, no footprint); larger ones whose passing tiles the
+// count pass listed read entry k from the list; the others fetch their build record, run the
+// overlap tests again and keep the passing candidates whose entry index falls in the window.  Entries are (Gaussian index << 32 | tile id); their positions follow depth order, so
 // the depth word of the key is carried by the position and the stable tile sort that follows
 // needs the tile bits alone.  Every window holds the same number of entries, so the work is
 // spread evenly whatever the footprint sizes, and the list is written front to back.
@@ -667,6 +695,10 @@ __global__ __launch_bounds__(kBuildThreads) __attribute__((amdgpu_waves_per_eu(G
                     s_rect[w][lane] = make_int4(vd.tile_base + corner, (int)(er.w >> kEmitCornerBits) + 1,
                                                 vd.geom.tiles_x, 0);
                     s_cand[w][lane] = 0;
+                } else if (er.w & kEmitSide) {
+                    // listed by the count pass: entry k is the tile at side[x + k]
+                    s_rect[w][lane] = make_int4(-1, 0, 0, (int)er.x);
+                    s_cand[w][lane] = 0;
                 } else if (hi > lo) {
                     const float4 r0 = a.records[2 * (size_t)er.x];
                     const float4 r1 = a.records[2 * (size_t)er.x + 1];
@@ -699,11 +731,17 @@ __global__ __launch_bounds__(kBuildThreads) __attribute__((amdgpu_waves_per_eu(G
                 const int o = step_owner(s_mark[w], base, lo, span, lane);
                 if (e >= first && e < last && s_cand[w][o] == 0) {
                     const int k = e - s_rel[w][o];
-                    const int l = select_bit(s_mask[w][o], k);
                     const int4 rc = s_rect[w][o];
-                    int qq, rr;
-                    divmod_small(l, rc.y, qq, rr);
-                    put(k0 + (uint32_t)e, (uint32_t)(rc.x + qq * rc.z + rr), o);
+                    uint32_t tile;
+                    if (rc.x < 0) {
+                        tile = a.side[(uint32_t)rc.w + (uint32_t)k];
+                    } else {
+                        const int l = select_bit(s_mask[w][o], k);
+                        int qq, rr;
+                        divmod_small(l, rc.y, qq, rr);
+                        tile = (uint32_t)(rc.x + qq * rc.z + rr);
+                    }
+                    put(k0 + (uint32_t)e, tile, o);
                 }
                 wave_sync();
             }
@@ -792,7 +830,8 @@ hipError_t launch_build_count(const BuildArgs& a, hipStream_t s) {
     if (a.records) {
         if (!a.record_slots || !a.record_total || !a.depth_keys || !a.depth_words || !a.emit_records ||
             a.index_bits < 1 || a.index_bits > 31 ||
-            ((int64_t)1 << a.index_bits) < (int64_t)a.nviews * a.count)
+            ((int64_t)1 << a.index_bits) < (int64_t)a.nviews * a.count ||
+            (a.side && (a.nviews != 1 || a.side_words <= 0 || a.side_words > (int64_t)UINT32_MAX)))
             return hipErrorInvalidValue;
         hipLaunchKernelGGL(record_count_kernel, build_grid(a), dim3(kBuildThreads), 0, s, a);
     } else {
