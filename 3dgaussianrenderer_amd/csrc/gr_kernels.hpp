@@ -179,8 +179,9 @@ struct BuildArgs {
     uint32_t* side;              // count pass, one view: the passing tiles of footprints over 64
                                  // candidates, listed for the emit (NULL: the emit tests their
                                  // candidates again)
-    int64_t side_words;          // capacity of side, cut into equal slices, one per count-pass wave;
-                                 // a record that does not fit its wave's slice is not listed
+    int64_t side_words;          // capacity of side, cut into equal slices, one per count-pass wave
+                                 // of the frame's records; a record that does not fit its wave's
+                                 // slice is not listed
     int32_t index_bits;          // packed_index_bits(nviews * count)
     const uint32_t* sorted_words;   // emit pass: the packed record words in depth order
     const uint32_t* depth_prefix;   // emit pass: output offset of each depth-order position

@@ -510,11 +510,11 @@ __global__ __launch_bounds__(kBuildThreads) void record_count_kernel(BuildArgs a
     const int excl = incl - cnt;
     s_excl[w][lane] = excl;
     // footprints over 64 candidates take cnt words each of the wave's slice of a.side (slices of
-    // equal size, one per wave of the largest record count: no atomics); those that fit list
-    // their passing tiles there, in entry order
+    // equal size, one per wave of the frame's records: no atomics); those that fit list their
+    // passing tiles there, in entry order
     uint32_t side_at = ~0u;
     if (a.side) {
-        const int64_t slice = a.side_words / (((int64_t)a.count + kBuildBatch - 1) / kBuildBatch);
+        const int64_t slice = a.side_words / (((int64_t)limit + kBuildBatch - 1) / kBuildBatch);
         const int want = cnt > 64 ? cnt : 0;
         const int wexcl = wave_incl_scan(want, lane) - want;
         if (want > 0 && (int64_t)wexcl + want <= slice)
