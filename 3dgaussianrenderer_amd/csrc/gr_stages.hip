@@ -857,6 +857,21 @@ hipError_t launch_build_emit(const BuildArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Non-temporal stores for the front end's outputs (colour, clip xy / z, conic, build records):
+// each is read once, by a later kernel, long after it is written (6M/1080p frame 852 -> 861
+// frames/s against plain stores, same box).
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_nt(float4* p, const float4& v) {
+    const nt_f4 n = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(n, reinterpret_cast<nt_f4*>(p));
+}
+__device__ __forceinline__ void st_nt(float2* p, const float2& v) {
+    const nt_f2 n = {v.x, v.y};
+    __builtin_nontemporal_store(n, reinterpret_cast<nt_f2*>(p));
+}
+__device__ __forceinline__ void st_nt(float* p, float v) { __builtin_nontemporal_store(v, p); }
+
 static unsigned grid_for(int64_t n) {
     int64_t b = (n + 255) / 256;
     return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
@@ -905,7 +920,7 @@ __global__ __launch_bounds__(kFrontThreads) __attribute__((amdgpu_waves_per_eu(G
             opacity = pd.w;
         }
         for (int v = 0; v < a.nviews; ++v)
-            a.color[(size_t)v * a.count + i] = sh_color<DEG>(c, pd, view_of(v, a.v0, a.views).cam.position, opacity);
+            st_nt(&a.color[(size_t)v * a.count + i], sh_color<DEG>(c, pd, view_of(v, a.v0, a.views).cam.position, opacity));
     }
     // projections; the record of view 0 stays in registers
     float4 ra0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb0 = ra0;
@@ -918,10 +933,10 @@ __global__ __launch_bounds__(kFrontThreads) __attribute__((amdgpu_waves_per_eu(G
             if (in) {
                 const size_t o = (size_t)v * a.count + i;
                 const ProjOut po = project_view(pd, sigma, vd);
-                a.clip_xy[o] = make_float2(po.cx, po.cy);
-                if (a.clip_z) a.clip_z[o] = po.cz;
-                if (a.ellipse) a.ellipse[o] = po.ellipse;
-                a.conic[o] = po.conic;
+                st_nt(&a.clip_xy[o], make_float2(po.cx, po.cy));
+                if (a.clip_z) st_nt(&a.clip_z[o], po.cz);
+                if (a.ellipse) st_nt(&a.ellipse[o], po.ellipse);
+                st_nt(&a.conic[o], po.conic);
                 vis = has_tiles(po, vd.geom);
                 if (v == 0) {
                     ra0 = make_float4(po.cx, po.cy, po.ellipse.x, po.ellipse.y);
@@ -958,8 +973,8 @@ __global__ __launch_bounds__(kFrontThreads) __attribute__((amdgpu_waves_per_eu(G
                 rb = make_float4(po.ellipse.z, po.ellipse.w, __uint_as_float(depth_word(po.cz)),
                                  __int_as_float(v * a.count + i));
             }
-            slice[2 * (size_t)r] = ra;
-            slice[2 * (size_t)r + 1] = rb;
+            st_nt(&slice[2 * (size_t)r], ra);
+            st_nt(&slice[2 * (size_t)r + 1], rb);
         }
         base += vtot;
     }
